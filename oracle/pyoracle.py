@@ -1,0 +1,135 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes binding of the CPU restatement of the serial chain (lib/liboracle.so)
+and of the reference's own stage A/B code built from the reference sources
+(_ref/libref.so).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "lib", "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+
+_DTYPES = {1: np.uint8, 2: np.uint16, 3: np.int16, 4: np.uint32, 5: np.int32, 6: np.float32, 7: np.float64}
+_libs = {}
+
+
+def _load(path, prefix):
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built (make -C oracle)")
+        lib = C.CDLL(path)
+        vp = C.c_void_p
+        getattr(lib, prefix + "_conditions_create").restype = vp
+        getattr(lib, prefix + "_conditions_create").argtypes = [vp] * 10 + [C.c_uint32]
+        getattr(lib, prefix + "_conditions_destroy").argtypes = [vp]
+        getattr(lib, prefix + "_process").restype = vp
+        getattr(lib, prefix + "_process").argtypes = [vp, vp, C.c_size_t, C.c_int]
+        getattr(lib, prefix + "_event_destroy").argtypes = [vp]
+        getattr(lib, prefix + "_field").restype = vp
+        getattr(lib, prefix + "_field").argtypes = [vp, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]
+        getattr(lib, prefix + "_scalar").restype = C.c_uint32
+        getattr(lib, prefix + "_scalar").argtypes = [vp, C.c_char_p]
+        getattr(lib, prefix + "_atan2s7").restype = C.c_int16
+        getattr(lib, prefix + "_atan2s7").argtypes = [C.c_float, C.c_float]
+        if prefix == "orc":
+            lib.orc_run_many.restype = C.c_uint32
+            lib.orc_run_many.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_uint32]
+        else:
+            for f in ("ref_divu52", "ref_localX", "ref_localY"):
+                getattr(lib, f).restype = C.c_uint16
+                getattr(lib, f).argtypes = [C.c_uint16]
+        _libs[path] = lib
+    return _libs[path]
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+class _Chain:
+    """Serial chain over one detector's conditions; subclasses pick the library."""
+
+    prefix = None
+    path = None
+
+    def __init__(self, conditions):
+        self.lib = _load(self.path, self.prefix)
+        self._keep = {k: np.ascontiguousarray(v) for k, v in conditions.items() if isinstance(v, np.ndarray)}
+        p = lambda k: self._keep[k].ctypes.data_as(C.c_void_p)  # noqa: E731
+        fed = self._keep["fed_ids"]
+        self._h = getattr(self.lib, self.prefix + "_conditions_create")(
+            p("cabling"), p("mod_to_unp"), p("gain"), p("gain_data"), p("cpe_common"), p("cpe_det"),
+            p("cpe_layer"), p("cpe_average"), p("beamspot"), p("fed_ids"), len(fed))
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            getattr(self.lib, self.prefix + "_conditions_destroy")(h)
+
+    def process(self, raw, upto=3):
+        """Runs the chain on one raw record; returns a dict of numpy arrays and scalars."""
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        e = getattr(self.lib, self.prefix + "_process")(self._h, raw.ctypes.data_as(C.c_void_p), raw.size, upto)
+        if not e:
+            raise RuntimeError("raw record rejected")
+        try:
+            out = {}
+            for name in FIELDS:
+                n, dt = C.c_size_t(), C.c_int()
+                ptr = getattr(self.lib, self.prefix + "_field")(e, name.encode(), C.byref(n), C.byref(dt))
+                if not dt.value:
+                    continue
+                dtype = _DTYPES[dt.value]
+                count = n.value
+                if count == 0:
+                    out[name] = np.zeros(0, dtype=dtype)
+                else:
+                    buf = (C.c_uint8 * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+                    out[name] = np.frombuffer(bytes(buf), dtype=dtype).copy()
+            for s in SCALARS:
+                out[s] = int(getattr(self.lib, self.prefix + "_scalar")(e, s.encode()))
+            return out
+        finally:
+            getattr(self.lib, self.prefix + "_event_destroy")(e)
+
+    def atan2s7(self, y, x):
+        return int(getattr(self.lib, self.prefix + "_atan2s7")(y, x))
+
+
+FIELDS = ["words", "fedIds", "xx", "yy", "adc", "moduleInd", "clus", "pdigi", "rawIdArr", "errors",
+          "moduleStart", "clusInModule", "moduleId", "clusModuleStart", "xl", "yl", "xerr", "yerr", "xg", "yg",
+          "zg", "rg", "iphi", "charge", "xsize", "ysize", "detInd", "hitsLayerStart", "phiOff", "phiBins",
+          "avgCorrected"]
+SCALARS = ["nWords", "nModules", "nDigis", "nClusters", "nHits", "fedCounter"]
+
+
+class Oracle(_Chain):
+    prefix = "orc"
+    path = ORACLE_LIB
+
+    def run_many(self, raws, upto=3, threads=1, repeat=1):
+        """Processes the raw records `repeat` times over `threads` workers; returns events done."""
+        blob = np.concatenate(raws).astype(np.uint8)
+        offs = np.zeros(len(raws) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([len(r) for r in raws])
+        return int(self.lib.orc_run_many(self._h, blob.ctypes.data_as(C.c_void_p), offs.ctypes.data_as(C.c_void_p),
+                                         len(raws), upto, threads, repeat))
+
+
+class Reference(_Chain):
+    prefix = "ref"
+    path = REF_LIB
+
+    def divu52(self, n):
+        return int(self.lib.ref_divu52(n))
+
+    def localX(self, n):
+        return int(self.lib.ref_localX(n))
+
+    def localY(self, n):
+        return int(self.lib.ref_localY(n))

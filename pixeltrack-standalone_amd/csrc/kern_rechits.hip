@@ -1,0 +1,357 @@
+// Stage B: clusters -> rec-hits, layer offsets and the per-layer phi histogram.
+//
+//   hitsKernel      one workgroup per (module run, event): cluster extents and
+//                   integer charges accumulated with LDS atomics (exact in any
+//                   order), then one cluster per lane for the cluster position
+//                   estimate and the local -> global transform.  The float
+//                   arithmetic keeps the serial program's operation order and
+//                   the library is compiled without FMA contraction, so the
+//                   positions are bit-identical to the serial ones.
+//   phiBinnerKernel one workgroup per event, one wavefront per layer: layer
+//                   offsets, per-layer counts of the 128 phi bins, a workgroup
+//                   scan, then the fill.  The serial fill decrements the bin
+//                   counters, which lists every bin in decreasing hit index;
+//                   each wavefront reproduces that order deterministically by
+//                   ranking equal bins inside a 64-hit chunk with 7 ballots.
+//
+// Reference behaviour: plugin-SiPixelRecHits/gpuPixelRecHits.h:16-207,
+// CondFormats/pixelCPEforGPU.h:106-340, DataFormats/SOARotation.h:55-94,
+// DataFormats/approx_atan2.h:230-263, PixelRecHits.cc:15-48,
+// CUDACore/HistoContainer.h:18-73.
+#include "pxgpu_internal.h"
+
+namespace pxg {
+
+  namespace {
+    constexpr int kBlock = 256;
+    constexpr int kChunk = 256;  // clusters per pass
+
+    __device__ __forceinline__ uint32_t div52(uint32_t n) { return n / 52; }
+    __device__ __forceinline__ bool bigX(uint32_t px) { return px == 79 || px == 80; }
+    __device__ __forceinline__ bool bigY(uint32_t py) {
+      const uint32_t ly = py - 52 * div52(py);
+      return ly == 0 || ly == 51;
+    }
+    __device__ __forceinline__ uint32_t unitX(uint32_t px) { return px + (px > 79 ? 1 : 0) + (px > 80 ? 1 : 0); }
+    __device__ __forceinline__ uint32_t unitY(uint32_t py) {
+      const uint32_t roc = div52(py);
+      return py + 2 * roc + ((py - 52 * roc) > 0 ? 1 : 0);
+    }
+
+    __device__ __forceinline__ float edgeCorrection(int sizeM1, int qf, int ql, uint32_t upperFirst, uint32_t lowerLast,
+                                                    float lorentz, float thickness, float cotAngle, float pitch,
+                                                    bool firstBig, bool lastBig) {
+      if (sizeM1 == 0)
+        return 0.f;
+      float weff = 0.f;
+      bool simple = true;
+      if (sizeM1 == 1) {
+        const float winner = pitch * float(int(lowerLast) - int(upperFirst));
+        const float wpred = thickness * cotAngle - lorentz;
+        weff = fabsf(wpred) - winner;
+        simple = (weff < 0.0f) | (weff > pitch);
+      }
+      if (simple) {
+        float edges = 2.0f;
+        if (firstBig)
+          edges += 1.0f;
+        if (lastBig)
+          edges += 1.0f;
+        weff = pitch * 0.5f * edges;
+      }
+      const float qdiff = float(ql - qf);
+      float qsum = float(ql + qf);
+      if (qsum == 0.f)
+        qsum = 1.0f;
+      return 0.5f * (qdiff / qsum) * weff;
+    }
+
+    __device__ __forceinline__ int16_t atan2s7(float y, float x) {
+      float r = (fabsf(x) - fabsf(y)) / (fabsf(x) + fabsf(y));
+      if (x < 0)
+        r = -r;
+      const float z = r * r;
+      const float poly =
+          r * ((-10422.177734375f) + z * (3349.97412109375f + z * ((-1525.589599609375f) + z * 406.64190673828125f)));
+      int angle = (x >= 0) ? 8192 : 24576;
+      angle = int16_t(angle + int(int16_t(poly)));
+      return int16_t(y < 0 ? -angle : angle);
+    }
+  }  // namespace
+
+  __global__ void __launch_bounds__(kBlock) hitsKernel(DevConditions c, Pool p) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t e = blockIdx.y;
+    const EventDesc& ev = p.events[e];
+    const BeamSpotPOD bs = ev.bs;
+    const uint32_t nModules = p.counts[e].nModules;
+
+    if (k == 0) {  // average geometry corrected for the beam spot
+      const AverageGeometry& ag = *c.cpeAverage;
+      AverageGeometry& agc = p.avg[e];
+      for (uint32_t il = threadIdx.x; il < kLaddersInBarrel; il += kBlock) {
+        const float lx = ag.ladderX[il] - bs.x, ly = ag.ladderY[il] - bs.y;
+        agc.ladderZ[il] = ag.ladderZ[il] - bs.z;
+        agc.ladderX[il] = lx;
+        agc.ladderY[il] = ly;
+        agc.ladderR[il] = sqrtf(lx * lx + ly * ly);
+        agc.ladderMinZ[il] = ag.ladderMinZ[il] - bs.z;
+        agc.ladderMaxZ[il] = ag.ladderMaxZ[il] - bs.z;
+      }
+      if (threadIdx.x == 0) {
+        agc.endCapZ[0] = ag.endCapZ[0] - bs.z;
+        agc.endCapZ[1] = ag.endCapZ[1] - bs.z;
+      }
+    }
+    if (k >= nModules)
+      return;
+
+    const size_t mo = size_t(e) * kMaxModuleRanges;
+    const uint32_t me = p.moduleId[mo + k];
+    if (me >= kNumModules)
+      return;
+    const uint32_t nclus = p.clusInModule[mo + me];
+    if (nclus == 0)
+      return;
+    const uint32_t first = p.moduleStart[mo + 1 + k];
+    const uint32_t end = (k + 1 < nModules) ? p.moduleStart[mo + 2 + k] : ev.nWords;
+    const uint32_t wo = ev.wordOffset;
+    const uint32_t hitBase = p.clusModuleStart[mo + me];
+    const size_t ho = size_t(e) * kMaxNumClusters;
+    const CPEDetParams& det = c.cpeDet[me];
+    const CPECommonParams com = *c.cpeCommon;
+
+    __shared__ uint32_t minRow[kChunk], maxRow[kChunk], minCol[kChunk], maxCol[kChunk];
+    __shared__ int32_t qsum[kChunk], qfx[kChunk], qlx[kChunk], qfy[kChunk], qly[kChunk];
+
+    for (uint32_t start = 0; start < nclus; start += kChunk) {
+      const uint32_t nIn = min(uint32_t(kChunk), nclus - start);
+      for (uint32_t ic = threadIdx.x; ic < nIn; ic += kBlock) {
+        minRow[ic] = minCol[ic] = 0xffffffffu;
+        maxRow[ic] = maxCol[ic] = 0;
+        qsum[ic] = qfx[ic] = qlx[ic] = qfy[ic] = qly[ic] = 0;
+      }
+      __syncthreads();
+      for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t g = wo + i;
+        if (p.moduleInd[g] != me)
+          continue;  // invalid (or dropped) pixel
+        const int32_t cl = p.clus[g];
+        if (cl < int32_t(start) || cl >= int32_t(start + nIn))
+          continue;
+        const uint32_t j = uint32_t(cl) - start;
+        const uint32_t x = p.xx[g], y = p.yy[g];
+        atomicMin(&minRow[j], x);
+        atomicMax(&maxRow[j], x);
+        atomicMin(&minCol[j], y);
+        atomicMax(&maxCol[j], y);
+      }
+      __syncthreads();
+      for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t g = wo + i;
+        if (p.moduleInd[g] != me)
+          continue;
+        const int32_t cl = p.clus[g];
+        if (cl < int32_t(start) || cl >= int32_t(start + nIn))
+          continue;
+        const uint32_t j = uint32_t(cl) - start;
+        const uint32_t x = p.xx[g], y = p.yy[g];
+        const int32_t ch = p.adc[g];  // the 65535 clamp of the reference is a no-op on uint16
+        atomicAdd(&qsum[j], ch);
+        if (minRow[j] == x)
+          atomicAdd(&qfx[j], ch);
+        if (maxRow[j] == x)
+          atomicAdd(&qlx[j], ch);
+        if (minCol[j] == y)
+          atomicAdd(&qfy[j], ch);
+        if (maxCol[j] == y)
+          atomicAdd(&qly[j], ch);
+      }
+      __syncthreads();
+      for (uint32_t ic = threadIdx.x; ic < nIn; ic += kBlock) {
+        const uint32_t h = hitBase + start + ic;
+        if (h >= kMaxNumClusters)
+          break;
+        const uint32_t r0 = minRow[ic], r1 = maxRow[ic], c0 = minCol[ic], c1 = maxCol[ic];
+        const int32_t fx = qfx[ic], lx = qlx[ic], fy = qfy[ic], ly = qly[ic];
+        // position (pixelCPEforGPU.h:176-259)
+        const uint32_t llxl = unitX(r0 + 1), llyl = unitY(c0 + 1), urxl = unitX(r1), uryl = unitY(c1);
+        const int mx = int(llxl + urxl), my = int(llyl + uryl);
+        int xsize = int(urxl) + 2 - int(llxl);
+        int ysize = int(uryl) + 2 - int(llyl);
+        if (bigX(r0))
+          ++xsize;
+        if (bigX(r1))
+          ++xsize;
+        if (bigY(c0))
+          ++ysize;
+        if (bigY(c1))
+          ++ysize;
+        const int unbX = int(8. * double(fabsf(float(fx - lx))) / double(float(fx + lx)));
+        const int unbY = int(8. * double(fabsf(float(fy - ly))) / double(float(fy + ly)));
+        xsize = 8 * xsize - unbX;
+        ysize = 8 * ysize - unbY;
+        int16_t sx = int16_t(min(xsize, 1023));
+        int16_t sy = int16_t(min(ysize, 1023));
+        if (r0 == 0 || r1 == kLastRow)
+          sx = int16_t(-sx);
+        if (c0 == 0 || c1 == kLastCol)
+          sy = int16_t(-sy);
+        const float xPos = det.shiftX + com.thePitchX * (0.5f * float(mx) + float(kXOffset));
+        const float yPos = det.shiftY + com.thePitchY * (0.5f * float(my) + float(kYOffset));
+        const float gvz = -1.f / det.z0;
+        const float cotalpha = (xPos - det.x0) * gvz;
+        const float cotbeta = (yPos - det.y0) * gvz;
+        const float thickness = det.isBarrel ? com.theThicknessB : com.theThicknessE;
+        const float xcorr = edgeCorrection(int(r1) - int(r0), fx, lx, llxl, urxl, det.chargeWidthX, thickness, cotalpha,
+                                           com.thePitchX, bigX(r0), bigX(r1));
+        const float ycorr = edgeCorrection(int(c1) - int(c0), fy, ly, llyl, uryl, det.chargeWidthY, thickness, cotbeta,
+                                           com.thePitchY, bigY(c0), bigY(c1));
+        const float xl = xPos + xcorr;
+        const float yl = yPos + ycorr;
+        // errors (pixelCPEforGPU.h:316-340)
+        const uint32_t dsx = r1 - r0, dsy = c1 - c0;
+        float xerr = 0.0050f, yerr = 0.0085f;
+        const uint32_t ix = (dsx == 0) + ((dsx == 0) && bigX(r0));
+        const uint32_t iy = (dsy == 0) + ((dsy == 0) && bigY(c0));
+        if (!(r0 == 0 || r1 == kLastRow))
+          xerr = det.sx[ix];
+        if (!(c0 == 0 || c1 == kLastCol))
+          yerr = det.sy[iy];
+        // local -> global, beam spot subtracted
+        const SOAFrameF& f = det.frame;
+        float xg = f.R11 * xl + f.R21 * yl;
+        float yg = f.R12 * xl + f.R22 * yl;
+        float zg = f.R13 * xl + f.R23 * yl;
+        xg += f.px;
+        yg += f.py;
+        zg += f.pz;
+        xg -= bs.x;
+        yg -= bs.y;
+        zg -= bs.z;
+        const size_t o = ho + h;
+        p.charge[o] = qsum[ic];
+        p.detInd[o] = uint16_t(me);
+        p.xl[o] = xl;
+        p.yl[o] = yl;
+        p.xsize[o] = sx;
+        p.ysize[o] = sy;
+        p.xerr[o] = xerr * xerr;
+        p.yerr[o] = yerr * yerr;
+        p.xg[o] = xg;
+        p.yg[o] = yg;
+        p.zg[o] = zg;
+        p.rg[o] = sqrtf(xg * xg + yg * yg);
+        p.iphi[o] = atan2s7(yg, xg);
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---------------------------------------------------------------------------
+  namespace {
+    constexpr int kPhiWaves = kNumLayers;  // one wavefront per layer
+  }
+
+  __global__ void __launch_bounds__(64 * kPhiWaves) phiBinnerKernel(DevConditions c, Pool p) {
+    const uint32_t e = blockIdx.x;
+    const uint32_t nHits = p.counts[e].nClusters;
+    const uint32_t* cms = p.clusModuleStart + size_t(e) * kMaxModuleRanges;
+    uint32_t* hls = p.hitsLayerStart + size_t(e) * kLayerStartStride;
+    uint32_t* off = p.phiOff + size_t(e) * kPhiOffStride;
+    uint16_t* bins = p.phiBins + size_t(e) * kMaxNumClusters;
+    const int16_t* iphi = p.iphi + size_t(e) * kMaxNumClusters;
+
+    __shared__ uint32_t ls[kNumLayers + 1];
+    __shared__ uint32_t cnt[kPhiHistTotBins + 1];
+    __shared__ uint32_t wsum[kPhiWaves];
+    if (threadIdx.x <= kNumLayers)
+      ls[threadIdx.x] = nHits ? cms[c.cpeLayer->layerStart[threadIdx.x]] : 0;
+    for (uint32_t i = threadIdx.x; i <= kPhiHistTotBins; i += blockDim.x)
+      cnt[i] = 0;
+    __syncthreads();
+    if (threadIdx.x <= kNumLayers)
+      hls[threadIdx.x] = ls[threadIdx.x];
+    if (nHits == 0) {
+      for (uint32_t i = threadIdx.x; i < kPhiHistTotBins; i += blockDim.x)
+        off[i] = 0;
+      return;
+    }
+    const uint32_t lane = threadIdx.x & 63, layer = threadIdx.x >> 6;
+    const uint32_t h0 = ls[layer], h1 = ls[layer + 1];
+    for (uint32_t i = h0 + lane; i < h1; i += 64)
+      atomicAdd(&cnt[layer * kPhiBins + ((int(iphi[i]) >> 9) & 127)], 1u);
+    __syncthreads();
+    // inclusive scan of the 1281 counters: 640 threads x 2 consecutive bins (+ the last)
+    {
+      const uint32_t b0 = 2 * threadIdx.x;
+      const uint32_t v0 = cnt[b0], v1 = cnt[b0 + 1];
+      const uint32_t s = v0 + v1;
+      uint32_t incl = s;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o);
+        if (int(lane) >= o)
+          incl += t;
+      }
+      if (lane == 63)
+        wsum[layer] = incl;
+      __syncthreads();
+      uint32_t before = 0;
+      for (uint32_t w = 0; w < layer; ++w)
+        before += wsum[w];
+      const uint32_t excl = before + incl - s;
+      __syncthreads();
+      cnt[b0] = excl + v0;       // inclusive: end of bin b0
+      cnt[b0 + 1] = excl + s;    // end of bin b0 + 1
+      // bin 1280 has no entries: its end is the total
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      cnt[kPhiHistTotBins - 1] = cnt[kPhiHistTotBins - 2];
+    __syncthreads();
+    // per wave: fill its layer in 64-hit chunks, ranking equal bins by lane
+    __shared__ uint32_t taken[kPhiHistTotBins];
+    for (uint32_t b = lane; b < kPhiBins; b += 64)
+      taken[layer * kPhiBins + b] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t c0 = h0; c0 < h1; c0 += 64) {
+      const uint32_t i = c0 + lane;
+      const bool act = i < h1;
+      const uint32_t key = act ? ((int(iphi[i]) >> 9) & 127) : 0;
+      uint64_t peers = __ballot(act);
+#pragma unroll
+      for (int bit = 0; bit < 7; ++bit) {
+        const bool set = (key >> bit) & 1;
+        const uint64_t m = __ballot(set);
+        peers &= set ? m : ~m;
+      }
+      const uint32_t rank = __popcll(peers & ((1ull << lane) - 1ull));
+      const uint32_t group = __popcll(peers);
+      const uint32_t bin = layer * kPhiBins + key;
+      uint32_t before = 0;
+      if (act)
+        before = taken[bin];
+      __builtin_amdgcn_wave_barrier();
+      if (act) {
+        bins[cnt[bin] - 1 - (before + rank)] = uint16_t(i);
+        if (rank == group - 1)
+          taken[bin] = before + group;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // final offsets = bin starts (what the serial decrementing fill leaves)
+    for (uint32_t b = threadIdx.x; b < kPhiHistTotBins; b += blockDim.x)
+      off[b] = b == 0 ? 0 : cnt[b - 1];
+  }
+
+  void launchHits(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
+    hitsKernel<<<dim3(kMaxNumModules, nEvents), kBlock, 0, s>>>(c, p);
+  }
+
+  void launchPhiBinner(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
+    phiBinnerKernel<<<nEvents, 64 * kPhiWaves, 0, s>>>(c, p);
+  }
+
+}  // namespace pxg

@@ -1,0 +1,516 @@
+// C ABI of libpxgpu.so (include/pxgpu.h): context, per-stream device pools,
+// launch sequences and product views.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pxgpu.h"
+#include "pxgpu_internal.h"
+
+using namespace pxg;
+
+static_assert(sizeof(pxg_beamspot) == sizeof(BeamSpotPOD), "beam spot layout");
+
+namespace {
+  thread_local std::string g_lastError;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_lastError = buf;
+    return code;
+  }
+
+#define PXG_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t err_ = (call);                                                                  \
+    if (err_ != hipSuccess)                                                                    \
+      return fail(PXG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(err_), __FILE__, __LINE__); \
+  } while (0)
+
+  template <typename T>
+  hipError_t devAlloc(T*& p, size_t n, std::vector<void*>& owned) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) + 256);
+    if (e == hipSuccess) {
+      owned.push_back(q);
+      p = static_cast<T*>(q);
+    }
+    return e;
+  }
+
+  uint32_t alignUp(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
+}  // namespace
+
+struct pxg_context {
+  int device = 0;
+  DevConditions cond{};
+  std::vector<void*> owned;
+};
+
+struct ProfileMark {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct pxg_stream {
+  pxg_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  Pool pool{};
+  std::vector<void*> owned;
+  EventDesc* hostEvents = nullptr;    // pinned
+  EventCounts* hostCounts = nullptr;  // pinned
+  uint32_t nLoaded = 0;
+  uint32_t wordCapacity = 0;
+  bool profiling = false;
+  std::vector<ProfileMark> marks;
+  std::map<std::string, std::pair<uint32_t, double>> profile;
+};
+
+namespace {
+  // per-event arrays of the pool shifted to event `first`
+  Pool viewFrom(const Pool& p, uint32_t first) {
+    Pool v = p;
+    v.events = p.events + first;
+    v.counts = p.counts + first;
+    const size_t m = size_t(first) * kMaxModuleRanges;
+    v.moduleStart = p.moduleStart + m;
+    v.clusInModule = p.clusInModule + m;
+    v.moduleId = p.moduleId + m;
+    v.clusModuleStart = p.clusModuleStart + m;
+    const size_t h = size_t(first) * kMaxNumClusters;
+    v.xl = p.xl + h;
+    v.yl = p.yl + h;
+    v.xerr = p.xerr + h;
+    v.yerr = p.yerr + h;
+    v.xg = p.xg + h;
+    v.yg = p.yg + h;
+    v.zg = p.zg + h;
+    v.rg = p.rg + h;
+    v.iphi = p.iphi + h;
+    v.charge = p.charge + h;
+    v.xsize = p.xsize + h;
+    v.ysize = p.ysize + h;
+    v.detInd = p.detInd + h;
+    v.phiBins = p.phiBins + h;
+    v.hitsLayerStart = p.hitsLayerStart + size_t(first) * kLayerStartStride;
+    v.phiOff = p.phiOff + size_t(first) * kPhiOffStride;
+    v.avg = p.avg + first;
+    return v;
+  }
+
+  template <typename F>
+  int launchTimed(pxg_stream* s, const char* name, F&& f) {
+    if (!s->profiling) {
+      f();
+    } else {
+      ProfileMark m{name, nullptr, nullptr};
+      PXG_HIP(hipEventCreate(&m.start));
+      PXG_HIP(hipEventCreate(&m.stop));
+      PXG_HIP(hipEventRecord(m.start, s->stream));
+      f();
+      PXG_HIP(hipEventRecord(m.stop, s->stream));
+      s->marks.push_back(m);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return fail(PXG_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+    return 0;
+  }
+
+  int runStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto) {
+    if (count == 0 || first + count > s->nLoaded)
+      return fail(PXG_ERR_ARG, "events [%u, %u) not loaded (%u resident)", first, first + count, s->nLoaded);
+    const Pool v = viewFrom(s->pool, first);
+    const DevConditions& c = s->ctx->cond;
+    const uint32_t blockBase = s->hostEvents[first].blockOffset;
+    const EventDesc& last = s->hostEvents[first + count - 1];
+    const uint32_t nBlocks = last.blockOffset + (last.nWords + kDecodeBlock - 1) / kDecodeBlock - blockBase;
+    int rc = 0;
+    if (from <= 1 && upto >= 1) {
+      if ((rc = launchTimed(s, "decode", [&] { launchDecode(c, v, count, nBlocks, blockBase, s->stream); })))
+        return rc;
+      if ((rc = launchTimed(s, "moduleStarts", [&] { launchModuleStarts(v, count, s->stream); })))
+        return rc;
+    }
+    if (from <= 2 && upto >= 2) {
+      if ((rc = launchTimed(s, "clusterize", [&] { launchClusterize(v, count, s->stream); })))
+        return rc;
+    }
+    if (from <= 3 && upto >= 3) {
+      if ((rc = launchTimed(s, "hits", [&] { launchHits(c, v, count, s->stream); })))
+        return rc;
+      if ((rc = launchTimed(s, "phiBinner", [&] { launchPhiBinner(c, v, count, s->stream); })))
+        return rc;
+    }
+    return 0;
+  }
+
+  void fillProducts(pxg_stream* s, uint32_t e, pxg_digis* d, pxg_clusters* cl, pxg_errors* er, pxg_hits* h) {
+    const Pool& p = s->pool;
+    const EventDesc& ev = s->hostEvents[e];
+    const EventCounts& cnt = s->hostCounts[e];
+    const size_t w = ev.wordOffset;
+    const size_t m = size_t(e) * kMaxModuleRanges;
+    const size_t ho = size_t(e) * kMaxNumClusters;
+    if (d) {
+      *d = {p.xx + w, p.yy + w, p.adc + w, p.moduleInd + w, p.clus + w, p.pdigi + w, p.rawIdArr + w, cnt.nModules,
+            ev.nWords};
+    }
+    if (cl)
+      *cl = {p.moduleStart + m, p.clusInModule + m, p.moduleId + m, p.clusModuleStart + m, cnt.nClusters};
+    if (er)
+      *er = {p.errors + w, cnt.nErrors};
+    if (h) {
+      *h = {p.xl + ho,
+            p.yl + ho,
+            p.xerr + ho,
+            p.yerr + ho,
+            p.xg + ho,
+            p.yg + ho,
+            p.zg + ho,
+            p.rg + ho,
+            p.iphi + ho,
+            p.charge + ho,
+            p.xsize + ho,
+            p.ysize + ho,
+            p.detInd + ho,
+            p.clusModuleStart + m,
+            p.hitsLayerStart + size_t(e) * kLayerStartStride,
+            p.phiOff + size_t(e) * kPhiOffStride,
+            p.phiBins + ho,
+            reinterpret_cast<float*>(p.avg + e),
+            cnt.nClusters};
+    }
+  }
+}  // namespace
+
+extern "C" {
+
+const char* pxg_last_error(void) { return g_lastError.c_str(); }
+
+int pxg_device_name(char* buf, size_t n) {
+  int dev = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  PXG_HIP(hipGetDeviceProperties(&prop, dev));
+  std::snprintf(buf, n, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return 0;
+}
+
+int pxg_init(const pxg_conditions* c, pxg_context** out) {
+  if (!c || !out || !c->cabling || !c->mod_to_unp || !c->gain || !c->gain_data || !c->cpe_common || !c->cpe_det ||
+      !c->cpe_layer || !c->cpe_average)
+    return fail(PXG_ERR_ARG, "pxg_init: missing conditions");
+  if (c->n_det != kNumModules)
+    return fail(PXG_ERR_ARG, "pxg_init: %u CPE det params, expected %u", c->n_det, kNumModules);
+  auto* ctx = new pxg_context;
+  auto up = [&](const void* src, size_t bytes, void*& dst) -> hipError_t {
+    hipError_t e = hipMalloc(&dst, bytes);
+    if (e != hipSuccess)
+      return e;
+    ctx->owned.push_back(dst);
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  void *cab, *m2u, *gain, *gdata, *com, *det, *lay, *avg;
+  hipError_t e = hipGetDevice(&ctx->device);
+  if (e == hipSuccess)
+    e = up(c->cabling, sizeof(SiPixelFedCablingMapGPU), cab);
+  if (e == hipSuccess)
+    e = up(c->mod_to_unp, kCablingSize, m2u);
+  if (e == hipSuccess)
+    e = up(c->gain, sizeof(SiPixelGainForHLTonGPU), gain);
+  if (e == hipSuccess)
+    e = up(c->gain_data, c->gain_data_bytes, gdata);
+  if (e == hipSuccess)
+    e = up(c->cpe_common, sizeof(CPECommonParams), com);
+  if (e == hipSuccess)
+    e = up(c->cpe_det, sizeof(CPEDetParams) * c->n_det, det);
+  if (e == hipSuccess)
+    e = up(c->cpe_layer, sizeof(CPELayerGeometry), lay);
+  if (e == hipSuccess)
+    e = up(c->cpe_average, sizeof(AverageGeometry), avg);
+  if (e != hipSuccess) {
+    pxg_finalize(ctx);
+    return fail(PXG_ERR_HIP, "pxg_init: %s", hipGetErrorString(e));
+  }
+  ctx->cond = {static_cast<const SiPixelFedCablingMapGPU*>(cab), static_cast<const uint8_t*>(m2u),
+               static_cast<const SiPixelGainForHLTonGPU*>(gain), static_cast<const GainDecoding*>(gdata),
+               c->gain_data_bytes,
+               static_cast<const CPECommonParams*>(com),      static_cast<const CPEDetParams*>(det),
+               static_cast<const CPELayerGeometry*>(lay),     static_cast<const AverageGeometry*>(avg)};
+  *out = ctx;
+  return 0;
+}
+
+void pxg_finalize(pxg_context* ctx) {
+  if (!ctx)
+    return;
+  for (void* p : ctx->owned)
+    (void)hipFree(p);
+  delete ctx;
+}
+
+int pxg_stream_create(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, pxg_stream** out) {
+  if (!ctx || !out || maxEvents == 0 || maxWords == 0)
+    return fail(PXG_ERR_ARG, "pxg_stream_create: bad arguments");
+  auto* s = new pxg_stream;
+  s->ctx = ctx;
+  const uint32_t W = alignUp(maxWords, kWordAlign) + kWordAlign * maxEvents;
+  s->wordCapacity = W;
+  Pool& p = s->pool;
+  p.maxEvents = maxEvents;
+  p.maxWords = W;
+  const size_t E = maxEvents, H = size_t(maxEvents) * kMaxNumClusters, M = size_t(maxEvents) * kMaxModuleRanges;
+  auto& o = s->owned;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+#define ALLOC(ptr, n) \
+  if (e == hipSuccess) \
+  e = devAlloc(ptr, n, o)
+  ALLOC(p.words, W);
+  ALLOC(p.fedIds, W / 2);
+  ALLOC(p.events, E);
+  ALLOC(p.counts, E);
+  ALLOC(p.xx, W);
+  ALLOC(p.yy, W);
+  ALLOC(p.adc, W);
+  ALLOC(p.moduleInd, W);
+  ALLOC(p.clus, W);
+  ALLOC(p.pdigi, W);
+  ALLOC(p.rawIdArr, W);
+  ALLOC(p.boundaryMask, 2 * (W / 64));
+  ALLOC(p.errorsSparse, W);
+  ALLOC(p.errors, W);
+  ALLOC(p.moduleStart, M);
+  ALLOC(p.clusInModule, M);
+  ALLOC(p.moduleId, M);
+  ALLOC(p.clusModuleStart, M);
+  ALLOC(p.xl, H);
+  ALLOC(p.yl, H);
+  ALLOC(p.xerr, H);
+  ALLOC(p.yerr, H);
+  ALLOC(p.xg, H);
+  ALLOC(p.yg, H);
+  ALLOC(p.zg, H);
+  ALLOC(p.rg, H);
+  ALLOC(p.iphi, H);
+  ALLOC(p.charge, H);
+  ALLOC(p.xsize, H);
+  ALLOC(p.ysize, H);
+  ALLOC(p.detInd, H);
+  ALLOC(p.hitsLayerStart, E * kLayerStartStride);
+  ALLOC(p.phiOff, E * kPhiOffStride);
+  ALLOC(p.phiBins, H);
+  ALLOC(p.avg, E);
+#undef ALLOC
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->hostEvents), E * sizeof(EventDesc), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->hostCounts), E * sizeof(EventCounts), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(p.counts, 0, E * sizeof(EventCounts), s->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    pxg_stream_destroy(s);
+    return fail(PXG_ERR_HIP, "pxg_stream_create: %s", hipGetErrorString(e));
+  }
+  std::memset(s->hostCounts, 0, E * sizeof(EventCounts));
+  *out = s;
+  return 0;
+}
+
+void pxg_stream_destroy(pxg_stream* s) {
+  if (!s)
+    return;
+  if (s->stream)
+    (void)hipStreamSynchronize(s->stream);
+  for (auto& m : s->marks) {
+    (void)hipEventDestroy(m.start);
+    (void)hipEventDestroy(m.stop);
+  }
+  for (void* p : s->owned)
+    (void)hipFree(p);
+  if (s->hostEvents)
+    (void)hipHostFree(s->hostEvents);
+  if (s->hostCounts)
+    (void)hipHostFree(s->hostCounts);
+  if (s->stream)
+    (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int pxg_sync(pxg_stream* s) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_sync: null stream");
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs) {
+  if (!s || !raws || !bs || n == 0)
+    return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
+  if (n > s->pool.maxEvents)
+    return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
+  uint32_t off = 0, blocks = 0;
+  for (uint32_t e = 0; e < n; ++e) {
+    if (raws[e].nWords % 2)
+      return fail(PXG_ERR_ARG, "event %u: odd number of words %u", e, raws[e].nWords);
+    if (raws[e].nWords > kMaxFedWords)
+      return fail(PXG_ERR_CAPACITY, "event %u: %u words > %u", e, raws[e].nWords, kMaxFedWords);
+    if (uint64_t(off) + raws[e].nWords > s->wordCapacity)
+      return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u words exceed the pool", off + raws[e].nWords);
+    EventDesc& d = s->hostEvents[e];
+    d.wordOffset = off;
+    d.nWords = raws[e].nWords;
+    d.blockOffset = blocks;
+    d.pad = 0;
+    std::memcpy(&d.bs, bs, sizeof(BeamSpotPOD));
+    if (raws[e].nWords) {
+      PXG_HIP(hipMemcpyAsync(s->pool.words + off, raws[e].words, size_t(raws[e].nWords) * 4, hipMemcpyHostToDevice,
+                             s->stream));
+      PXG_HIP(hipMemcpyAsync(s->pool.fedIds + off / 2, raws[e].fedIds, raws[e].nWords / 2, hipMemcpyHostToDevice,
+                             s->stream));
+    }
+    blocks += (raws[e].nWords + kDecodeBlock - 1) / kDecodeBlock;
+    off = alignUp(off + raws[e].nWords, kWordAlign);
+  }
+  PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, n * sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  s->nLoaded = n;
+  return 0;
+}
+
+int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
+  if (!s || upto < 1 || upto > 3)
+    return fail(PXG_ERR_ARG, "pxg_run: bad arguments");
+  return runStages(s, first, count, 1, upto);
+}
+
+int pxg_fetch_counts(pxg_stream* s, uint32_t first, uint32_t count) {
+  if (!s || first + count > s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_fetch_counts: bad range");
+  PXG_HIP(hipMemcpyAsync(s->hostCounts + first, s->pool.counts + first, count * sizeof(EventCounts),
+                         hipMemcpyDeviceToHost, s->stream));
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  uint32_t st = 0;
+  for (uint32_t e = first; e < first + count; ++e)
+    st |= s->hostCounts[e].status;
+  return int(st);
+}
+
+int pxg_event_products(pxg_stream* s, uint32_t e, pxg_digis* d, pxg_clusters* c, pxg_errors* er, pxg_hits* h) {
+  if (!s || e >= s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_event_products: event %u not resident", e);
+  fillProducts(s, e, d, c, er, h);
+  return 0;
+}
+
+int pxg_copy_to_host(pxg_stream* s, void* dst, const void* src, size_t bytes) {
+  if (!s || (!dst && bytes) || (!src && bytes))
+    return fail(PXG_ERR_ARG, "pxg_copy_to_host: bad arguments");
+  if (bytes) {
+    PXG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+    PXG_HIP(hipStreamSynchronize(s->stream));
+  }
+  return 0;
+}
+
+// ---- plain per-event entry points ---------------------------------------------------
+static int plainEvent(pxg_stream* s, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords, int upto) {
+  if (!s || (nWords && (!words || !fedIds)))
+    return fail(PXG_ERR_ARG, "null input");
+  pxg_raw_view v{words, fedIds, nWords};
+  pxg_beamspot bs{};
+  int rc = pxg_load_events(s, 1, &v, &bs);
+  if (rc)
+    return rc;
+  rc = runStages(s, 0, 1, 1, upto);
+  if (rc)
+    return rc;
+  return pxg_fetch_counts(s, 0, 1);
+}
+
+int pxg_raw_to_digi(pxg_stream* s, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords, pxg_digis* d,
+                    pxg_errors* er) {
+  int rc = plainEvent(s, words, fedIds, nWords, 1);
+  if (rc < 0)
+    return rc;
+  fillProducts(s, 0, d, nullptr, er, nullptr);
+  return rc;
+}
+
+int pxg_raw_to_cluster(pxg_stream* s, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords, pxg_digis* d,
+                       pxg_clusters* c, pxg_errors* er) {
+  int rc = plainEvent(s, words, fedIds, nWords, 2);
+  if (rc < 0)
+    return rc;
+  fillProducts(s, 0, d, c, er, nullptr);
+  return rc;
+}
+
+int pxg_rechits(pxg_stream* s, const pxg_digis* d, const pxg_clusters* c, const pxg_beamspot* bs, pxg_hits* h) {
+  if (!s || !d || !c || !bs || !h)
+    return fail(PXG_ERR_ARG, "pxg_rechits: null argument");
+  if (s->nLoaded < 1 || d->xx != s->pool.xx + s->hostEvents[0].wordOffset || c->moduleStart != s->pool.moduleStart)
+    return fail(PXG_ERR_ARG, "pxg_rechits: digis/clusters are not the last products of this stream");
+  std::memcpy(&s->hostEvents[0].bs, bs, sizeof(BeamSpotPOD));
+  PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
+  int rc = runStages(s, 0, 1, 3, 3);
+  if (rc)
+    return rc;
+  rc = pxg_fetch_counts(s, 0, 1);
+  if (rc < 0)
+    return rc;
+  fillProducts(s, 0, nullptr, nullptr, nullptr, h);
+  return rc;
+}
+
+// ---- profiling ------------------------------------------------------------------------
+int pxg_profile_enable(pxg_stream* s, int on) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "null stream");
+  s->profiling = on != 0;
+  return 0;
+}
+
+int pxg_profile_read(pxg_stream* s, const char** names, uint32_t* calls, double* totalMs, int max) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "null stream");
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  for (auto& m : s->marks) {
+    float ms = 0;
+    PXG_HIP(hipEventElapsedTime(&ms, m.start, m.stop));
+    auto& a = s->profile[m.name];
+    a.first += 1;
+    a.second += ms;
+    (void)hipEventDestroy(m.start);
+    (void)hipEventDestroy(m.stop);
+  }
+  s->marks.clear();
+  int i = 0;
+  static thread_local std::vector<std::string> keep;
+  keep.clear();
+  for (auto& [name, v] : s->profile)
+    keep.push_back(name);
+  for (auto& [name, v] : s->profile) {
+    if (i < max) {
+      names[i] = keep[i].c_str();
+      calls[i] = v.first;
+      totalMs[i] = v.second;
+    }
+    ++i;
+  }
+  s->profile.clear();
+  return i;
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+"""ctypes binding of libpxgpu.so (include/pxgpu.h) and of the host FED loop.
+
+No fallback: if the HIP library is not built, or no GPU is present, these
+calls raise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import load_library
+
+vp = C.c_void_p
+u32 = C.c_uint32
+
+
+class Conditions(C.Structure):
+    _fields_ = [("cabling", vp), ("mod_to_unp", vp), ("gain", vp), ("gain_data", vp), ("gain_data_bytes", u32),
+                ("cpe_common", vp), ("cpe_det", vp), ("n_det", u32), ("cpe_layer", vp), ("cpe_average", vp)]
+
+
+class BeamSpot(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x", "y", "z", "sigmaZ", "beamWidthX", "beamWidthY", "dxdz", "dydz",
+                                         "emittanceX", "emittanceY", "betaStar")]
+
+
+class Digis(C.Structure):
+    _fields_ = [("xx", vp), ("yy", vp), ("adc", vp), ("moduleInd", vp), ("clus", vp), ("pdigi", vp),
+                ("rawIdArr", vp), ("nModules", u32), ("nDigis", u32)]
+
+
+class Clusters(C.Structure):
+    _fields_ = [("moduleStart", vp), ("clusInModule", vp), ("moduleId", vp), ("clusModuleStart", vp),
+                ("nClusters", u32)]
+
+
+class Errors(C.Structure):
+    _fields_ = [("data", vp), ("size", u32)]
+
+
+class Hits(C.Structure):
+    _fields_ = [(n, vp) for n in ("xl", "yl", "xerr", "yerr", "xg", "yg", "zg", "rg", "iphi", "charge", "xsize",
+                                  "ysize", "detInd", "hitsModuleStart", "hitsLayerStart", "phiOff", "phiBins",
+                                  "averageGeometry")] + [("nHits", u32)]
+
+
+class RawView(C.Structure):
+    _fields_ = [("words", vp), ("fedIds", vp), ("nWords", u32)]
+
+
+DIGI_FIELDS = {"xx": np.uint16, "yy": np.uint16, "adc": np.uint16, "moduleInd": np.uint16, "clus": np.int32,
+               "pdigi": np.uint32, "rawIdArr": np.uint32}
+CLUSTER_FIELDS = {"moduleStart": 2001, "clusInModule": 2000, "moduleId": 2000, "clusModuleStart": 2001}
+HIT_FIELDS = {"xl": np.float32, "yl": np.float32, "xerr": np.float32, "yerr": np.float32, "xg": np.float32,
+              "yg": np.float32, "zg": np.float32, "rg": np.float32, "iphi": np.int16, "charge": np.int32,
+              "xsize": np.int16, "ysize": np.int16, "detInd": np.uint16}
+
+
+def _gpu():
+    lib = load_library("libpxgpu.so")
+    if not getattr(lib, "_typed", False):
+        lib.pxg_last_error.restype = C.c_char_p
+        lib.pxg_init.argtypes = [C.POINTER(Conditions), C.POINTER(vp)]
+        lib.pxg_finalize.argtypes = [vp]
+        lib.pxg_device_name.argtypes = [C.c_char_p, C.c_size_t]
+        lib.pxg_stream_create.argtypes = [vp, u32, u32, C.POINTER(vp)]
+        lib.pxg_stream_destroy.argtypes = [vp]
+        lib.pxg_sync.argtypes = [vp]
+        lib.pxg_raw_to_digi.argtypes = [vp, vp, vp, u32, C.POINTER(Digis), C.POINTER(Errors)]
+        lib.pxg_raw_to_cluster.argtypes = [vp, vp, vp, u32, C.POINTER(Digis), C.POINTER(Clusters),
+                                           C.POINTER(Errors)]
+        lib.pxg_rechits.argtypes = [vp, C.POINTER(Digis), C.POINTER(Clusters), C.POINTER(BeamSpot), C.POINTER(Hits)]
+        lib.pxg_load_events.argtypes = [vp, u32, C.POINTER(RawView), C.POINTER(BeamSpot)]
+        lib.pxg_run.argtypes = [vp, u32, u32, C.c_int]
+        lib.pxg_fetch_counts.argtypes = [vp, u32, u32]
+        lib.pxg_event_products.argtypes = [vp, u32, C.POINTER(Digis), C.POINTER(Clusters), C.POINTER(Errors),
+                                           C.POINTER(Hits)]
+        lib.pxg_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+        lib.pxg_profile_enable.argtypes = [vp, C.c_int]
+        lib.pxg_profile_read.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(u32), C.POINTER(C.c_double), C.c_int]
+        lib._typed = True
+    return lib
+
+
+def _host():
+    lib = load_library("libpxhost.so")
+    if not getattr(lib, "_typed", False):
+        lib.pxh_unpack_raw_record.argtypes = [vp, C.c_size_t, vp, u32, vp, vp, u32, C.POINTER(u32)]
+        lib._typed = True
+    return lib
+
+
+def unpack_raw(raw, fed_ids, capacity=300000):
+    """Host FED loop of the raw-to-cluster producer: raw record -> (words, fedIds)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    fed_ids = np.ascontiguousarray(fed_ids, dtype=np.uint32)
+    words = np.zeros(capacity, dtype=np.uint32)
+    fed = np.zeros(capacity // 2, dtype=np.uint8)
+    n = u32()
+    rc = _host().pxh_unpack_raw_record(raw.ctypes.data, raw.size, fed_ids.ctypes.data, fed_ids.size, words.ctypes.data,
+                                       fed.ctypes.data, capacity, C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"FED unpacking failed ({rc})")
+    return words[: n.value].copy(), fed[: n.value // 2].copy()
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise RuntimeError(f"{what}: {_gpu().pxg_last_error().decode()}")
+    return rc
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    _check(_gpu().pxg_device_name(buf, 256), "pxg_device_name")
+    return buf.value.decode()
+
+
+class Context:
+    """Device context with the conditions uploaded once."""
+
+    def __init__(self, cond):
+        self.lib = _gpu()
+        self._keep = cond
+        c = Conditions()
+        c.cabling = cond["cabling"].ctypes.data
+        c.mod_to_unp = cond["mod_to_unp"].ctypes.data
+        c.gain = cond["gain"].ctypes.data
+        c.gain_data = cond["gain_data"].ctypes.data
+        c.gain_data_bytes = cond["gain_data"].size
+        c.cpe_common = cond["cpe_common"].ctypes.data
+        c.cpe_det = cond["cpe_det"].ctypes.data
+        c.n_det = cond["n_det"]
+        c.cpe_layer = cond["cpe_layer"].ctypes.data
+        c.cpe_average = cond["cpe_average"].ctypes.data
+        self._h = vp()
+        _check(self.lib.pxg_init(C.byref(c), C.byref(self._h)), "pxg_init")
+        self.fed_ids = cond["fed_ids"]
+        bs = np.frombuffer(cond["beamspot"].tobytes(), dtype=np.float32)
+        self.beamspot = BeamSpot(*[float(v) for v in bs])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.pxg_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def stream(self, max_events=1, max_words=300000):
+        return Stream(self, max_events, max_words)
+
+
+class Stream:
+    """One HIP stream with its device-resident product pool."""
+
+    def __init__(self, ctx, max_events, max_words):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._h = vp()
+        _check(self.lib.pxg_stream_create(ctx._h, max_events, max_words, C.byref(self._h)), "pxg_stream_create")
+        self._inputs = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.pxg_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- device -> host copies of products
+    def _copy(self, ptr, dtype, n):
+        out = np.zeros(n, dtype=dtype)
+        if n:
+            _check(self.lib.pxg_copy_to_host(self._h, out.ctypes.data, ptr, out.nbytes), "pxg_copy_to_host")
+        return out
+
+    def digis_to_host(self, d):
+        out = {k: self._copy(getattr(d, k), t, d.nDigis) for k, t in DIGI_FIELDS.items()}
+        out["nModules"], out["nDigis"] = d.nModules, d.nDigis
+        return out
+
+    def clusters_to_host(self, c):
+        out = {k: self._copy(getattr(c, k), np.uint32, n) for k, n in CLUSTER_FIELDS.items()}
+        out["nClusters"] = c.nClusters
+        return out
+
+    def errors_to_host(self, e):
+        return self._copy(e.data, np.uint8, 12 * e.size)
+
+    def hits_to_host(self, h):
+        out = {k: self._copy(getattr(h, k), t, h.nHits) for k, t in HIT_FIELDS.items()}
+        out["hitsLayerStart"] = self._copy(h.hitsLayerStart, np.uint32, 11)
+        out["phiOff"] = self._copy(h.phiOff, np.uint32, 1281)
+        out["phiBins"] = self._copy(h.phiBins, np.uint16, h.nHits)
+        out["avgCorrected"] = self._copy(h.averageGeometry, np.float32, 6 * 148 + 2)
+        out["nHits"] = h.nHits
+        return out
+
+    # -- plain per-event path
+    def raw_to_cluster(self, words, fed_ids):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        fed_ids = np.ascontiguousarray(fed_ids, dtype=np.uint8)
+        d, c, e = Digis(), Clusters(), Errors()
+        st = _check(self.lib.pxg_raw_to_cluster(self._h, words.ctypes.data, fed_ids.ctypes.data, words.size,
+                                                C.byref(d), C.byref(c), C.byref(e)), "pxg_raw_to_cluster")
+        return st, d, c, e
+
+    def raw_to_digi(self, words, fed_ids):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        fed_ids = np.ascontiguousarray(fed_ids, dtype=np.uint8)
+        d, e = Digis(), Errors()
+        st = _check(self.lib.pxg_raw_to_digi(self._h, words.ctypes.data, fed_ids.ctypes.data, words.size,
+                                             C.byref(d), C.byref(e)), "pxg_raw_to_digi")
+        return st, d, e
+
+    def rechits(self, d, c, beamspot=None):
+        h = Hits()
+        bs = beamspot if beamspot is not None else self.ctx.beamspot
+        st = _check(self.lib.pxg_rechits(self._h, C.byref(d), C.byref(c), C.byref(bs), C.byref(h)), "pxg_rechits")
+        return st, h
+
+    def process(self, words, fed_ids, upto=3):
+        """Plain path on one event; returns a dict of host arrays named like the oracle's fields."""
+        if upto == 1:
+            st, d, e = self.raw_to_digi(words, fed_ids)
+            out = self.digis_to_host(d)
+        else:
+            st, d, c, e = self.raw_to_cluster(words, fed_ids)
+            out = self.digis_to_host(d)
+            out.update(self.clusters_to_host(c))
+            if upto >= 3:
+                st2, h = self.rechits(d, c)
+                st |= st2
+                out.update(self.hits_to_host(h))
+        out["errors"] = self.errors_to_host(e)
+        out["status"] = st
+        return out
+
+    # -- resident / batched path
+    def load(self, events, beamspot=None):
+        """events: list of (words, fedIds) host arrays; copies them into the device pool."""
+        self._inputs = [(np.ascontiguousarray(w, dtype=np.uint32), np.ascontiguousarray(f, dtype=np.uint8))
+                        for w, f in events]
+        views = (RawView * len(self._inputs))()
+        for i, (w, f) in enumerate(self._inputs):
+            views[i].words, views[i].fedIds, views[i].nWords = w.ctypes.data, f.ctypes.data, w.size
+        bs = beamspot if beamspot is not None else self.ctx.beamspot
+        _check(self.lib.pxg_load_events(self._h, len(self._inputs), views, C.byref(bs)), "pxg_load_events")
+
+    def run(self, first, count, upto=3):
+        _check(self.lib.pxg_run(self._h, first, count, upto), "pxg_run")
+
+    def fetch_counts(self, first, count):
+        return _check(self.lib.pxg_fetch_counts(self._h, first, count), "pxg_fetch_counts")
+
+    def event_products(self, e, upto=3):
+        d, c, er, h = Digis(), Clusters(), Errors(), Hits()
+        _check(self.lib.pxg_event_products(self._h, e, C.byref(d), C.byref(c), C.byref(er), C.byref(h)),
+               "pxg_event_products")
+        out = self.digis_to_host(d)
+        if upto >= 2:
+            out.update(self.clusters_to_host(c))
+        if upto >= 3:
+            out.update(self.hits_to_host(h))
+        out["errors"] = self.errors_to_host(er)
+        return out
+
+    def sync(self):
+        _check(self.lib.pxg_sync(self._h), "pxg_sync")
+
+    def profile(self, on=True):
+        _check(self.lib.pxg_profile_enable(self._h, 1 if on else 0), "pxg_profile_enable")
+
+    def profile_read(self):
+        n = 32
+        names, calls, ms = (C.c_char_p * n)(), (u32 * n)(), (C.c_double * n)()
+        k = _check(self.lib.pxg_profile_read(self._h, names, calls, ms, n), "pxg_profile_read")
+        return {names[i].decode(): (calls[i], ms[i]) for i in range(min(k, n))}

@@ -1,0 +1,62 @@
+"""Pins the oracle's stage A/B restatement to the reference's own stage A/B
+code (compiled from the reference sources into oracle/_ref by oracle/Makefile),
+bit-exact on every product field, on seeded events including error words,
+dead/noisy columns, bad ROCs and PU200 occupancy."""
+import numpy as np
+import pytest
+
+import pyoracle
+from conftest import assert_same, event
+
+FIELDS = [f for f in pyoracle.FIELDS]
+
+CASES = [("serial100", i, {}) for i in range(4)] + [
+    ("serial100", 4, {"error_rate": 0.7}),
+    ("pu50", 0, {}),
+    ("pu50", 5, {"error_rate": 0.1}),
+    ("pu20_80", 2, {}),
+    ("pu200", 1, {}),
+]
+
+
+@pytest.mark.parametrize("preset,index,kw", CASES)
+def test_oracle_matches_reference(detector, oracle, reference, preset, index, kw):
+    raw, _ = event(detector, preset, index, **kw)
+    a = oracle.process(raw)
+    b = reference.process(raw)
+    for s in pyoracle.SCALARS:
+        assert a[s] == b[s], f"{s}: {a[s]} != {b[s]}"
+    assert_same(a, b, [f for f in FIELDS if f in a and f in b], f"{preset}[{index}]")
+    assert set(FIELDS) <= set(a) and set(FIELDS) <= set(b)
+
+
+def test_error_records_present(detector, oracle):
+    """Error injection really exercises the error path (types 25-31, 37, 40)."""
+    raw, truth = event(detector, "serial100", 4, error_rate=0.7)
+    a = oracle.process(raw)
+    recs = a["errors"].reshape(-1, 12)
+    types = set(recs[:, 8].tolist())
+    assert len(recs) > 0 and truth.n_error_words > 0
+    assert {26, 27, 28, 29, 31, 37} <= types, types
+    assert types & {30, 40}
+
+
+def test_divu52_is_division(reference):
+    """The reference's shift/multiply divu52 equals n/52 for every column index used."""
+    for n in range(0, 512):
+        assert reference.divu52(n) == n // 52, n
+
+
+def test_atan2s7_matches_reference_and_bound(oracle, reference):
+    """16-bit atan2: oracle == reference bit for bit; within 2 units of the exact angle
+    (approx_atan2.h documents an absolute accuracy of 2 for degree 7)."""
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-20, 20, size=(4000, 2)).astype(np.float32)
+    pts = np.concatenate([pts, [[1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, -1], [3, -3], [-2, 2]]]).astype(np.float32)
+    for y, x in pts:
+        a = oracle.atan2s7(float(y), float(x))
+        b = reference.atan2s7(float(y), float(x))
+        assert a == b, (y, x, a, b)
+        exact = np.arctan2(np.float64(y), np.float64(x)) * 32768.0 / np.pi
+        d = (a - exact + 32768) % 65536 - 32768
+        assert abs(d) <= 2.5, (y, x, a, exact)
