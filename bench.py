@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X pixel reconstruction chain on PU50-like events.
+
+Workload: 1000 synthetic PU50 events (50 vertices x Poisson(35) tracks, 1/pT^2
+spectrum above 0.5 GeV, ~50k digis and ~13k clusters per event), words
+resident in HBM before the timed region.  One step = one pass of the chain
+over all events.  The batched path runs B events per launch of each stage;
+the plain path runs one launch sequence per event with the reference's
+per-event host synchronisation.  Prints one JSON line.
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "pixeltrack-standalone_amd"))
+
+METRIC = "events/s at PU50-like load; p50/p99 latency, batched vs plain per-event GPU"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per kernel launch (unique input read + output written with the
+# reference element widths), from the per-event counters W (words), D (digis),
+# H (hits), M (module runs).  decode: words 4 B + fedIds 0.5 B per word in, seven
+# digi arrays (20 B) out, ~2 B of gain table per digi.  clusterize: x, y, adc,
+# moduleInd in (8 B), clus + moduleInd out (6 B).  hits: moduleInd, clus, x, y,
+# adc in (12 B per digi), 44 B per hit out.  moduleStarts: 16 B of ballot masks
+# per 64 words in, 4 B per module run out.  phiBinner: iphi in, bin contents out
+# (4 B per hit) + 5 kB of offsets.
+def algorithmic_bytes(kernel, W, D, H, M):
+    return {
+        "decode": 24.5 * W + 2 * D,
+        "moduleStarts": W / 64 * 16 + 4 * M,
+        "clusterize": 14 * D,
+        "hits": 12 * D + 44 * H,
+        "phiBinner": 4 * H + 5124,
+    }[kernel]
+
+
+def gen_events(detector, cfg, n, threads):
+    from pxtrack import gen  # noqa: F401
+    with cf.ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda i: detector.event(cfg, i)[0], range(n)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--events", type=int, default=1000)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: each box here has one MI355X and events shard across GPUs "
+              f"without any exchange, so --gpus {args.gpus} is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    from pxtrack import gen, gpu
+    import torch
+
+    t0 = time.time()
+    det = gen.Detector()
+    cond = det.conditions()
+    cfg = gen.preset("pu50")
+    threads = min(16, os.cpu_count() or 1)
+    raws = gen_events(det, cfg, args.events, threads)
+    inputs = [gpu.unpack_raw(r, cond["fed_ids"]) for r in raws]
+    W = np.array([len(w) for w, _ in inputs], dtype=np.int64)
+    print(f"[bench] generated {len(raws)} PU50 events in {time.time() - t0:.1f}s, mean words {W.mean():.0f}",
+          file=sys.stderr, flush=True)
+
+    ctx = gpu.Context(cond)
+    stream = ctx.stream(max_events=len(inputs), max_words=int(W.sum()))
+    stream.load(inputs)
+    n = len(inputs)
+    B = max(1, min(args.batch, n))
+
+    def batched_pass():
+        for first in range(0, n, B):
+            stream.run(first, min(B, n - first), 3)
+        return stream.fetch_counts(0, n)
+
+    # ---- batched path: warmup, then K timed steps (one step = all events)
+    for _ in range(args.warmup):
+        batched_pass()
+    torch.cuda.synchronize()
+    stream.profile(True)
+    stream.profile_read()
+    torch.cuda.synchronize()
+    stream.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        status = batched_pass()
+    stream.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    prof = stream.profile_read()
+    stream.profile(False)
+    ms_per_step = 1e3 * elapsed / args.steps
+    evps = n * args.steps / elapsed
+
+    # per-event counters for the algorithmic bytes
+    D = W.astype(np.float64)
+    cnts = np.array([stream.event_counts(e) for e in range(n)], dtype=np.float64)
+    Mmean, Hmean = float(cnts[:, 0].mean()), float(cnts[:, 1].mean())
+    launches_per_kernel = {k: v[0] for k, v in prof.items()}
+    kern = max(prof, key=lambda k: prof[k][1])
+    calls, total_ms = prof[kern]
+    avg_s = total_ms / 1e3 / calls
+    events_per_launch = n * args.steps / calls
+    alg = algorithmic_bytes(kern, W.mean(), D.mean(), Hmean, Mmean) * events_per_launch
+    achieved = alg / avg_s / 1e9
+    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in prof.items()}
+
+    # ---- plain per-event path: one launch sequence + host sync per event
+    plain = None
+    if not args.no_plain:
+        lat = []
+        for e in range(min(n, 50)):  # warmup
+            stream.run(e, 1, 3)
+            stream.fetch_counts(e, 1)
+        t1 = time.perf_counter()
+        for e in range(n):
+            a = time.perf_counter()
+            stream.run(e, 1, 3)
+            stream.fetch_counts(e, 1)
+            lat.append(time.perf_counter() - a)
+        tp = time.perf_counter() - t1
+        lat = np.array(lat) * 1e3
+        plain = {"events_per_s": round(n / tp, 1), "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
+                 "latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
+                 "latency_ms_p999": round(float(np.percentile(lat, 99.9)), 4)}
+
+    # ---- CPU baseline: the oracle (serial restatement) on the host cores, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle
+        orc = pyoracle.Oracle(cond)
+        sample = raws[:100]
+        ncores = min(16, os.cpu_count() or 1)
+        a = time.perf_counter()
+        done = orc.run_many(sample, upto=3, threads=ncores, repeat=1)
+        one = time.perf_counter() - a
+        rep = max(1, int(args.cpu_seconds / max(one, 1e-3)))
+        a = time.perf_counter()
+        done = orc.run_many(sample, upto=3, threads=ncores, repeat=rep)
+        ct = time.perf_counter() - a
+        cpu = {"value": round(done / ct, 1), "unit": "events/s", "cores": ncores, "kind": "port",
+               "sample": f"{len(sample)} of the PU50 events x {rep} passes, raw->hits, one event per thread "
+                         f"at a time ({ncores} threads), {ct:.1f} s"}
+
+    batch_lat_ms = ms_per_step / ((n + B - 1) // B)
+    out = {
+        "metric": METRIC,
+        "value": round(evps, 1),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32/f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n} PU50 events (stages raw->digis->clusters->rec-hits), batched path, "
+                               f"{B} events per launch",
+                   "events": n, "batch": B, "mean_words": round(float(W.mean()), 1),
+                   "mean_clusters": round(Hmean, 1), "stages": "A+B"},
+        "batched": {"events_per_s": round(evps, 1), "batch_latency_ms": round(batch_lat_ms, 4),
+                    "status": status},
+        "plain": plain,
+        "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
+        "kernel_ms_per_launch": kernel_ms,
+        "launches": launches_per_kernel,
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes_per_launch": round(alg), "avg_launch_ms": round(avg_s * 1e3, 4)},
+        "cpu_baseline": cpu,
+        "device": gpu.device_name(),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,10 @@ int pxg_fetch_counts(pxg_stream* s, uint32_t first, uint32_t count);
 int pxg_event_products(pxg_stream* s, uint32_t e, pxg_digis* digis, pxg_clusters* clusters, pxg_errors* errors,
                        pxg_hits* hits);
 
+/* Host copies of resident event e's counters (after pxg_fetch_counts covering e):
+ * out[0] module runs, out[1] clusters (= hits), out[2] error records, out[3] status bits. */
+int pxg_event_counts(pxg_stream* s, uint32_t e, uint32_t out[4]);
+
 /* Device -> host copy of product memory (validation, --transfer). */
 int pxg_copy_to_host(pxg_stream* s, void* dst, const void* src, size_t bytes);
 

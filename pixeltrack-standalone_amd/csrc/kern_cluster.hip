@@ -1,18 +1,28 @@
-// Stage A, part 2: per-module clustering, cluster charge cut and the
-// clusters-per-module prefix (one fused kernel).
+// Stage A, part 2: per-module clustering with the cluster charge cut, and
+// the clusters-per-module prefix.
 //
-// One workgroup per (module run, event).  The module's pixels are staged in
-// LDS; a column histogram sorts them by column; every pixel unites with the
-// pixels of its own and the next column within one row (8-connectivity)
-// through a lock-free union-find in LDS that always hooks the larger root
-// under the smaller one, so every component ends with its smallest pixel
-// index as root.  Roots are then numbered in increasing pixel index by a
-// workgroup scan, which is exactly the serial program's cluster numbering
-// (gpuClustering.h:40-273: labels converge to the component's minimum index,
-// roots are numbered in index order).  The charge cut (gpuClusterChargeCut.h:
-// 14-114) runs on the same LDS state: integer LDS atomics for the charges, a
-// scan for the renumbering.  The last workgroup of an event to finish builds
-// clusModuleStart (SiPixelRawToClusterGPUKernel.cc:464-506).
+// The serial program labels every pixel of a module with the smallest pixel
+// index of its 8-connected component and numbers the components by that
+// index (gpuClustering.h:40-273), then drops clusters under the charge cut
+// and renumbers the survivors in order (gpuClusterChargeCut.h:14-114).  The
+// result depends only on the connected components, so any exact algorithm
+// gives the same labels; two are used here, picked per module run:
+//
+//   clusterSmallKernel  one wavefront per module run of at most 64 digis
+//                       (most modules at PU50 carry ~30): a 64-bit neighbour
+//                       mask per lane from shuffles, min-label propagation
+//                       with pointer jumping through a per-wave LDS row until
+//                       a ballot sees no change, roots ranked with popcounts.
+//                       No workgroup barrier, no atomics but the charge sums.
+//                       Larger runs are appended to a work list.
+//   clusterBigKernel    persistent workgroups draining that list: pixels
+//                       staged in LDS, column histogram, lock-free union-find
+//                       that hooks the larger root under the smaller (so the
+//                       root of a component is its smallest index), roots
+//                       numbered by a workgroup scan.
+//   clusStartKernel     one workgroup per event: the clamped prefix over the
+//                       2000 cluster counts (SiPixelRawToClusterGPUKernel.cc:
+//                       464-506).
 #include "pxgpu_internal.h"
 
 namespace pxg {
@@ -22,6 +32,7 @@ namespace pxg {
     constexpr int kWaves = kBlock / 64;
     constexpr uint32_t kNoPixel = 0xffffffffu;
     constexpr int kCols = kColsPerModule;  // 416
+    constexpr int kBigWorkgroups = 512;  // persistent workgroups for the large module runs
 
     // exclusive workgroup scan of one 32-bit value per thread; returns the total
     __device__ __forceinline__ uint32_t scanExclusive(uint32_t v, uint32_t& excl, uint32_t* wsum) {
@@ -77,21 +88,25 @@ namespace pxg {
     }
   }  // namespace
 
-  __global__ void __launch_bounds__(kBlock) clusterKernel(Pool p) {
-    const uint32_t k = blockIdx.x;  // module run index
-    const uint32_t e = blockIdx.y;
+  struct BigLds {
+    uint32_t sxy[kMaxPixInModule];     // x | y << 16, kNoPixel if invalid
+    uint32_t parent[kMaxPixInModule];  // union-find; later the cluster id
+    uint32_t work[kMaxPixInModule];    // column-sorted pixels; later charges/renumbering
+    uint32_t colStart[kCols + 2];
+    uint32_t colFill[kCols];
+    uint32_t wsum[kWaves];
+  };
+
+  // one module run with a whole workgroup (any size up to 4000 digis)
+  __device__ void clusterRunWG(const Pool& p, uint32_t e, uint32_t k, BigLds& L) {
+    uint32_t* sxy = L.sxy;
+    uint32_t* parent = L.parent;
+    uint32_t* work = L.work;
+    uint32_t* colStart = L.colStart;
+    uint32_t* wsum = L.wsum;
     const EventDesc& ev = p.events[e];
     EventCounts& cnt = p.counts[e];
     const uint32_t nModules = cnt.nModules;
-    if (k >= nModules)
-      return;
-
-    __shared__ uint32_t sxy[kMaxPixInModule];     // x | y << 16, kNoPixel if invalid
-    __shared__ uint32_t parent[kMaxPixInModule];  // union-find; later the cluster id
-    __shared__ uint32_t work[kMaxPixInModule];    // column-sorted pixels; later charges/renumbering
-    __shared__ uint32_t colStart[kCols + 2];
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ uint32_t shared1[4];
 
     const uint32_t* moduleStart = p.moduleStart + size_t(e) * kMaxModuleRanges;
     const uint32_t wo = ev.wordOffset;
@@ -152,7 +167,7 @@ namespace pxg {
     }
     __syncthreads();
     // fill the column lists (order inside a column is irrelevant)
-    __shared__ uint32_t colFill[kCols];
+    uint32_t* colFill = L.colFill;
     for (int i = threadIdx.x; i < kCols; i += kBlock)
       colFill[i] = colStart[i];
     __syncthreads();
@@ -268,44 +283,182 @@ namespace pxg {
       p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
     }
 
-    // ---- the last workgroup of the event builds clusModuleStart
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t done = atomicAdd(&cnt.modulesDone, 1u);
-      shared1[0] = (done == nModules - 1) ? 1u : 0u;
+  }
+
+  namespace {
+    __device__ __forceinline__ void waveSync() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __syncthreads();
-    if (shared1[0] == 0)
+    __device__ __forceinline__ uint64_t lanesBelow(uint32_t lane) { return (1ull << lane) - 1ull; }
+  }  // namespace
+
+  // ---- module runs of at most 64 digis: one wavefront each -------------------------
+  __global__ void __launch_bounds__(kBlock) clusterSmallKernel(Pool p) {
+    __shared__ int32_t rowLds[kWaves][64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t k = blockIdx.x * kWaves + wave;
+    const uint32_t e = blockIdx.y;
+    const EventDesc& ev = p.events[e];
+    const uint32_t nModules = p.counts[e].nModules;
+    if (k >= nModules)
+      return;  // uniform per wavefront; this kernel has no workgroup barrier
+    const uint32_t* ms = p.moduleStart + size_t(e) * kMaxModuleRanges;
+    const uint32_t first = ms[1 + k];
+    const uint32_t end = (k + 1 < nModules) ? ms[2 + k] : ev.nWords;
+    const uint32_t n = end - first;
+    if (n > 64) {
+      if (lane == 0) {
+        const uint32_t slot = atomicAdd(&p.work[0], 1u);
+        p.bigList[slot] = (e << 16) | k;
+      }
       return;
-    __threadfence();
+    }
+    const uint32_t wo = ev.wordOffset;
+    const uint16_t me = p.moduleInd[wo + first];
+    if (me >= kNumModules)
+      return;  // a run always starts on a valid digi; never taken for valid conditions
+    int32_t* row = rowLds[wave];
+
+    const bool act = lane < n;
+    const uint32_t g = wo + first + lane;
+    uint32_t x = 0, y = 0;
+    int32_t adc = 0;
+    bool valid = false;
+    if (act) {
+      x = p.xx[g];
+      y = p.yy[g];
+      adc = p.adc[g];
+      valid = p.moduleInd[g] != kInvId && x < kRowsPerModule && y < kColsPerModule;
+    }
+    // 8-neighbourhood as a lane mask (duplicated pixels are neighbours too)
+    uint64_t nb = 0;
+    const int vflag = valid ? 1 : 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const int xj = __shfl(int(x), int(j)), yj = __shfl(int(y), int(j)), vj = __shfl(vflag, int(j));
+      const int dx = xj - int(x), dy = yj - int(y);
+      if (valid && vj && j != lane && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
+        nb |= 1ull << j;
+    }
+    // min-label propagation with pointer jumping; every label is a lane of the same component
+    uint32_t label = lane;
+    while (true) {
+      row[lane] = int32_t(label);
+      waveSync();
+      uint32_t m = label;
+      for (uint64_t b = nb; b; b &= b - 1) {
+        const uint32_t l = uint32_t(row[__ffsll((unsigned long long)b) - 1]);
+        m = l < m ? l : m;
+      }
+      const uint32_t jump = uint32_t(row[m]);
+      m = jump < m ? jump : m;
+      waveSync();
+      const bool changed = m != label;
+      label = m;
+      if (__ballot(changed) == 0)
+        break;
+    }
+    // roots (label == own index) numbered in index order
+    const uint64_t roots = __ballot(valid && label == lane);
+    const uint32_t nclus = __popcll(roots);
+    const uint32_t cid = __popcll(roots & lanesBelow(label));
+    // charge cut
+    row[lane] = 0;
+    waveSync();
+    if (valid)
+      atomicAdd(&row[cid], adc);
+    waveSync();
+    const bool isRoot = (roots >> lane) & 1;
+    const int cut = me < 96 ? 2000 : 4000;
+    const uint64_t okRoots = __ballot(isRoot && row[__popcll(roots & lanesBelow(lane))] > cut);
+    const uint32_t kept = __popcll(okRoots);
+    const bool renumber = nclus > 0 && kept != nclus;
+    if (act) {
+      if (!valid) {
+        p.clus[g] = -9999;
+      } else if (!renumber) {
+        p.clus[g] = int32_t(cid);
+      } else if ((okRoots >> label) & 1) {
+        p.clus[g] = int32_t(__popcll(okRoots & lanesBelow(label)));
+      } else {
+        p.clus[g] = 9999;
+        p.moduleInd[g] = kInvId;
+      }
+    }
+    if (lane == 0) {
+      p.clusInModule[size_t(e) * kMaxModuleRanges + me] = renumber ? kept : nclus;
+      p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
+    }
+  }
+
+  // ---- larger module runs: persistent workgroups over the work list ---------------
+  __global__ void __launch_bounds__(kBlock) clusterBigKernel(Pool p) {
+    __shared__ BigLds L;
+    __shared__ uint32_t item;
+    const uint32_t total = p.work[0];
+    while (true) {
+      if (threadIdx.x == 0)
+        item = atomicAdd(&p.work[1], 1u);
+      __syncthreads();
+      const uint32_t it = item;
+      __syncthreads();
+      if (it >= total)
+        break;  // every workgroup leaves once the list is drained
+      const uint32_t code = p.bigList[it];
+      clusterRunWG(p, code >> 16, code & 0xffff, L);
+      __syncthreads();
+    }
+  }
+
+  // ---- clamped prefix of the clusters per module -----------------------------------
+  __global__ void __launch_bounds__(1024) clusStartKernel(Pool p) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t e = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t* cim = p.clusInModule + size_t(e) * kMaxModuleRanges;
     uint32_t* cms = p.clusModuleStart + size_t(e) * kMaxModuleRanges;
     uint32_t run = 0;
-    for (uint32_t c0 = 0; c0 < kMaxNumModules; c0 += kBlock) {
+    for (uint32_t c0 = 0; c0 < kMaxNumModules; c0 += 1024) {
       const uint32_t m = c0 + threadIdx.x;
-      uint32_t v = 0;
-      if (m < kMaxNumModules) {
-        v = __hip_atomic_load(&clusInModule[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v = v < kMaxClustersPerModule ? v : kMaxClustersPerModule;
+      uint32_t v = m < kMaxNumModules ? cim[m] : 0;
+      v = v < kMaxClustersPerModule ? v : kMaxClustersPerModule;
+      uint32_t incl = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (int(lane) >= o)
+          incl += t;
       }
-      uint32_t excl;
-      const uint32_t tot = scanExclusive(v, excl, wsum);
+      if (lane == 63)
+        wsum[wave] = incl;
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (uint32_t w = 0; w < 16; ++w) {
+        const uint32_t sw = wsum[w];
+        before += w < wave ? sw : 0;
+        tot += sw;
+      }
+      __syncthreads();
       if (m < kMaxNumModules) {
-        const uint32_t s = run + excl + v;
-        cms[m + 1] = s < kMaxNumClusters ? s : kMaxNumClusters;
+        const uint32_t sum = run + before + incl;
+        cms[m + 1] = sum < kMaxNumClusters ? sum : kMaxNumClusters;
       }
       run += tot;
     }
     if (threadIdx.x == 0) {
       cms[0] = 0;
+      EventCounts& cnt = p.counts[e];
       cnt.nClusters = run < kMaxNumClusters ? run : kMaxNumClusters;
       if (run > kMaxNumClusters)
-        atomicOr(&cnt.status, kStatusHitOverflow);
+        cnt.status |= kStatusHitOverflow;
     }
   }
 
   void launchClusterize(const Pool& p, uint32_t nEvents, hipStream_t s) {
-    clusterKernel<<<dim3(kMaxNumModules, nEvents), kBlock, 0, s>>>(p);
+    clusterSmallKernel<<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(p);
+    clusterBigKernel<<<kBigWorkgroups, kBlock, 0, s>>>(p);
+    clusStartKernel<<<nEvents, 1024, 0, s>>>(p);
   }
 
 }  // namespace pxg

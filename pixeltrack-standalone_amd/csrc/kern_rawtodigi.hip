@@ -64,7 +64,7 @@ namespace pxg {
           // raw id of the error (getErrRawID)
           uint32_t rid = 0xffffffffu;
           uint32_t eLink = 0, eRoc = 1;
-          bool look = true;
+          bool look = false;  // types 26, 27, 28 carry no module (getErrRawID's default)
           if (type == 29) {
             int ch = int(w & 1) + 2 * int((w >> 1) & 1) + 3 * int((w >> 2) & 1) + 4 * int((w >> 3) & 1) +
                      5 * int((w >> 4) & 1);
@@ -72,8 +72,9 @@ namespace pxg {
             int chan = (block & 1) == 0 ? (block / 2) * 9 + ch : ((block - 1) / 2) * 9 + 4 + ch;
             look = chan >= 1 && chan <= 36;
             eLink = uint32_t(chan);
-          } else {
-            eLink = link;  // 25, 30, 31, 40
+          } else if (type == 25 || type == 30 || type == 31 || type == 40) {
+            eLink = link;
+            look = true;
           }
           const uint32_t eSlot = fed * kSlotsPerFed + (eLink - 1) * kMaxRoc + eRoc;
           if (look && eSlot < kCablingSize) {
@@ -362,6 +363,8 @@ namespace pxg {
       baseMod += uint32_t(total & 0xffffffffu);
       baseErr += uint32_t(total >> 32);
     }
+    if (e == 0 && threadIdx.x < 2)
+      p.work[threadIdx.x] = 0;  // the clusteriser's list of large module runs
     if (threadIdx.x == 0) {
       const uint32_t nMod = baseMod < kMaxNumModules ? baseMod : kMaxNumModules;
       moduleStart[0] = nMod;

@@ -289,6 +289,8 @@ int pxg_stream_create(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, p
   ALLOC(p.boundaryMask, 2 * (W / 64));
   ALLOC(p.errorsSparse, W);
   ALLOC(p.errors, W);
+  ALLOC(p.work, 64);
+  ALLOC(p.bigList, M);
   ALLOC(p.moduleStart, M);
   ALLOC(p.clusInModule, M);
   ALLOC(p.moduleId, M);
@@ -411,6 +413,17 @@ int pxg_event_products(pxg_stream* s, uint32_t e, pxg_digis* d, pxg_clusters* c,
   if (!s || e >= s->nLoaded)
     return fail(PXG_ERR_ARG, "pxg_event_products: event %u not resident", e);
   fillProducts(s, e, d, c, er, h);
+  return 0;
+}
+
+int pxg_event_counts(pxg_stream* s, uint32_t e, uint32_t out[4]) {
+  if (!s || !out || e >= s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_event_counts: event %u not resident", e);
+  const EventCounts& c = s->hostCounts[e];
+  out[0] = c.nModules;
+  out[1] = c.nClusters;
+  out[2] = c.nErrors;
+  out[3] = c.status;
   return 0;
 }
 

@@ -81,6 +81,8 @@ namespace pxg {
     uint64_t* boundaryMask;  // [maxWords / 64]
     PixelErrorCompact* errorsSparse;  // [maxWords], error records at their word index
     PixelErrorCompact* errors;        // event e: errors[wordOffset[e] ...], word order
+    uint32_t* work;     // [0] large module runs listed, [1] next one to take
+    uint32_t* bigList;  // large module runs of the launch: event << 16 | run
     // clusters, per event
     uint32_t *moduleStart, *clusInModule, *moduleId, *clusModuleStart;  // stride kMaxModuleRanges
     // hits, per event (stride kMaxNumClusters)

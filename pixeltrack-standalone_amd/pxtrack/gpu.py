@@ -75,6 +75,7 @@ def _gpu():
         lib.pxg_event_products.argtypes = [vp, u32, C.POINTER(Digis), C.POINTER(Clusters), C.POINTER(Errors),
                                            C.POINTER(Hits)]
         lib.pxg_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+        lib.pxg_event_counts.argtypes = [vp, u32, C.POINTER(u32)]
         lib.pxg_profile_enable.argtypes = [vp, C.c_int]
         lib.pxg_profile_read.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(u32), C.POINTER(C.c_double), C.c_int]
         lib._typed = True
@@ -264,6 +265,12 @@ class Stream:
             out.update(self.hits_to_host(h))
         out["errors"] = self.errors_to_host(er)
         return out
+
+    def event_counts(self, e):
+        """(module runs, clusters, error records, status) of resident event e."""
+        out = (u32 * 4)()
+        _check(self.lib.pxg_event_counts(self._h, e, out), "pxg_event_counts")
+        return tuple(out)
 
     def sync(self):
         _check(self.lib.pxg_sync(self._h), "pxg_sync")

@@ -66,3 +66,22 @@ def assert_same(a, b, keys, ctx=""):
             bad = np.nonzero(x != y)[0]
             raise AssertionError(f"{ctx} {k}: {bad.size} of {x.size} differ, first at {bad[:5]}: "
                                  f"{np.asarray(a[k])[bad[:5]]} vs {np.asarray(b[k])[bad[:5]]}")
+
+
+ERROR_DTYPE = np.dtype({"names": ["rawId", "word", "errorType", "fedId"], "formats": [np.uint32, np.uint32, np.uint8, np.uint8],
+                        "offsets": [0, 4, 8, 9], "itemsize": 12})
+
+
+def error_records(raw_bytes):
+    """PixelErrorCompact records (12 bytes, 2 of them padding) -> structured array of the 4 fields."""
+    rec = np.frombuffer(np.ascontiguousarray(raw_bytes).tobytes(), dtype=ERROR_DTYPE)
+    out = np.zeros(rec.size, dtype=[("rawId", np.uint32), ("word", np.uint32), ("errorType", np.uint8), ("fedId", np.uint8)])
+    for f in out.dtype.names:
+        out[f] = rec[f]
+    return out
+
+
+def assert_same_errors(a, b, ctx=""):
+    ea, eb = error_records(a), error_records(b)
+    assert ea.size == eb.size, f"{ctx}: {ea.size} != {eb.size} error records"
+    assert np.array_equal(ea, eb), f"{ctx}: error records differ"

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from conftest import assert_same, event
+from conftest import assert_same, assert_same_errors, event
 
 FIELDS = [f for f in pyoracle.FIELDS]
 
@@ -26,7 +26,8 @@ def test_oracle_matches_reference(detector, oracle, reference, preset, index, kw
     b = reference.process(raw)
     for s in pyoracle.SCALARS:
         assert a[s] == b[s], f"{s}: {a[s]} != {b[s]}"
-    assert_same(a, b, [f for f in FIELDS if f in a and f in b], f"{preset}[{index}]")
+    assert_same(a, b, [f for f in FIELDS if f in a and f in b and f != "errors"], f"{preset}[{index}]")
+    assert_same_errors(a["errors"], b["errors"], f"{preset}[{index}]")
     assert set(FIELDS) <= set(a) and set(FIELDS) <= set(b)
 
 
