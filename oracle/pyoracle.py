@@ -37,6 +37,14 @@ def _load(path, prefix):
         getattr(lib, prefix + "_atan2s7").restype = C.c_int16
         getattr(lib, prefix + "_atan2s7").argtypes = [C.c_float, C.c_float]
         if prefix == "orc":
+            lib.orc_conditions_set_profile.restype = C.c_int
+            lib.orc_conditions_set_profile.argtypes = [vp, C.c_int]
+            lib.orc_fit_helix.restype = C.c_int
+            lib.orc_fit_helix.argtypes = [C.c_int, vp, vp, C.c_double, vp, vp, vp, vp, vp]
+            lib.orc_invert_spd.argtypes = [vp, vp, C.c_int]
+            lib.orc_vertices.restype = C.c_int
+            lib.orc_vertices.argtypes = [C.c_uint32] + [vp] * 10
+            lib.orc_circle.argtypes = [C.c_float] * 6 + [C.POINTER(C.c_float)] * 2
             lib.orc_run_many.restype = C.c_uint32
             lib.orc_run_many.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_uint32]
         else:
@@ -105,12 +113,68 @@ FIELDS = ["words", "fedIds", "xx", "yy", "adc", "moduleInd", "clus", "pdigi", "r
           "moduleStart", "clusInModule", "moduleId", "clusModuleStart", "xl", "yl", "xerr", "yerr", "xg", "yg",
           "zg", "rg", "iphi", "charge", "xsize", "ysize", "detInd", "hitsLayerStart", "phiOff", "phiBins",
           "avgCorrected"]
-SCALARS = ["nWords", "nModules", "nDigis", "nClusters", "nHits", "fedCounter"]
+# stage C/D products (only with upto >= 4): cells as parallel arrays plus CSR
+# neighbour/track lists, tuples as CSR (tupleOff/tupleBins), fits per tuple,
+# vertices per track / per vertex
+CA_FIELDS = ["cellInner", "cellOuter", "cellDoubletId", "cellLayerPair", "cellUsed", "cellNeighborOff",
+             "cellNeighbors", "cellTrackOff", "cellTracks", "outerHitOff", "outerHitCells", "tupleOff", "tupleBins",
+             "quality", "multOff", "multBins", "detIndices", "hitToTupleOff", "hitToTupleBins", "state",
+             "covariance", "pt", "eta", "chi2", "idv", "zv", "wv", "chi2v", "ptv2", "ndof", "sortInd"]
+FIELDS = FIELDS + CA_FIELDS
+AB_SCALARS = ["nWords", "nModules", "nDigis", "nClusters", "nHits", "fedCounter"]
+SCALARS = AB_SCALARS + ["nCells", "nTuples", "caStatus", "nVertices", "vtxTracks", "nNeighborLists", "nTrackLists"]
 
 
 class Oracle(_Chain):
     prefix = "orc"
     path = ORACLE_LIB
+
+    def __init__(self, conditions, profile=0):
+        super().__init__(conditions)
+        if self.lib.orc_conditions_set_profile(self._h, profile) != 0:
+            raise ValueError(f"unknown capacity profile {profile}")
+
+    def fit_helix(self, hits, ge, B):
+        """Broken-line fit of one track: hits (3, n) global xyz in cm (double), ge (6, n) global
+        errors (xx, xy, yy, xz, yz, zz; float); returns state[5], cov[15], pt, eta, chi2."""
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ge = np.ascontiguousarray(ge, dtype=np.float32)
+        n = hits.shape[1]
+        st = np.zeros(5, np.float32)
+        cv = np.zeros(15, np.float32)
+        sc = np.zeros(3, np.float32)
+        a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        rc = self.lib.orc_fit_helix(n, a(hits), a(ge), B, a(st), a(cv), a(sc[0:]), a(sc[1:]), a(sc[2:]))
+        if rc != 0:
+            raise ValueError(f"fit_helix: unsupported number of hits {n}")
+        return st, cv, float(sc[0]), float(sc[1]), float(sc[2])
+
+    def invert_spd(self, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        out = np.zeros_like(m)
+        self.lib.orc_invert_spd(m.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), m.shape[0])
+        return out
+
+    def vertices(self, zt, ezt2, ptt2):
+        """Vertex finder on given tracks; returns dict of per-track vertex ids and per-vertex arrays."""
+        zt, ezt2, ptt2 = (np.ascontiguousarray(a, dtype=np.float32) for a in (zt, ezt2, ptt2))
+        nt = zt.size
+        iv = np.zeros(nt, np.int16)
+        out = {k: np.zeros(1024, np.float32) for k in ("zv", "wv", "chi2", "ptv2")}
+        ndof = np.zeros(1024, np.int32)
+        srt = np.zeros(1024, np.uint16)
+        a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        nv = self.lib.orc_vertices(nt, a(zt), a(ezt2), a(ptt2), a(iv), a(out["zv"]), a(out["wv"]), a(out["chi2"]),
+                                   a(out["ptv2"]), a(ndof), a(srt))
+        res = {k: v[:nv] for k, v in out.items()}
+        res.update(iv=iv, ndof=ndof[:nv], sortInd=srt[:nv], nv=nv)
+        return res
+
+    def circle(self, x1, y1, x2, y2, x3, y3):
+        """Signed curvature c of the circle through 3 points and dca0 = c * (distance of closest approach to (0,0))."""
+        c, d = C.c_float(), C.c_float()
+        self.lib.orc_circle(x1, y1, x2, y2, x3, y3, C.byref(c), C.byref(d))
+        return c.value, d.value
 
     def run_many(self, raws, upto=3, threads=1, repeat=1):
         """Processes the raw records `repeat` times over `threads` workers; returns events done."""

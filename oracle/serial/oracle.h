@@ -12,8 +12,15 @@
 //     on seeded events, and against committed golden fixtures made from them;
 //     plus brute-force connected components, closed-form clamped prefix sums,
 //     atan2 accuracy bounds and frame orthogonality.
+//   stage C/D (a11-a25): the reference's CA and vertexing sources need Eigen,
+//     which is not available, so they cannot be compiled here; these parts are
+//     pinned by brute force and invariants (doublets against an O(n^2) pair
+//     search with the same cuts, tuples against an exhaustive walk of the cell
+//     graph, fits against noiseless helices and matrix identities).  The fit's
+//     numerical parity with the reference is unpinned.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -35,6 +42,81 @@ namespace orc {
     const BeamSpotPOD* beamSpot;
     const uint32_t* fedIds;
     uint32_t nFeds;
+    CACapacities caps = kCapsReference;
+  };
+
+  // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
+  // run-time ones are in Event::caps (pxformats.h CACapacities)
+  constexpr int kMaxCellsPerHit = 128;
+  constexpr uint32_t kMaxVertices = 1024;
+  constexpr double kBField = 0.0114256972711507;  // CAHitNtupletCUDA.cc:31
+  enum : uint8_t { kQualityBad = 0, kQualityDup = 1, kQualityLoose = 2 };
+  enum : uint32_t {
+    kCaCellOverflow = 1u << 0,
+    kCaOuterHitOverflow = 1u << 1,
+    kCaNeighborOverflow = 1u << 2,
+    kCaNeighborStoreOverflow = 1u << 3,
+    kCaTrackOverflow = 1u << 4,
+    kCaTrackStoreOverflow = 1u << 5,
+    kCaTupleOverflow = 1u << 6,
+    kCaDepthOverflow = 1u << 7,
+    kCaHitToTupleOverflow = 1u << 8,
+    kCaMultiplicityOverflow = 1u << 9,
+  };
+
+  template <class T, int N>
+  struct Vec {
+    int size = 0;
+    T data[N];
+    int push(T v) {
+      if (size < N) {
+        data[size] = v;
+        return size++;
+      }
+      return -1;
+    }
+  };
+
+  struct Cell {
+    int32_t doubletId;
+    int16_t layerPairId;
+    uint16_t used;
+    float innerZ, innerR;
+    uint16_t innerHit, outerHit;
+    int32_t neighbors;  // slot in the neighbour store (0: none)
+    int32_t tracks;     // slot in the track store (0: none)
+  };
+
+  struct CAState {
+    uint32_t status = 0;
+    uint32_t nCells = 0;
+    std::vector<Cell> cells;
+    std::vector<Vec<uint32_t, kMaxCellsPerHit>> outerHitOfCell;
+    std::vector<Vec<uint32_t, 36>> neighborStore;
+    std::vector<Vec<uint16_t, 48>> trackStore;
+    uint32_t apcM = 0, apcN = 0, nTuples = 0;
+    std::vector<uint32_t> tupleOff;   // [32769]
+    std::vector<uint16_t> tupleBins;  // hit indices
+    std::vector<uint8_t> quality;
+    std::vector<uint32_t> multOff;  // [9]
+    std::vector<uint16_t> multBins;
+    std::vector<uint16_t> detIndices;
+    std::vector<uint32_t> hitToTupleOff;
+    std::vector<uint16_t> hitToTupleBins;
+    std::vector<std::array<float, 5>> state;
+    std::vector<std::array<float, 15>> cov;
+    std::vector<float> pt, eta, chi2;
+  };
+
+  struct VertexState {
+    uint32_t status = 0;
+    uint32_t ntrks = 0, nvFinal = 0;
+    std::vector<int16_t> idv;
+    std::vector<int32_t> ndof;
+    std::vector<float> zv, wv, chi2, ptv2;
+    std::vector<uint16_t> sortInd;
+    std::vector<float> zt;
+    std::vector<int32_t> iv;
   };
 
   struct Event {
@@ -60,6 +142,10 @@ namespace orc {
     std::vector<uint32_t> phiOff;     // 1281
     std::vector<uint16_t> phiBins;    // nHits
     AverageGeometry avgCorrected;
+    // a11-a22: CA ntuplets and fits; a23-a25: vertices
+    CACapacities caps = kCapsReference;
+    CAState ca;
+    VertexState vtx;
   };
 
   // a1 SiPixelRawToClusterCUDA.cc:87-138 on one raw record (raw.bin event layout)
@@ -77,7 +163,31 @@ namespace orc {
   void setHitsLayerStart(const Conditions& c, Event& ev);
   void fillPhiBinner(Event& ev);
 
-  // runs the chain up to and including the given stage: 1 = digis, 2 = clusters, 3 = rec-hits
+  // a11-a17, a20-a22 (orc_stageC.cc)
+  void buildDoublets(Event& ev);
+  void connectCells(Event& ev);
+  void fishbone(Event& ev);
+  void findNtuplets(Event& ev);
+  void earlyDuplicatesAndMultiplicity(Event& ev);
+  void fillHitDetIndices(Event& ev);
+  void classifyTracks(Event& ev);
+  void fastDuplicateRemover(Event& ev);
+  void hitToTupleAndTripletCleaner(Event& ev);
+  void circleForTest(float x1, float y1, float x2, float y2, float x3, float y3, float* curvature, float* dca);
+  // a18-a19 (orc_fit.cc)
+  void fitTracks(const Conditions& c, Event& ev);
+  int fitHelix(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt, float* eta,
+               float* chi2);
+  void invertSPDForTest(const double* a, double* inv, int n);
+  // a23-a25 (orc_stageD.cc)
+  void findVertices(Event& ev);
+  void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
+                     const std::vector<float>& ezt2, const std::vector<float>& ptt2);
+  int verticesDirect(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv,
+                     float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd);
+
+  // runs the chain up to and including the given stage: 1 = digis, 2 = clusters,
+  // 3 = rec-hits, 4 = ntuplets + fits, 5 = vertices
   bool process(const Conditions& c, const uint8_t* raw, size_t len, int upto, Event& ev);
 
   // exposed for unit tests

@@ -3,6 +3,7 @@
 // baseline: run the serial chain on one raw record and read its products by
 // field name.
 #include <atomic>
+#include <map>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -21,7 +22,97 @@ struct orc_conditions {
 
 struct orc_event {
   Event ev;
+  std::map<std::string, std::pair<std::vector<uint8_t>, int>> exported;  // materialised views
 };
+
+}  // extern "C"
+
+namespace {
+  template <typename T>
+  void exportVec(orc_event* e, const std::string& name, const std::vector<T>& v, int code) {
+    auto& slot = e->exported[name];
+    slot.first.resize(v.size() * sizeof(T));
+    if (!v.empty())
+      std::memcpy(slot.first.data(), v.data(), slot.first.size());
+    slot.second = code;
+  }
+
+  // stage C/D products as flat arrays
+  void exportCA(orc_event* e) {
+    const Event& ev = e->ev;
+    const CAState& C = ev.ca;
+    std::vector<uint16_t> inner, outer;
+    std::vector<int32_t> did;
+    std::vector<int16_t> lp;
+    std::vector<uint16_t> used;
+    std::vector<uint32_t> nbOff{0}, nbVal, trOff{0};
+    std::vector<uint16_t> trVal;
+    for (const Cell& c : C.cells) {
+      inner.push_back(c.innerHit);
+      outer.push_back(c.outerHit);
+      did.push_back(c.doubletId);
+      lp.push_back(c.layerPairId);
+      used.push_back(c.used);
+      const auto& nb = C.neighborStore[c.neighbors];
+      for (int k = 0; k < nb.size; ++k)
+        nbVal.push_back(nb.data[k]);
+      nbOff.push_back(uint32_t(nbVal.size()));
+      const auto& tr = C.trackStore[c.tracks];
+      for (int k = 0; k < tr.size; ++k)
+        trVal.push_back(tr.data[k]);
+      trOff.push_back(uint32_t(trVal.size()));
+    }
+    exportVec(e, "cellInner", inner, 2);
+    exportVec(e, "cellOuter", outer, 2);
+    exportVec(e, "cellDoubletId", did, 5);
+    exportVec(e, "cellLayerPair", lp, 3);
+    exportVec(e, "cellUsed", used, 2);
+    exportVec(e, "cellNeighborOff", nbOff, 4);
+    exportVec(e, "cellNeighbors", nbVal, 4);
+    exportVec(e, "cellTrackOff", trOff, 4);
+    exportVec(e, "cellTracks", trVal, 2);
+    std::vector<uint32_t> ohOff{0}, ohVal;
+    for (uint32_t h = 0; h < ev.nHits && h < C.outerHitOfCell.size(); ++h) {
+      const auto& v = C.outerHitOfCell[h];
+      for (int k = 0; k < v.size; ++k)
+        ohVal.push_back(v.data[k]);
+      ohOff.push_back(uint32_t(ohVal.size()));
+    }
+    exportVec(e, "outerHitOff", ohOff, 4);
+    exportVec(e, "outerHitCells", ohVal, 4);
+    exportVec(e, "tupleOff", C.tupleOff, 4);
+    std::vector<uint16_t> bins(C.tupleBins.begin(), C.tupleBins.begin() + std::min<size_t>(C.tupleBins.size(),
+                                                                                              C.tupleOff.empty() ? 0 : C.tupleOff.back()));
+    exportVec(e, "tupleBins", bins, 2);
+    exportVec(e, "quality", C.quality, 1);
+    exportVec(e, "multOff", C.multOff, 4);
+    exportVec(e, "multBins", C.multBins, 2);
+    std::vector<uint16_t> det(C.detIndices.begin(), C.detIndices.begin() + bins.size());
+    exportVec(e, "detIndices", det, 2);
+    exportVec(e, "hitToTupleOff", C.hitToTupleOff, 4);
+    exportVec(e, "hitToTupleBins", C.hitToTupleBins, 2);
+    std::vector<float> st, cv;
+    for (auto& a : C.state)
+      st.insert(st.end(), a.begin(), a.end());
+    for (auto& a : C.cov)
+      cv.insert(cv.end(), a.begin(), a.end());
+    exportVec(e, "state", st, 6);
+    exportVec(e, "covariance", cv, 6);
+    exportVec(e, "pt", C.pt, 6);
+    exportVec(e, "eta", C.eta, 6);
+    exportVec(e, "chi2", C.chi2, 6);
+    const VertexState& V = ev.vtx;
+    exportVec(e, "idv", V.idv, 3);
+    exportVec(e, "zv", V.zv, 6);
+    exportVec(e, "wv", V.wv, 6);
+    exportVec(e, "chi2v", V.chi2, 6);
+    exportVec(e, "ptv2", V.ptv2, 6);
+    exportVec(e, "ndof", V.ndof, 5);
+    exportVec(e, "sortInd", V.sortInd, 2);
+  }
+}  // namespace
+
+extern "C" {
 
 // dtype codes shared with the Python side: 1 u8, 2 u16, 3 i16, 4 u32, 5 i32, 6 f32, 7 f64
 orc_conditions* orc_conditions_create(const void* cabling, const uint8_t* modToUnp, const void* gain,
@@ -45,11 +136,24 @@ orc_conditions* orc_conditions_create(const void* cabling, const uint8_t* modToU
 
 void orc_conditions_destroy(orc_conditions* h) { delete h; }
 
+// capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
+int orc_conditions_set_profile(orc_conditions* h, int profile) {
+  if (profile == 0)
+    h->c.caps = kCapsReference;
+  else if (profile == 1)
+    h->c.caps = kCapsHighOccupancy;
+  else
+    return -1;
+  return 0;
+}
+
 // upto: 1 digis, 2 clusters, 3 rec-hits
 orc_event* orc_process(const orc_conditions* h, const uint8_t* raw, size_t len, int upto) {
   auto ev = std::make_unique<orc_event>();
   if (!process(h->c, raw, len, upto, ev->ev))
     return nullptr;
+  if (upto >= 4)
+    exportCA(ev.get());
   return ev.release();
 }
 
@@ -104,6 +208,13 @@ const void* orc_field(const orc_event* e, const char* name, size_t* count, int* 
     *dtype = 6;
     return &ev.avgCorrected;
   }
+  auto it = e->exported.find(n);
+  if (it != e->exported.end()) {
+    static const size_t sizes[] = {0, 1, 2, 2, 4, 4, 4, 8};
+    *dtype = it->second.second;
+    *count = it->second.first.size() / sizes[*dtype];
+    return it->second.first.data();
+  }
   *count = 0;
   *dtype = 0;
   return nullptr;
@@ -125,7 +236,40 @@ uint32_t orc_scalar(const orc_event* e, const char* name) {
     return ev.nHits;
   if (n == "fedCounter")
     return ev.fedCounter;
+  if (n == "nCells")
+    return ev.ca.nCells;
+  if (n == "nTuples")
+    return ev.ca.nTuples;
+  if (n == "caStatus")
+    return ev.ca.status;
+  if (n == "nVertices")
+    return ev.vtx.nvFinal;
+  if (n == "vtxTracks")
+    return ev.vtx.ntrks;
+  if (n == "nNeighborLists")
+    return uint32_t(ev.ca.neighborStore.size());
+  if (n == "nTrackLists")
+    return uint32_t(ev.ca.trackStore.size());
   return 0xffffffffu;
+}
+
+int orc_fit_helix(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt, float* eta,
+                  float* chi2) {
+  return fitHelix(n, hits, ge, B, state, cov, pt, eta, chi2);
+}
+
+void orc_invert_spd(const double* a, double* inv, int n) { invertSPDForTest(a, inv, n); }
+
+// vertex finder on given track z, z error^2 and pt^2 (all tracks used)
+int orc_vertices(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv, float* wv,
+                 float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd) {
+  return verticesDirect(nt, zt, ezt2, ptt2, iv, zv, wv, chi2, ptv2, ndof, sortInd);
+}
+
+// circle through three points as the CA cuts see it: curvature and dca0 (distance of closest
+// approach to the origin times the curvature)
+void orc_circle(float x1, float y1, float x2, float y2, float x3, float y3, float* curvature, float* dca) {
+  circleForTest(x1, y1, x2, y2, x3, y3, curvature, dca);
 }
 
 int16_t orc_atan2s7(float y, float x) { return atan2s7(y, x); }

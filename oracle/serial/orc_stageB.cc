@@ -289,6 +289,7 @@ namespace orc {
   }
 
   bool process(const Conditions& c, const uint8_t* raw, size_t len, int upto, Event& ev) {
+    ev.caps = c.caps;
     if (!unpackFeds(c, raw, len, ev))
       return false;
     rawToDigi(c, ev);
@@ -304,6 +305,25 @@ namespace orc {
     getHits(c, ev);
     setHitsLayerStart(c, ev);
     fillPhiBinner(ev);
+    if (upto < 4)
+      return true;
+    // CAHitNtupletGeneratorOnGPU::makeTuples (CAHitNtupletGeneratorOnGPU.cc:96-126)
+    buildDoublets(ev);
+    connectCells(ev);
+    if (ev.nHits > 1)
+      fishbone(ev);
+    findNtuplets(ev);
+    earlyDuplicatesAndMultiplicity(ev);
+    fillHitDetIndices(ev);
+    fitTracks(c, ev);
+    if (ev.nHits > 0) {
+      classifyTracks(ev);
+      fastDuplicateRemover(ev);
+      hitToTupleAndTripletCleaner(ev);
+    }
+    if (upto < 5)
+      return true;
+    findVertices(ev);
     return true;
   }
 

@@ -51,6 +51,21 @@ namespace pxf {
   constexpr uint32_t kPhiBins = 128;
   constexpr uint32_t kPhiHistTotBins = kNumLayers * kPhiBins + 1;  // 1281
 
+  // Capacities of the ntuplet containers.  The reference profile is the
+  // serial program's (CAConstants.h:21-64, CAHitNtupletGeneratorOnGPU.cc:48,
+  // PixelTrackHeterogeneous.h:62); the high-occupancy profile scales them for
+  // PU200-like events.  Tuple indices are 16 bit and 60000 is the "none"
+  // marker of the duplicate remover, so maxTuples stays below it.
+  struct CACapacities {
+    uint32_t maxDoublets;    // cells
+    uint32_t maxTuples;      // tuple container bins (TrackSoA stride)
+    uint32_t maxTuplesMult;  // tuple multiplicity entries
+    uint32_t maxHitToTuple;  // hit -> tuple entries
+    uint32_t maxCellLists;   // cell neighbour lists / cell track lists (each store)
+  };
+  constexpr CACapacities kCapsReference{458752, 32 * 1024, 24 * 1024, 4 * 24 * 1024, 64 * 1024};
+  constexpr CACapacities kCapsHighOccupancy{4 * 458752, 56 * 1024, 56 * 1024, 4 * 56 * 1024, 256 * 1024};
+
   // ---- conditions ------------------------------------------------------------
   struct alignas(128) SiPixelFedCablingMapGPU {
     alignas(128) uint32_t fed[kCablingSize];

@@ -1,0 +1,429 @@
+"""Pins the oracle's ntuplet (stage C) and vertex (stage D) restatement.
+
+The reference's CA, fit and vertexing sources include Eigen through the track
+SoA headers, which this image does not have, so they cannot be compiled here.
+These tests pin the oracle instead by
+  * brute force with a different enumeration: doublets from an all-pairs search
+    ordered by the serial sweep key, neighbour lists from all cell pairs,
+    ntuplets from an exhaustive walk of the cell graph, per-cell track lists
+    and the multiplicity/hit-to-tuple containers rebuilt from the tuples;
+  * mathematics: the CA circle against the circumcircle, the broken-line fit on
+    noiseless helices (parameters recovered, chi2 ~ 0, SPD covariance), the
+    SPD inverse, and the vertex fit against weighted means and chi2 sums.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import event
+
+F = np.float32
+
+# gpuPixelDoublets.h:14-54: layer pairs, phi windows, z and r windows (data tables)
+PAIRS = [(0, 1), (0, 4), (0, 7), (1, 2), (1, 4), (1, 7), (4, 5), (7, 8), (2, 3), (2, 4), (2, 7), (5, 6), (8, 9),
+         (0, 2), (1, 3), (0, 5), (0, 8), (4, 6), (7, 9)]
+PHICUT = [522, 730, 730, 522, 626, 626, 522, 522, 626, 626, 626, 522, 522, 522, 522, 522, 522, 522, 522]
+MINZ = [-20., 0., -30., -22., 10., -30., -70., -70., -22., 15., -30, -70., -70., -20., -22., 0, -30., -70., -70.]
+MAXZ = [20., 30., 0., 22., 30., -10., 70., 70., 22., 30., -15., 70., 70., 20., 22., 30., 0., 70., 70.]
+MAXR = [20., 9., 9., 20., 7., 7., 5., 5., 20., 6., 6., 5., 5., 20., 20., 9., 9., 9., 9.]
+
+
+@pytest.fixture(scope="module")
+def small_events(detector, oracle):
+    out = []
+    for preset, idx, kw in [("serial100", 0, {}), ("serial100", 3, {}), ("pu50", 0, dict(pu_lo=5, pu_hi=5)),
+                            ("pu50", 1, dict(pu_lo=12, pu_hi=12))]:
+        raw, truth = event(detector, preset, idx, **kw)
+        out.append((f"{preset}[{idx}]{kw}", oracle.process(raw, upto=5), truth))
+    return out
+
+
+def _i16(a):
+    return np.asarray(a).astype(np.int64).astype(np.int16)
+
+
+def _doublets_bruteforce(o):
+    """All-pairs doublet search with the serial program's cuts, ordered by the serial sweep."""
+    ls = o["hitsLayerStart"]
+    z, r, iphi, ys = o["zg"], o["rg"], _i16(o["iphi"]), o["ysize"].astype(np.int64)
+    phipos = np.empty(o["nHits"], np.int64)  # position of each hit in the phi histogram
+    phipos[o["phiBins"]] = np.arange(o["nHits"])
+    i2p = F(math.pi / 32768)
+    minR2T4 = F(4) * F(F(0.5) * F(87.78)) * F(F(0.5) * F(87.78))
+    dzdr = F(8 * 0.0285 / 0.015)
+    keys = []
+    for p, (li, lo) in enumerate(PAIRS):
+        I = np.arange(ls[li], ls[li + 1])
+        O = np.arange(ls[lo], ls[lo + 1])
+        if I.size == 0 or O.size == 0:
+            continue
+        I = I[(z[I] >= F(MINZ[p])) & (z[I] <= F(MAXZ[p]))]
+        mes = ys[I]
+        if li in (0, 1) and lo > 3:
+            lim = 36 if li == 0 else 28
+            I, mes = I[~((mes > 0) & (mes < lim))], mes[~((mes > 0) & (mes < lim))]
+        if I.size == 0:
+            continue
+        ii, oo = np.meshgrid(I, O, indexing="ij")
+        mez, mer, zo, ro = z[ii], r[ii], z[oo], r[oo]
+        dr = ro - mer
+        z0 = (dr > F(MAXR[p])) | (dr < 0) | (np.abs(mez * ro - mer * zo) > F(12) * dr)
+        d1 = np.abs(_i16(iphi[oo].astype(np.int64) - iphi[ii].astype(np.int64)).astype(np.int64))
+        d2 = np.abs(_i16(iphi[ii].astype(np.int64) - iphi[oo].astype(np.int64)).astype(np.int64))
+        idphi = np.minimum(d1, d2) & 0xFFFF
+        phi_ok = idphi <= PHICUT[p]
+        so = ys[oo]
+        mesm = ys[ii]
+        if lo < 4:
+            dy = 28 if li == 0 else 20
+            zs = (mesm > 0) & (so > 0) & (np.abs(so - mesm) > dy)
+        else:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                pred = (np.abs((mez - zo) / (mer - ro)) * dzdr + F(0.5)).astype(np.float32)
+                pred = np.trunc(np.nan_to_num(pred, nan=0.0, posinf=2**31, neginf=-2**31)).astype(np.int64)
+            zs = (li < 4) & (mesm > 0) & (np.abs(mesm - pred) > 20)
+        dphi = idphi.astype(np.int16).astype(np.float32) * i2p
+        pt = dphi * dphi * (minR2T4 - mer * ro) > (ro - mer) * (ro - mer)
+        ok = ~z0 & phi_ok & ~zs & ~pt
+        # serial sweep order: phi bins from the low edge of the window upwards (with wrap)
+        kl = (_i16(iphi[ii].astype(np.int64) - PHICUT[p]).astype(np.int64) >> 9) & 127
+        bo = (iphi[oo].astype(np.int64) >> 9) & 127
+        sweep = (bo - kl) % 128
+        sel = np.nonzero(ok)
+        for a, b in zip(*sel):
+            keys.append((p, int(ii[a, b]), int(sweep[a, b]), int(phipos[oo[a, b]]), int(oo[a, b])))
+    keys.sort()
+    return np.array([k[1] for k in keys], np.int64), np.array([k[4] for k in keys], np.int64), \
+        np.array([k[0] for k in keys], np.int64)
+
+
+def test_doublets_match_all_pairs_search(small_events):
+    for name, o, _ in small_events:
+        inner, outer, pair = _doublets_bruteforce(o)
+        assert o["nCells"] == inner.size, name
+        assert np.array_equal(o["cellInner"], inner), name
+        assert np.array_equal(o["cellOuter"], outer), name
+        assert np.array_equal(o["cellLayerPair"], pair), name
+
+
+def test_outer_hit_lists(small_events):
+    for name, o, _ in small_events:
+        off, cells = o["outerHitOff"], o["outerHitCells"]
+        outer = o["cellOuter"]
+        for h in range(o["nHits"]):
+            want = np.nonzero(outer == h)[0][:128]
+            assert np.array_equal(cells[off[h]:off[h + 1]], want), (name, h)
+
+
+def _aligned(r1, z1, ri, zi, ro, zo, ptmin, thcut):
+    rd = np.abs(r1 - ro)
+    d13 = rd * rd + (z1 - zo) * (z1 - zo)
+    pmin = ptmin * np.sqrt(d13)
+    tan = np.abs(z1 * (ri - ro) + zi * (ro - r1) + zo * (r1 - ri))
+    return tan * pmin <= thcut * d13 * rd
+
+
+def test_neighbours_match_all_cell_pairs(small_events, oracle):
+    """Every (outer cell, inner cell) sharing a hit, cut with the CA's rz alignment and circle."""
+    for name, o, _ in small_events:
+        x, y, z, r, det = o["xg"], o["yg"], o["zg"], o["rg"], o["detInd"]
+        inner, outer = o["cellInner"], o["cellOuter"]
+        nb_off, nb = o["cellNeighborOff"], o["cellNeighbors"]
+        used = np.zeros(inner.size, bool)
+        by_inner = {}
+        for c, h in enumerate(inner):
+            by_inner.setdefault(int(h), []).append(c)
+        for oc in range(inner.size):
+            want = []
+            for me in by_inner.get(int(outer[oc]), []):
+                hi, hm, ho = inner[oc], inner[me], outer[me]
+                barrel = det[hm] < 1184
+                if not _aligned(r[hi], z[hi], r[hm], z[hm], r[ho], z[ho], F(0.899999976158),
+                                F(0.00200000009499) if barrel else F(0.00300000002608)):
+                    continue
+                c, dca0 = oracle.circle(x[hi], y[hi], x[hm], y[hm], x[ho], y[ho])
+                if c > F(0.0328407224959):
+                    continue
+                cut = F(0.15000000596) if det[hi] < 96 else F(0.25)
+                if not abs(F(dca0)) < cut * abs(F(c)):
+                    continue
+                want.append(me)
+                used[me] = used[oc] = True
+            got = nb[nb_off[oc]:nb_off[oc + 1]]
+            assert list(got) == want[:36], (name, oc)
+        assert np.array_equal(o["cellUsed"] != 0, used), name
+
+
+def test_circle_matches_circumcircle(oracle):
+    """Three points on a circle of radius R whose distance to the origin is d, as the CA sees them
+    (detector radii, pT 0.5-50 GeV): |c| = 1/R, sign of c against the turning direction, |dca0/c| = d."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        R = rng.uniform(40, 4000)
+        d = rng.uniform(-0.5, 0.5)
+        phi = rng.uniform(-np.pi, np.pi)
+        # centre at distance R + d from the origin
+        cx, cy = (R + d) * np.cos(phi), (R + d) * np.sin(phi)
+        pts = []
+        for rr in (2.9, 6.8, 10.9):
+            # intersection of the circle with the cylinder of radius rr (the branch near the origin)
+            D = R + d
+            cosA = (D * D + rr * rr - R * R) / (2 * D * rr)
+            a = phi + np.arccos(np.clip(cosA, -1, 1))
+            pts.append((rr * np.cos(a), rr * np.sin(a)))
+        (x1, y1), (x2, y2), (x3, y3) = pts
+        c, dca0 = oracle.circle(x1, y1, x2, y2, x3, y3)
+        turn = np.sign((x2 - x1) * (y3 - y2) - (y2 - y1) * (x3 - x2))
+        assert abs(abs(c) * R - 1) < 1e-2 + 2e-5 * R, (R, c)
+        assert np.sign(c) == -turn
+        assert abs(abs(dca0 / c) - abs(d)) < 1e-3 + 1e-6 * R * R / 100, (R, d, dca0 / c)
+
+
+def _walk(o):
+    """Exhaustive depth-first walk of the cell graph from the serial program's start cells."""
+    nb_off, nb = o["cellNeighborOff"], o["cellNeighbors"]
+    alive = o["cellDoubletId"] >= 0
+    inner, outer, pair = o["cellInner"], o["cellOuter"], o["cellLayerPair"]
+    tuples = []
+
+    def visit(stack):
+        c = stack[-1]
+        nxt = [n for n in nb[nb_off[c]:nb_off[c + 1]] if alive[n]]
+        for n in nxt:
+            visit(stack + [int(n)])
+        if not nxt and len(stack) >= 2:
+            tuples.append([int(inner[s]) for s in stack] + [int(outer[c])])
+
+    for c in range(inner.size):
+        if alive[c] and not (8 <= pair[c] <= 12):
+            visit([c])
+    return tuples
+
+
+def test_ntuplets_match_exhaustive_walk(small_events):
+    for name, o, _ in small_events:
+        want = _walk(o)
+        off, bins = o["tupleOff"], o["tupleBins"]
+        got = [list(bins[off[t]:off[t + 1]]) for t in range(o["nTuples"])]
+        assert got == want, name
+        assert off[o["nTuples"]] == off[-1], name
+
+
+def test_cell_track_lists_and_containers(small_events):
+    for name, o, _ in small_events:
+        nt = o["nTuples"]
+        off, bins = o["tupleOff"], o["tupleBins"]
+        inner, outer = o["cellInner"], o["cellOuter"]
+        cell_of = {(int(a), int(b)): c for c, (a, b) in enumerate(zip(inner, outer))}
+        want = [[] for _ in range(inner.size)]
+        for t in range(nt):
+            h = bins[off[t]:off[t + 1]]
+            for k in range(len(h) - 1):
+                want[cell_of[(int(h[k]), int(h[k + 1]))]].append(t)
+        toff, tr = o["cellTrackOff"], o["cellTracks"]
+        for c in range(inner.size):
+            assert list(tr[toff[c]:toff[c + 1]]) == want[c][:48], (name, c)
+        # multiplicity: non-duplicate tuples by size, descending index within a size
+        q = o["quality"]
+        size = np.diff(off[:nt + 1])
+        moff, mb = o["multOff"], o["multBins"]
+        for nh in range(8):
+            sel = [t for t in range(nt) if size[t] == nh and nh >= 3 and q_before_fit_ok(o, t)]
+            assert list(mb[moff[nh]:moff[nh + 1]]) == sel[::-1], (name, nh)
+        # detector indices follow the hit indices
+        assert np.array_equal(o["detIndices"], o["detInd"][bins[:off[nt]]]), name
+        # hit -> loose tuples, descending tuple index within a hit
+        hoff, hb = o["hitToTupleOff"], o["hitToTupleBins"]
+        per_hit = {}
+        for t in hit_to_tuple_sources(o):
+            for h in bins[off[t]:off[t + 1]]:
+                per_hit.setdefault(int(h), []).append(t)
+        for h in range(o["nHits"]):
+            assert list(hb[hoff[h]:hoff[h + 1]]) == per_hit.get(h, [])[::-1], (name, h)
+
+
+def q_before_fit_ok(o, t):
+    """A tuple is in the multiplicity container unless the early duplicate remover marked it."""
+    return t not in early_dups(o)
+
+
+def early_dups(o):
+    key = id(o)
+    if key not in _cache:
+        nt = o["nTuples"]
+        off = o["tupleOff"]
+        size = np.diff(off[:nt + 1])
+        toff, tr = o["cellTrackOff"], o["cellTracks"]
+        dups = set()
+        for c in range(o["nCells"]):
+            ts = tr[toff[c]:toff[c + 1]]
+            if ts.size >= 2:
+                m = size[ts].max()
+                dups.update(int(t) for t in ts if size[t] != m)
+        _cache[key] = dups
+    return _cache[key]
+
+
+_cache = {}
+
+
+def hit_to_tuple_sources(o):
+    """Loose tuples after the fast duplicate remover, recomputed from the classification inputs."""
+    nt = o["nTuples"]
+    off = o["tupleOff"]
+    size = np.diff(off[:nt + 1])
+    st, pt, chi2 = o["state"].reshape(-1, 5), o["pt"], o["chi2"]
+    coeff = [F(0.68177776), F(0.74609577), F(-0.08035491), F(0.00315399)]
+    dups = early_dups(o)
+    q = np.zeros(nt, np.int64)
+    for t in range(nt):
+        if t in dups:
+            q[t] = 1
+            continue
+        if size[t] < 3 or np.isnan(st[t]).any():
+            continue
+        p = min(pt[t], F(10))
+        cut = F(30) * (coeff[0] + p * (coeff[1] + p * (coeff[2] + p * coeff[3])))
+        if F(3) * chi2[t] >= cut:
+            continue
+        quad = size[t] > 3
+        if abs(st[t, 1]) < (F(0.5) if quad else F(0.3)) and pt[t] > (F(0.3) if quad else F(0.5)) and \
+                abs(st[t, 4]) < F(12):
+            q[t] = 2
+    # fast duplicate remover, serial over cells
+    toff, tr = o["cellTrackOff"], o["cellTracks"]
+    for c in range(o["nCells"]):
+        ts = tr[toff[c]:toff[c + 1]]
+        if ts.size < 2:
+            continue
+        best, bt = F(10000), None
+        for t in ts:
+            if q[t] == 2 and abs(st[t, 1]) < best:
+                best, bt = abs(st[t, 1]), t
+        for t in ts:
+            if q[t] == 2 and t != bt:
+                q[t] = 1
+    return [t for t in range(nt) if q[t] == 2]
+
+
+def _helix_points(pt, phi0, cot, z0, q, radii, B):
+    """Points of a helix from (0,0,z0) with transverse momentum pt (GeV) in field B (GeV/cm units);
+    with the field along +z a positive charge turns clockwise seen from +z."""
+    R = pt / B
+    # centre at distance R from the origin, to the right of the initial direction for q > 0
+    cx, cy = q * R * math.sin(phi0), -q * R * math.cos(phi0)
+    pts = []
+    for rr in radii:
+        # angle turned to reach transverse radius rr: chord 2R sin(a/2) = rr
+        a = 2 * math.asin(rr / (2 * R))
+        ang0 = math.atan2(-cy, -cx)
+        ang = ang0 - q * a
+        x, y = cx + R * math.cos(ang), cy + R * math.sin(ang)
+        s = R * a
+        pts.append((x, y, z0 + s * cot))
+    return np.array(pts).T
+
+
+@pytest.mark.parametrize("n", [3, 4])
+@pytest.mark.parametrize("pt,phi0,cot,z0,q", [(1.0, 0.3, 0.4, 1.2, 1), (5.0, -2.0, -1.1, -3.0, -1),
+                                              (0.7, 2.5, 2.0, 0.5, 1), (20.0, 1.0, 0.0, 0.0, -1)])
+def test_fit_recovers_noiseless_helix(oracle, n, pt, phi0, cot, z0, q):
+    B = float(F(0.0114256972711507))
+    hits = _helix_points(pt, phi0, cot, z0, q, [2.9, 6.8, 10.9, 16.0][:n], B)
+    ge = np.zeros((6, n), np.float32)
+    ge[0] = ge[2] = 1e-7
+    ge[5] = 4e-6
+    st, cov, fpt, eta, chi2 = oracle.fit_helix(hits, ge, B)
+    # state: phi at the point of closest approach, transverse impact parameter, q/pt, cot(theta), z
+    assert abs(math.remainder(st[0] - phi0, 2 * math.pi)) < 1e-4
+    assert abs(st[1]) < 1e-4
+    assert abs(st[2] * pt * q - 1) < 1e-3
+    assert abs(st[3] - cot) < 1e-4
+    assert abs(st[4] - z0) < 1e-4
+    assert abs(fpt / pt - 1) < 1e-3
+    assert abs(eta - math.asinh(cot)) < 1e-4
+    assert chi2 < 1e-3
+    C5 = np.zeros((5, 5))
+    C5[np.triu_indices(5)] = cov  # packed upper triangle, row by row
+    C5 = C5 + np.triu(C5, 1).T
+    assert np.all(np.diag(C5) >= 0)
+    circ = C5[:3, :3]
+    assert np.all(np.linalg.eigvalsh(circ) > -1e-12 * np.abs(circ).max())
+
+
+def test_fit_is_symmetric_under_charge_and_rotation(oracle):
+    B = float(F(0.0114256972711507))
+    base = _helix_points(2.0, 0.5, 0.3, 0.0, 1, [2.9, 6.8, 10.9, 16.0], B)
+    ge = np.zeros((6, 4), np.float32)
+    ge[0] = ge[2] = 1e-6
+    ge[5] = 1e-5
+    s1 = oracle.fit_helix(base, ge, B)[0]
+    mirrored = base * np.array([[1], [-1], [1]])  # reflection y -> -y flips charge and phi
+    s2 = oracle.fit_helix(mirrored, ge, B)[0]
+    assert abs(s1[0] + s2[0]) < 1e-5 and abs(s1[2] + s2[2]) < 1e-4 * abs(s1[2])
+    assert abs(s1[3] - s2[3]) < 1e-6 and abs(s1[4] - s2[4]) < 1e-6
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6])
+def test_spd_inverse(oracle, n):
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n))
+    m = a @ a.T + n * np.eye(n)
+    inv = oracle.invert_spd(m)
+    assert np.allclose(inv @ m, np.eye(n), atol=1e-12)
+
+
+def test_vertices_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    zs, ez2, p2 = [], [], []
+    truth = [-5.0, -1.0, 2.5, 7.0]
+    for k, zt in enumerate(truth):
+        for _ in range(5 + k):
+            zs.append(zt + rng.normal(0, 0.003))
+            ez2.append(rng.uniform(0.5e-5, 1.5e-5))
+            p2.append(rng.uniform(0.3, 4.0) ** 2)
+    zs, ez2, p2 = (np.array(v, np.float32) for v in (zs, ez2, p2))
+    r = oracle.vertices(zs, ez2, p2)
+    assert r["nv"] == len(truth)
+    for k in range(r["nv"]):
+        m = r["iv"] == k
+        w = 1.0 / ez2[m].astype(np.float64)
+        zv = (zs[m] * w).sum() / w.sum()
+        chi2 = ((zs[m] - zv) ** 2 * w).sum()
+        assert abs(r["zv"][k] - zv) < 1e-5
+        assert abs(r["chi2"][k] / chi2 - 1) < 1e-3
+        assert r["ndof"][k] == m.sum() - 1
+        assert abs(r["wv"][k] / (w.sum() * (m.sum() - 1) / chi2) - 1) < 1e-3
+        assert abs(r["ptv2"][k] / p2[m].sum() - 1) < 1e-5
+    # sorted by increasing sum pt^2
+    assert np.all(np.diff(r["ptv2"][r["sortInd"]]) >= 0)
+    # every track is assigned to the vertex nearest its z
+    near = np.argmin(np.abs(zs[:, None] - r["zv"][None, :]), axis=1)
+    assert np.array_equal(near, r["iv"])
+
+
+def test_vertex_split(oracle):
+    """Two vertices 0.04 cm apart are linked into one density cluster (checked here by single
+    linkage) whose chi2 is too large, and the split step separates them."""
+    rng = np.random.default_rng(5)
+    zs = np.concatenate([rng.normal(1.0, 0.006, 12), rng.normal(1.04, 0.006, 12)]).astype(np.float32)
+    ez2 = np.full(zs.size, 2.5e-5, np.float32)
+    link = np.abs(zs[:, None] - zs[None, :]) <= np.minimum(0.07, 3 * np.sqrt(ez2[:, None] + ez2[None, :]))
+    comp = link.copy()
+    for _ in range(zs.size):
+        comp = (comp.astype(np.int32) @ link.astype(np.int32)) > 0
+    assert comp.all(), "inputs must form one density cluster"
+    r = oracle.vertices(zs, ez2, np.ones(zs.size, np.float32))
+    assert r["nv"] == 2
+    got = np.sort(r["zv"])
+    assert abs(got[0] - 1.0) < 0.006 and abs(got[1] - 1.04) < 0.006
+
+
+def test_chain_finds_generated_vertices(small_events):
+    for name, o, truth in small_events:
+        nv = o["nVertices"]
+        assert nv >= 1, name
+        tz = np.array(truth.vertex_z[:truth.n_vertices]) - 0.2  # beam spot z
+        d = np.abs(o["zv"][:nv, None] - tz[None, :]).min(axis=1)
+        assert np.median(d) < 0.05, name
