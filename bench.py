@@ -3,8 +3,9 @@
 
 Workload: 1000 synthetic PU50 events (50 vertices x Poisson(35) tracks, 1/pT^2
 spectrum above 0.5 GeV, ~50k digis and ~13k clusters per event), words
-resident in HBM before the timed region.  One step = one pass of the chain
-over all events.  The batched path runs B events per launch of each stage;
+resident in HBM before the timed region.  One step = one pass of the whole
+chain (raw words -> digis -> clusters -> rec-hits -> ntuplets with
+broken-line fits -> vertices) over all events.  The batched path runs B events per launch of each stage;
 the plain path runs one launch sequence per event with the reference's
 per-event host synchronisation.  Prints one JSON line.
 """
@@ -21,24 +22,62 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "pixeltrack-standalone_amd"))
 
 METRIC = "events/s at PU50-like load; p50/p99 latency, batched vs plain per-event GPU"
+STAGES = {1: "raw->digis", 2: "raw->clusters", 3: "raw->rec-hits", 4: "raw->ntuplets+fits",
+          5: "raw->ntuplets+fits->vertices"}
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per kernel launch (unique input read + output written with the
-# reference element widths), from the per-event counters W (words), D (digis),
-# H (hits), M (module runs).  decode: words 4 B + fedIds 0.5 B per word in, seven
-# digi arrays (20 B) out, ~2 B of gain table per digi.  clusterize: x, y, adc,
-# moduleInd in (8 B), clus + moduleInd out (6 B).  hits: moduleInd, clus, x, y,
-# adc in (12 B per digi), 44 B per hit out.  moduleStarts: 16 B of ballot masks
-# per 64 words in, 4 B per module run out.  phiBinner: iphi in, bin contents out
-# (4 B per hit) + 5 kB of offsets.
-def algorithmic_bytes(kernel, W, D, H, M):
-    return {
+# Algorithmic bytes per event of each kernel: the unique input it must read and
+# the output it must write, with the reference's element widths, from the mean
+# per-event counters (c): W words, D digis, H hits, M module runs, S doublet
+# slots (layer pair x inner hit), C cells, N neighbour entries, T tuples, TH
+# tuple hits, F fitted tuples, V vertex-finder tracks.  Per hit the CA kernels
+# read x, y, z, r (16 B), iphi, ysize, detInd (6 B); a cell is inner + outer hit
+# (4 B), pair and flags (2 B), list slots and counts (13 B).
+def algorithmic_bytes(kernel, c):
+    W, D, H, M = c["W"], c["D"], c["H"], c["M"]
+    S, C, N, T, TH, F, V = c["S"], c["C"], c["N"], c["T"], c["TH"], c["F"], c["V"]
+    table = {
+        # stage A/B: decode reads words (4 B) + fedIds (0.5 B) per word, writes seven digi
+        # arrays (20 B) and reads ~2 B of gains per digi; clusterize reads x, y, adc,
+        # moduleInd (8 B) and writes clus + moduleInd (6 B); hits reads 12 B per digi and
+        # writes 44 B per hit; moduleStarts reads 16 B of masks per 64 words and writes
+        # 4 B per module run; phiBinner reads iphi and writes the bin contents + offsets
         "decode": 24.5 * W + 2 * D,
         "moduleStarts": W / 64 * 16 + 4 * M,
         "clusterize": 14 * D,
         "hits": 12 * D + 44 * H,
         "phiBinner": 4 * H + 5124,
-    }[kernel]
+        # doublets: hits (z, r, iphi, ysize, detInd: 14 B) + phi histogram (2 B per hit
+        # + 5 kB) in, one count (4 B) per slot out; the fill pass writes the cells
+        "doubletCount": 16 * H + 5124 + 4 * S,
+        "doubletScan": 8 * S,
+        "doubletFill": 16 * H + 5124 + 4 * S + 19 * C + 4 * H,
+        "outerHitScan": 8 * H,
+        "outerHitFill": 2 * C + 4 * H + 4 * C,
+        "outerHitSort": 8 * C + C,
+        # connect: every cell (6 B), the hits' x, y, z, r, detInd (18 B), the slot offsets,
+        # the neighbour lists written (4 B per entry) and the used bits
+        "connect": 6 * C + 18 * H + 4 * S + 4 * N + 5 * C + C / 8,
+        "fishbone": 4 * C + 2 * C + 16 * H + C / 8 + C,
+        "ntupletCount": 7 * C + 4 * N + 8 * C,
+        "ntupletScan": 16 * C,
+        "ntupletFill": 7 * C + 4 * N + 8 * C + 4 * T + 2 * TH + 16 * T + T,
+        "trackListAlloc": 12 * C,
+        "trackListFill": 16 * T + 8 * C + 2 * TH,
+        "trackListSort": 8 * C + 2 * TH + 4 * T + T,
+        "multiplicity": 5 * T + 2 * T + 4 * TH + 92 * T,
+        # fit: per fitted tuple the offsets and hit indices, per hit x, y, z, xerr, yerr,
+        # detInd (22 B) and its 36 B local frame, 92 B of results out
+        "fit": F * (4 + 2 * 4 + 4 * (22 + 36) + 92),
+        "classify": T * (4 + 1 + 4 * 5 + 12 + 1),
+        "fastDuplicate": 8 * C + 2 * TH + 4 * T + T,
+        "hitToTupleCount": T * 5 + 2 * TH + 4 * H,
+        "hitToTupleScan": 12 * H,
+        "hitToTupleFill": T * 5 + 2 * TH + 8 * H + 2 * TH,
+        "tripletCleaner": 8 * H + 2 * TH + 4 * T + T,
+        "vertices": T * (4 + 1 + 4 + 4 + 4) + V * 24 + 4096 * 5,
+    }
+    return table.get(kernel)
 
 
 def gen_events(detector, cfg, n, threads):
@@ -57,6 +96,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: each box here has one MI355X and events shard across GPUs "
@@ -85,7 +125,7 @@ def main():
 
     def batched_pass():
         for first in range(0, n, B):
-            stream.run(first, min(B, n - first), 3)
+            stream.run(first, min(B, n - first), args.upto)
         return stream.fetch_counts(0, n)
 
     # ---- batched path: warmup, then K timed steps (one step = all events)
@@ -108,29 +148,38 @@ def main():
     evps = n * args.steps / elapsed
 
     # per-event counters for the algorithmic bytes
-    D = W.astype(np.float64)
     cnts = np.array([stream.event_counts(e) for e in range(n)], dtype=np.float64)
-    Mmean, Hmean = float(cnts[:, 0].mean()), float(cnts[:, 1].mean())
+    mean = {"W": float(W.mean()), "D": float(W.mean()), "M": float(cnts[:, 0].mean()), "H": float(cnts[:, 1].mean())}
+    if args.upto >= 4:
+        cac = [stream.ca_counts(e) for e in range(n)]
+        for key, name in (("S", "nSlots"), ("C", "nCells"), ("N", "nNeighbors"), ("T", "nTuples"), ("TH", "apcN"),
+                          ("F", "nMult"), ("V", "vtxTracks")):
+            mean[key] = float(np.mean([c[name] for c in cac]))
+        mean["vertices"] = float(np.mean([c["nVertices"] for c in cac]))
+    else:
+        for key in ("S", "C", "N", "T", "TH", "F", "V"):
+            mean[key] = 0.0
     launches_per_kernel = {k: v[0] for k, v in prof.items()}
     kern = max(prof, key=lambda k: prof[k][1])
     calls, total_ms = prof[kern]
     avg_s = total_ms / 1e3 / calls
     events_per_launch = n * args.steps / calls
-    alg = algorithmic_bytes(kern, W.mean(), D.mean(), Hmean, Mmean) * events_per_launch
+    alg = algorithmic_bytes(kern, mean) * events_per_launch
     achieved = alg / avg_s / 1e9
-    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in prof.items()}
+    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    share = {k: round(v[1] / sum(x[1] for x in prof.values()), 4) for k, v in prof.items()}
 
     # ---- plain per-event path: one launch sequence + host sync per event
     plain = None
     if not args.no_plain:
         lat = []
         for e in range(min(n, 50)):  # warmup
-            stream.run(e, 1, 3)
+            stream.run(e, 1, args.upto)
             stream.fetch_counts(e, 1)
         t1 = time.perf_counter()
         for e in range(n):
             a = time.perf_counter()
-            stream.run(e, 1, 3)
+            stream.run(e, 1, args.upto)
             stream.fetch_counts(e, 1)
             lat.append(time.perf_counter() - a)
         tp = time.perf_counter() - t1
@@ -148,14 +197,14 @@ def main():
         sample = raws[:100]
         ncores = min(16, os.cpu_count() or 1)
         a = time.perf_counter()
-        done = orc.run_many(sample, upto=3, threads=ncores, repeat=1)
+        done = orc.run_many(sample, upto=args.upto, threads=ncores, repeat=1)
         one = time.perf_counter() - a
         rep = max(1, int(args.cpu_seconds / max(one, 1e-3)))
         a = time.perf_counter()
-        done = orc.run_many(sample, upto=3, threads=ncores, repeat=rep)
+        done = orc.run_many(sample, upto=args.upto, threads=ncores, repeat=rep)
         ct = time.perf_counter() - a
         cpu = {"value": round(done / ct, 1), "unit": "events/s", "cores": ncores, "kind": "port",
-               "sample": f"{len(sample)} of the PU50 events x {rep} passes, raw->hits, one event per thread "
+               "sample": f"{len(sample)} of the PU50 events x {rep} passes, {STAGES[args.upto]}, one event per thread "
                          f"at a time ({ncores} threads), {ct:.1f} s"}
 
     batch_lat_ms = ms_per_step / ((n + B - 1) // B)
@@ -170,17 +219,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32/f32",
+        "dtype": "int32/f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"{n} PU50 events (stages raw->digis->clusters->rec-hits), batched path, "
-                               f"{B} events per launch",
+        "config": {"workload": f"{n} PU50 events ({STAGES[args.upto]}), batched path, {B} events per launch",
                    "events": n, "batch": B, "mean_words": round(float(W.mean()), 1),
-                   "mean_clusters": round(Hmean, 1), "stages": "A+B"},
+                   "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
+                   "stages": STAGES[args.upto]},
         "batched": {"events_per_s": round(evps, 1), "batch_latency_ms": round(batch_lat_ms, 4),
                     "status": status},
         "plain": plain,
         "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
         "kernel_ms_per_launch": kernel_ms,
+        "kernel_time_share": share,
         "launches": launches_per_kernel,
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,

@@ -30,6 +30,10 @@ enum {
   PXG_STATUS_PIXEL_OVERFLOW = 2,   /* a module with > 4000 pixels */
   PXG_STATUS_CLUSTER_OVERFLOW = 4, /* a module with > 1024 clusters */
   PXG_STATUS_HIT_OVERFLOW = 8,     /* > 49152 clusters in an event */
+  /* ntuplet stage: the container bits of pxformats.h (kCa*) shifted by 8 */
+  PXG_STATUS_CA_SHIFT = 8,
+  /* vertex stage: more than 1024 vertices */
+  PXG_STATUS_VERTEX_OVERFLOW = 1 << 20,
   PXG_ERR_HIP = -1,
   PXG_ERR_ARG = -2,
   PXG_ERR_CAPACITY = -3,
@@ -91,6 +95,46 @@ typedef struct {
   uint32_t nHits;
 } pxg_hits;
 
+/* PixelTrackHeterogeneous / TrackSoAT (CUDADataFormats/PixelTrackHeterogeneous.h:13-70):
+ * tuples as an offsets/contents pair (hitIndices), per-tuple fit results at index t,
+ * state and covariance as [component][stride] (TrajectoryStateSoA.h:8-60). */
+typedef struct {
+  uint32_t* hitOff;      /* [stride + 1] */
+  uint16_t* hitIndices;  /* [5 * stride] */
+  uint16_t* detIndices;  /* [5 * stride] */
+  uint8_t* quality;      /* trackQuality: 0 bad, 1 dup, 2 loose */
+  float *chi2, *eta, *pt;
+  float* state;          /* [5][stride]: phi, tip, q/pt, cot(theta), zip */
+  float* covariance;     /* [15][stride]: packed upper triangle, row by row */
+  uint32_t stride;       /* tuple capacity */
+  uint32_t nTuples;
+} pxg_tracks;
+
+/* ZVertexSoA (CUDADataFormats/ZVertexSoA.h:10-26) */
+typedef struct {
+  int16_t* idv;          /* [stride]: vertex of each track, -1 none */
+  float *zv, *wv, *chi2, *ptv2;  /* [1024] */
+  int32_t* ndof;         /* [stride]; the first nvFinal entries are per vertex */
+  uint16_t* sortInd;     /* [1024]: vertices by increasing sum pt^2 */
+  uint32_t nvFinal;
+} pxg_vertices;
+
+/* Internal containers of the ntuplet stage, exposed for parity checks
+ * (GPUCACell, CAConstants.h containers).  Lists are (slot, count) pairs into a pool. */
+typedef struct {
+  uint16_t *cellInner, *cellOuter;
+  uint8_t* cellPair;
+  uint8_t* cellFlags;    /* bit0 listed in its outer hit's list, bit1 killed by the fishbone */
+  uint32_t* cellUsed;    /* one bit per cell */
+  uint32_t *outerHitOff, *outerHitCells;   /* cells by outer hit, ascending */
+  int32_t* neighborSlot; uint8_t* neighborCount; uint32_t* neighborPool;  /* 36 per slot */
+  int32_t* trackSlot; uint32_t* trackCount; uint16_t* trackPool;          /* 48 per slot */
+  uint32_t* multOff; uint16_t* multBins;   /* tuple multiplicity */
+  uint32_t *hitToTupleOff; uint16_t* hitToTupleBins;
+  uint32_t counts[16];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,
+                            nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors */
+} pxg_ca_internals;
+
 /* Context: device + conditions uploaded once, immutable afterwards (pxg_init is the
  * ESProducer -> device transfer the CUDA program does in its ES products). */
 int pxg_init(const pxg_conditions* cond, pxg_context** out);
@@ -102,6 +146,11 @@ int pxg_device_name(char* buf, size_t n);
  * max_words raw words in total (one per EDM stream; the pool replaces the caching
  * allocator of src/cuda/CUDACore). */
 int pxg_stream_create(pxg_context* ctx, uint32_t max_events, uint32_t max_words, pxg_stream** out);
+/* Same, choosing the capacities of the ntuplet containers: PXG_PROFILE_REFERENCE (the serial
+ * program's, CAConstants.h:21-64) or PXG_PROFILE_HIGH_OCCUPANCY (scaled for PU200-like events). */
+enum { PXG_PROFILE_REFERENCE = 0, PXG_PROFILE_HIGH_OCCUPANCY = 1 };
+int pxg_stream_create_profile(pxg_context* ctx, uint32_t max_events, uint32_t max_words, int profile,
+                              pxg_stream** out);
 void pxg_stream_destroy(pxg_stream* s);
 int pxg_sync(pxg_stream* s);
 
@@ -124,6 +173,16 @@ int pxg_raw_to_cluster(pxg_stream* s, const uint32_t* words, const uint8_t* fedI
 int pxg_rechits(pxg_stream* s, const pxg_digis* digis, const pxg_clusters* clusters, const pxg_beamspot* bs,
                 pxg_hits* hits);
 
+/* Rec-hits -> ntuplets with broken-line fits and quality (CAHitNtupletGeneratorOnGPU::makeTuples,
+ * CAHitNtupletGeneratorOnGPU.cc:96-126; CAHitNtupletCUDA::produce, CAHitNtupletCUDA.cc:38-47); the
+ * hits must be the products of the last pxg_rechits on the same stream. */
+int pxg_ntuplets(pxg_stream* s, const pxg_hits* hits, pxg_tracks* tracks);
+
+/* Tracks -> vertices (PixelVertexProducerCUDA::produce, PixelVertexProducerCUDA.cc:26-40;
+ * gpuVertexFinder::Producer::make, gpuVertexFinderImpl.h:84-111); the tracks must be the
+ * products of the last pxg_ntuplets on the same stream. */
+int pxg_find_vertices(pxg_stream* s, const pxg_tracks* tracks, pxg_vertices* vertices);
+
 /* ---- resident / batched path --------------------------------------------------------- */
 
 typedef struct {
@@ -138,7 +197,7 @@ int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const p
 
 /* Enqueues the chain on resident events [first, first+count) as ONE launch per stage
  * (count = 1 is the plain per-event launch sequence).  upto: 1 digis, 2 clusters,
- * 3 rec-hits.  Asynchronous; pxg_fetch_counts synchronises. */
+ * 3 rec-hits, 4 ntuplets and fits, 5 vertices.  Asynchronous; pxg_fetch_counts synchronises. */
 int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto);
 
 /* Copies the per-event counters of [first, first+count) to the host and synchronises;
@@ -152,6 +211,12 @@ int pxg_event_products(pxg_stream* s, uint32_t e, pxg_digis* digis, pxg_clusters
 /* Host copies of resident event e's counters (after pxg_fetch_counts covering e):
  * out[0] module runs, out[1] clusters (= hits), out[2] error records, out[3] status bits. */
 int pxg_event_counts(pxg_stream* s, uint32_t e, uint32_t out[4]);
+
+/* Track and vertex views of resident event e (after pxg_fetch_counts covering e). */
+int pxg_event_tracks(pxg_stream* s, uint32_t e, pxg_tracks* tracks, pxg_vertices* vertices);
+
+/* Views of the ntuplet stage's internal containers of resident event e. */
+int pxg_event_ca_internals(pxg_stream* s, uint32_t e, pxg_ca_internals* out);
 
 /* Device -> host copy of product memory (validation, --transfer). */
 int pxg_copy_to_host(pxg_stream* s, void* dst, const void* src, size_t bytes);

@@ -48,21 +48,7 @@ namespace orc {
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
   // run-time ones are in Event::caps (pxformats.h CACapacities)
   constexpr int kMaxCellsPerHit = 128;
-  constexpr uint32_t kMaxVertices = 1024;
   constexpr double kBField = 0.0114256972711507;  // CAHitNtupletCUDA.cc:31
-  enum : uint8_t { kQualityBad = 0, kQualityDup = 1, kQualityLoose = 2 };
-  enum : uint32_t {
-    kCaCellOverflow = 1u << 0,
-    kCaOuterHitOverflow = 1u << 1,
-    kCaNeighborOverflow = 1u << 2,
-    kCaNeighborStoreOverflow = 1u << 3,
-    kCaTrackOverflow = 1u << 4,
-    kCaTrackStoreOverflow = 1u << 5,
-    kCaTupleOverflow = 1u << 6,
-    kCaDepthOverflow = 1u << 7,
-    kCaHitToTupleOverflow = 1u << 8,
-    kCaMultiplicityOverflow = 1u << 9,
-  };
 
   template <class T, int N>
   struct Vec {

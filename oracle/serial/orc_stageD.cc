@@ -147,8 +147,10 @@ namespace orc {
         V.wv[i] = 0;
         V.chi2[i] = 0;
       }
+      // (vertex ids beyond the container, possible only when it overflowed,
+      // count as unassigned)
       for (uint32_t i = 0; i < nt; ++i) {
-        if (iv[i] > 9990)
+        if (iv[i] > 9990 || iv[i] >= int32_t(nc))
           continue;
         float w = 1.f / ezt2[i];
         V.zv[iv[i]] += zt[i] * w;
@@ -159,7 +161,7 @@ namespace orc {
         nn[i] = -1;
       }
       for (uint32_t i = 0; i < nt; ++i) {
-        if (iv[i] > 9990)
+        if (iv[i] > 9990 || iv[i] >= int32_t(nc))
           continue;
         float c2 = V.zv[iv[i]] - zt[i];
         c2 *= c2 / ezt2[i];
@@ -178,7 +180,7 @@ namespace orc {
 
     // splitVertices (gpuSplitVertices.h:15-125)
     const float maxChi2 = 9.f;
-    for (uint32_t kv = 0; kv < nvFinal; ++kv) {
+    for (uint32_t kv = 0; kv < nvFinal && kv < kMaxVertices; ++kv) {
       if (nn[kv] < 4)
         continue;
       if (V.chi2[kv] < maxChi2 * float(nn[kv]))
@@ -244,19 +246,20 @@ namespace orc {
     if (nvFinal >= 1) {
       for (uint32_t i = 0; i < nt; ++i)
         V.idv[itrk[i]] = int16_t(iv[i]);
-      for (uint32_t i = 0; i < nvFinal; ++i)
+      const uint32_t nc = std::min(nvFinal, kMaxVertices);
+      for (uint32_t i = 0; i < nc; ++i)
         V.ptv2[i] = 0;
       for (uint32_t i = 0; i < nt; ++i) {
-        if (iv[i] > 9990)
+        if (iv[i] > 9990 || iv[i] >= int32_t(nc))
           continue;
         V.ptv2[iv[i]] += ptt2[i];
       }
       if (nvFinal == 1) {
         V.sortInd[0] = 0;
       } else {
-        for (uint16_t i = 0; i < nvFinal; ++i)
+        for (uint16_t i = 0; i < nc; ++i)
           V.sortInd[i] = i;
-        std::stable_sort(V.sortInd.begin(), V.sortInd.begin() + nvFinal,
+        std::stable_sort(V.sortInd.begin(), V.sortInd.begin() + nc,
                          [&](uint16_t a, uint16_t b) { return V.ptv2[a] < V.ptv2[b]; });
       }
     }

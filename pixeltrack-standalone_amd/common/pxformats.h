@@ -63,6 +63,26 @@ namespace pxf {
     uint32_t maxHitToTuple;  // hit -> tuple entries
     uint32_t maxCellLists;   // cell neighbour lists / cell track lists (each store)
   };
+  // Per-event status bits of the ntuplet stage (a set bit names a container
+  // that reached its capacity; what was stored is still well defined).
+  enum : uint32_t {
+    kCaCellOverflow = 1u << 0,           // more doublets than maxDoublets
+    kCaOuterHitOverflow = 1u << 1,       // a hit is the outer hit of more than 128 cells
+    kCaNeighborOverflow = 1u << 2,       // a cell with more than 36 outer neighbours
+    kCaNeighborStoreOverflow = 1u << 3,  // neighbour lists store full
+    kCaTrackOverflow = 1u << 4,          // a cell in more than 48 tuples
+    kCaTrackStoreOverflow = 1u << 5,     // track lists store full
+    kCaTupleOverflow = 1u << 6,          // more tuples than maxTuples
+    kCaDepthOverflow = 1u << 7,          // a chain longer than 4 cells
+    kCaHitToTupleOverflow = 1u << 8,     // hit -> tuple container full
+    kCaMultiplicityOverflow = 1u << 9,   // tuple multiplicity container full
+  };
+  constexpr uint32_t kCaMaxNeighbors = 36;      // CAConstants.h CellNeighbors
+  constexpr uint32_t kCaMaxCellTracks = 48;     // CAConstants.h CellTracks
+  constexpr uint32_t kCaMaxCellsPerHit = 128;   // CAConstants.h maxCellsPerHit
+  constexpr uint32_t kMaxVertices = 1024;       // ZVertexSoA.h MAXVTX
+  enum : uint8_t { kQualityBad = 0, kQualityDup = 1, kQualityLoose = 2 };
+
   constexpr CACapacities kCapsReference{458752, 32 * 1024, 24 * 1024, 4 * 24 * 1024, 64 * 1024};
   constexpr CACapacities kCapsHighOccupancy{4 * 458752, 56 * 1024, 56 * 1024, 4 * 56 * 1024, 256 * 1024};
 

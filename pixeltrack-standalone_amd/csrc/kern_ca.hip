@@ -1,0 +1,771 @@
+// Ntuplet stage on gfx950: doublets, outer-hit cell lists, CA connection,
+// fishbone, depth-first ntuplet search, per-cell track lists and the early
+// duplicate remover, tuple multiplicity and detector indices.
+//
+// Every product is the serial program's, element by element: where the serial
+// code appends to a container in loop order, the kernels here count first,
+// scan, and write each element at its serial position (doublets, tuples), or
+// append with atomics and then sort each short list (outer-hit lists, track
+// lists); kills and flags are written by exactly one thread.  Grid-stride
+// kernels take blockIdx.y = event and stride over that event's items with
+// gridDim.x workgroups; per-event scans run one workgroup per event.
+#include <hip/hip_runtime.h>
+
+#include "ca_device.h"
+
+namespace pxg {
+
+  namespace {
+    struct HitView {
+      const float *xg, *yg, *zg, *rg;
+      const int16_t *iphi, *ysize;
+      const uint16_t* detInd;
+      const uint32_t* layerStart;
+      const uint32_t* phiOff;
+      const uint16_t* phiBins;
+    };
+
+    __device__ __forceinline__ HitView hitsOf(const Pool& p, uint32_t e) {
+      const size_t h = size_t(e) * kMaxNumClusters;
+      return {p.xg + h,
+              p.yg + h,
+              p.zg + h,
+              p.rg + h,
+              p.iphi + h,
+              p.ysize + h,
+              p.detInd + h,
+              p.hitsLayerStart + size_t(e) * kLayerStartStride,
+              p.phiOff + size_t(e) * kPhiOffStride,
+              p.phiBins + h};
+    }
+
+    __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
+    __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
+
+    // ---- a11/a12 doublets: gpuPixelDoubletsAlgos.h:25-238 ----------------------------
+    // Visits the accepted outer hits of inner slot j (pair, inner hit) in the
+    // serial order: phi bins from the window's low edge upwards, each bin's
+    // hits in histogram order.
+    template <typename F>
+    __device__ __forceinline__ void forEachDoublet(const HitView& H, const PairTable& T, uint32_t j, F&& f) {
+      int pairId = 0;
+      while (j >= T.cum[pairId])
+        ++pairId;
+      const int inner = kPairInner[pairId], outer = kPairOuter[pairId];
+      const uint32_t i = j - (pairId ? T.cum[pairId - 1] : 0) + T.layer[inner];
+      if (H.detInd[i] > 2000)
+        return;
+      const float mez = H.zg[i];
+      if (mez < kPairMinZ[pairId] || mez > kPairMaxZ[pairId])
+        return;
+      constexpr int minYsizeB1 = 36, minYsizeB2 = 28, maxDYsize12 = 28, maxDYsize = 20, maxDYPred = 20;
+      constexpr float dzdrFact = 8 * 0.0285 / 0.015;
+      constexpr float z0cut = 12.f;
+      constexpr float minRadius = 0.5f * 87.78f;
+      constexpr float minRadius2T4 = 4.f * minRadius * minRadius;
+      constexpr float i2p = 3.14159265358979323846 / 32768;
+      const int16_t mes = H.ysize[i];  // ideal conditions: every ladder counts as an outer ladder
+      if (inner == 0 && outer > 3 && mes > 0 && mes < minYsizeB1)
+        return;
+      if (inner == 1 && outer > 3 && mes > 0 && mes < minYsizeB2)
+        return;
+      const int16_t mep = H.iphi[i];
+      const float mer = H.rg[i];
+      const int16_t iphicut = kPairPhiCut[pairId];
+      const float maxr = kPairMaxR[pairId];
+      const bool onlyBarrel = outer < 4;
+      const int dy = inner == 0 ? maxDYsize12 : maxDYsize;
+      const uint32_t hoff = kPhiBins * outer;
+      const int kl = phiBinOf(int16_t(mep - iphicut));
+      const int kh = phiBinOf(int16_t(mep + iphicut));
+      const int khh = (kh + 1) % int(kPhiBins);
+      for (int kk = kl; kk != khh; kk = (kk + 1) % int(kPhiBins)) {
+        const uint32_t b = kk + hoff;
+        const uint32_t pe = H.phiOff[b + 1];
+        for (uint32_t pp = H.phiOff[b]; pp < pe; ++pp) {
+          const uint32_t oi = H.phiBins[pp];
+          if (H.detInd[oi] > 2000)
+            continue;
+          const float zo = H.zg[oi], ro = H.rg[oi];
+          const float dr = ro - mer;
+          if (dr > maxr || dr < 0 || fabsf(mez * ro - mer * zo) > z0cut * dr)
+            continue;
+          const int16_t mop = H.iphi[oi];
+          const int a1 = abs(int(int16_t(mop - mep))), a2 = abs(int(int16_t(mep - mop)));
+          const uint16_t idphi = uint16_t(a1 < a2 ? a1 : a2);
+          if (idphi > iphicut)
+            continue;
+          const int16_t so = H.ysize[oi];
+          const bool zsize =
+              onlyBarrel ? (mes > 0 && so > 0 && abs(so - mes) > dy)
+                         : (inner < 4) && mes > 0 && abs(mes - int(fabsf((mez - zo) / (mer - ro)) * dzdrFact + 0.5f)) > maxDYPred;
+          if (zsize)
+            continue;
+          const float dphi = float(int16_t(idphi)) * i2p;
+          if (dphi * dphi * (minRadius2T4 - mer * ro) > (ro - mer) * (ro - mer))
+            continue;
+          f(i, oi, pairId);
+        }
+      }
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) doubletCountKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      __shared__ PairTable T;
+      const HitView H = hitsOf(p, e);
+      if (threadIdx.x == 0)
+        loadPairTable(H.layerStart, T);
+      __syncthreads();
+      uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      const uint32_t ntot = T.cum[kNumPairs - 1];
+      for (uint32_t j = gridStart(); j < ntot; j += gridStride()) {
+        uint32_t n = 0;
+        forEachDoublet(H, T, j, [&](uint32_t, uint32_t, int) { ++n; });
+        dOff[j] = n;
+      }
+    }
+
+    // one workgroup per event: offsets of the doublets, cell count, counter reset
+    __global__ void __launch_bounds__(kEventBlock) doubletScanKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t lds[kEventBlock / 64];
+      __shared__ PairTable T;
+      const uint32_t* ls = p.hitsLayerStart + size_t(e) * kLayerStartStride;
+      if (threadIdx.x == 0)
+        loadPairTable(ls, T);
+      __syncthreads();
+      uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      const uint32_t ntot = T.cum[kNumPairs - 1];
+      const uint32_t total = wgScanArray<kEventBlock>(dOff, ntot, lds);
+      const uint32_t nCells = min(total, ca.caps.maxDoublets);
+      const uint32_t nHits = T.layer[10];
+      // cleared for the following kernels
+      uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      for (uint32_t w = threadIdx.x; w < (nCells + 31) / 32; w += kEventBlock)
+        used[w] = 0;
+      uint32_t* ohc = ca.ohCount + size_t(e) * kHitStride;
+      uint32_t* h2c = ca.h2tCount + size_t(e) * kHitStride;
+      for (uint32_t h = threadIdx.x; h < nHits + 1; h += kEventBlock) {
+        ohc[h] = 0;
+        h2c[h] = 0;
+      }
+      if (threadIdx.x == 0) {
+        CACounts c{};
+        c.nCells = nCells;
+        c.nDoublets = total;
+        c.nSlots = ntot;
+        c.status = total > ca.caps.maxDoublets ? kCaCellOverflow : 0;
+        ca.cnt[e] = c;
+      }
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) doubletFillKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      __shared__ PairTable T;
+      const HitView H = hitsOf(p, e);
+      if (threadIdx.x == 0)
+        loadPairTable(H.layerStart, T);
+      __syncthreads();
+      const uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      const uint32_t ntot = T.cum[kNumPairs - 1];
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t* ohc = ca.ohCount + size_t(e) * kHitStride;
+      for (uint32_t j = gridStart(); j < ntot; j += gridStride()) {
+        uint32_t c = dOff[j];
+        if (c >= nCells)
+          continue;
+        forEachDoublet(H, T, j, [&](uint32_t i, uint32_t oi, int pairId) {
+          if (c < nCells) {
+            ca.cellInner[cb + c] = uint16_t(i);
+            ca.cellOuter[cb + c] = uint16_t(oi);
+            ca.cellPair[cb + c] = uint8_t(pairId);
+            ca.cellFlags[cb + c] = 0;
+            ca.nbSlot[cb + c] = -1;
+            ca.nbCount[cb + c] = 0;
+            ca.trSlot[cb + c] = -1;
+            ca.trCount[cb + c] = 0;
+            atomicAdd(&ohc[oi], 1u);
+          }
+          ++c;
+        });
+      }
+    }
+
+    // ---- outer-hit cell lists (GPUCACell::OuterHitOfCell, first 128 cells ascending) ----
+    __global__ void __launch_bounds__(kEventBlock) ohScanKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t lds[kEventBlock / 64];
+      const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
+      uint32_t* cnt = ca.ohCount + size_t(e) * kHitStride;
+      uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
+      uint32_t carry = 0;
+      for (uint32_t base = 0; base < nHits; base += kEventBlock) {
+        const uint32_t h = base + threadIdx.x;
+        const uint32_t v = h < nHits ? cnt[h] : 0;
+        uint32_t tot;
+        const uint32_t ex = wgExclusiveScan<kEventBlock>(v, tot, lds);
+        if (h < nHits) {
+          off[h] = carry + ex;
+          cnt[h] = 0;  // fill cursor
+        }
+        carry += tot;
+      }
+      if (threadIdx.x == 0)
+        off[nHits] = carry;
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) ohFillKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t* cur = ca.ohCount + size_t(e) * kHitStride;
+      const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
+      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+        const uint32_t h = ca.cellOuter[cb + c];
+        const uint32_t pos = atomicAdd(&cur[h], 1u);
+        ca.ohCells[cb + off[h] + pos] = c;
+      }
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) ohSortKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
+      const size_t cb = ca.cellBase(e);
+      const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
+      uint32_t* cells = ca.ohCells + cb;
+      bool over = false;
+      for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
+        const uint32_t b = off[h], n = off[h + 1] - b;
+        for (uint32_t k = 1; k < n; ++k) {  // insertion sort, ascending cell index
+          const uint32_t v = cells[b + k];
+          uint32_t m = k;
+          while (m > 0 && cells[b + m - 1] > v) {
+            cells[b + m] = cells[b + m - 1];
+            --m;
+          }
+          cells[b + m] = v;
+        }
+        for (uint32_t k = 0; k < n; ++k)
+          ca.cellFlags[cb + cells[b + k]] = k < kCaMaxCellsPerHit ? 1 : 0;
+        over |= n > kCaMaxCellsPerHit;
+      }
+      if (over)
+        atomicOr(&ca.cnt[e].status, kCaOuterHitOverflow);
+    }
+
+    // ---- a13 kernel_connect: CAHitNtupletGeneratorKernelsImpl.h:191-262 ----------------
+    // Gathered per outer cell: the neighbours of cell `oc` are the cells `me`
+    // whose inner hit is oc's outer hit and which pass the cuts, ascending in
+    // `me` (the serial program's append order); `oc` must be among the first
+    // 128 cells of its outer hit's list.
+    __global__ void __launch_bounds__(kCaBlock) connectKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      __shared__ PairTable T;
+      const HitView H = hitsOf(p, e);
+      if (threadIdx.x == 0)
+        loadPairTable(H.layerStart, T);
+      __syncthreads();
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      const uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
+      uint32_t status = 0;
+      for (uint32_t oc = gridStart(); oc < nCells; oc += gridStride()) {
+        if (!(ca.cellFlags[cb + oc] & 1))
+          continue;
+        const uint32_t hi = ca.cellInner[cb + oc], hm = ca.cellOuter[cb + oc];
+        const int lo = kPairOuter[ca.cellPair[cb + oc]];
+        const float r1 = H.rg[hi], z1 = H.zg[hi], x1 = H.xg[hi], y1 = H.yg[hi];
+        const float ri = H.rg[hm], zi = H.zg[hm], x2 = H.xg[hm], y2 = H.yg[hm];
+        const float thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
+        const float dcut = H.detInd[hi] < 96 ? kCaDcaCutInner : kCaDcaCutOuter;
+        int32_t slot = -1;
+        uint32_t n = 0;
+        for (int q = 0; q < 6; ++q) {
+          const int pr = kPairsFromLayer[lo][q];
+          if (pr < 0)
+            break;
+          const uint32_t j = (pr ? T.cum[pr - 1] : 0) + (hm - T.layer[lo]);
+          const uint32_t beg = dOff[j], end = min(dOff[j + 1], nCells);
+          for (uint32_t me = beg; me < end; ++me) {
+            const uint32_t ho = ca.cellOuter[cb + me];
+            const float ro = H.rg[ho], zo = H.zg[ho];
+            if (!alignedRZ(r1, z1, ri, zi, ro, zo, kCaPtMin, thetaCut))
+              continue;
+            if (!dcaCut(x1, y1, x2, y2, H.xg[ho], H.yg[ho], dcut, kCaHardCurvCut))
+              continue;
+            atomicOr(&used[me >> 5], 1u << (me & 31));
+            if (n == 0)
+              atomicOr(&used[oc >> 5], 1u << (oc & 31));
+            if (n < kCaMaxNeighbors) {
+              if (slot == -1) {
+                const uint32_t s = atomicAdd(&ca.cnt[e].nbLists, 1u);
+                if (s + 1 < ca.caps.maxCellLists) {
+                  slot = int32_t(s);
+                } else {
+                  slot = -2;
+                  status |= kCaNeighborStoreOverflow;
+                }
+              }
+              if (slot >= 0)
+                pool[size_t(slot) * kCaMaxNeighbors + n] = me;
+            } else {
+              status |= kCaNeighborOverflow;
+            }
+            ++n;
+          }
+        }
+        if (slot >= 0) {
+          ca.nbSlot[cb + oc] = slot;
+          ca.nbCount[cb + oc] = uint8_t(min(n, kCaMaxNeighbors));
+          atomicAdd(&ca.cnt[e].nNeighbors, min(n, kCaMaxNeighbors));
+        }
+      }
+      if (status)
+        atomicOr(&ca.cnt[e].status, status);
+    }
+
+    // ---- a14 fishbone: gpuFishbone.h:21-90 (per hit: independent of the others) --------
+    __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const HitView H = hitsOf(p, e);
+      const uint32_t nHits = H.layerStart[10];
+      if (nHits < 2)
+        return;
+      const size_t cb = ca.cellBase(e);
+      const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
+      const uint32_t* cells = ca.ohCells + cb;
+      const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
+      for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
+        const uint32_t b = off[h];
+        const uint32_t n = min(off[h + 1] - b, kCaMaxCellsPerHit);
+        if (n < 2)
+          continue;
+        const float xo = H.xg[h], yo = H.yg[h], zo = H.zg[h];
+        for (uint32_t ic = 0; ic + 1 < n; ++ic) {
+          const uint32_t ci = cells[b + ic];
+          if (!isUsed(ci))
+            continue;
+          const uint32_t hi = ca.cellInner[cb + ci];
+          const uint16_t di = H.detInd[hi];
+          const float xi = H.xg[hi] - xo, yi = H.yg[hi] - yo, zi = H.zg[hi] - zo;
+          const float ni = xi * xi + yi * yi + zi * zi;
+          for (uint32_t jc = ic + 1; jc < n; ++jc) {
+            const uint32_t cj = cells[b + jc];
+            if (!isUsed(cj))
+              continue;
+            const uint32_t hj = ca.cellInner[cb + cj];
+            const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
+            const float nj = xj * xj + yj * yj + zj * zj;
+            const float cos12 = xi * xj + yi * yj + zi * zj;
+            if (di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj) {
+              if (ni > nj) {
+                ca.cellFlags[cb + ci] |= 2;
+                break;
+              } else {
+                ca.cellFlags[cb + cj] |= 2;
+              }
+            }
+          }
+        }
+      }
+    }
+
+    // ---- a15 kernel_find_ntuplets: GPUCACell.h:278-331 ----------------------------------
+    // Iterative depth-first walk from a start cell, visiting neighbours in list
+    // order: tuples come out in the serial program's order.  Count mode counts
+    // tuples and hits; fill mode writes them from the cell's scanned offsets.
+    template <bool kFill>
+    __device__ void walk(const CAPool& ca, uint32_t e, uint32_t c, uint32_t& nT, uint32_t& nH, uint32_t& status) {
+      const size_t cb = ca.cellBase(e);
+      const uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
+      const size_t tb = ca.tupleBase(e);
+      uint32_t stack[4];
+      uint8_t pos[4];
+      bool any[4];
+      int depth = 1;
+      stack[0] = c;
+      pos[0] = 0;
+      any[0] = false;
+      uint32_t tBase = 0, hBase = 0;
+      if (kFill) {
+        tBase = ca.dfsT[cb + c];
+        hBase = ca.dfsH[cb + c];
+      }
+      nT = nH = 0;
+      while (depth > 0) {
+        const int top = depth - 1;
+        const uint32_t cell = stack[top];
+        const int32_t slot = ca.nbSlot[cb + cell];
+        const uint32_t cnt = slot >= 0 ? ca.nbCount[cb + cell] : 0;
+        bool pushed = false;
+        while (pos[top] < cnt) {
+          const uint32_t other = pool[size_t(slot) * kCaMaxNeighbors + pos[top]];
+          ++pos[top];
+          if (ca.cellFlags[cb + other] & 2)
+            continue;
+          any[top] = true;
+          if (depth >= 4) {
+            status |= kCaDepthOverflow;
+            continue;
+          }
+          stack[depth] = other;
+          pos[depth] = 0;
+          any[depth] = false;
+          ++depth;
+          pushed = true;
+          break;
+        }
+        if (pushed)
+          continue;
+        if (!any[top] && depth >= 2) {
+          const uint32_t nh = uint32_t(depth) + 1;
+          if (kFill) {
+            const uint32_t m = tBase + nT;
+            if (m < ca.caps.maxTuples) {
+              const uint32_t first = hBase + nH;
+              ca.tupleOff[ca.offBase(e) + m] = first;
+              uint16_t* bins = ca.tupleBins + 5 * tb + first;
+              for (int k = 0; k < depth; ++k) {
+                bins[k] = ca.cellInner[cb + stack[k]];
+                ca.tupleCells[4 * tb + 4 * size_t(m) + k] = stack[k];
+                atomicAdd(&ca.trCount[cb + stack[k]], 1u);
+              }
+              bins[depth] = ca.cellOuter[cb + cell];
+              for (int k = depth; k < 4; ++k)
+                ca.tupleCells[4 * tb + 4 * size_t(m) + k] = 0xffffffffu;
+              ca.quality[tb + m] = kQualityBad;
+            }
+          }
+          ++nT;
+          nH += nh;
+        }
+        --depth;
+      }
+    }
+
+    __device__ __forceinline__ bool isStartCell(const CAPool& ca, size_t cb, uint32_t c) {
+      if (ca.cellFlags[cb + c] & 2)
+        return false;
+      const int pid = ca.cellPair[cb + c];
+      return pid < 8 || pid > 12;
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) ntupletCountKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t status = 0;
+      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+        uint32_t nT = 0, nH = 0;
+        if (isStartCell(ca, cb, c) && ca.nbSlot[cb + c] >= 0)
+          walk<false>(ca, e, c, nT, nH, status);
+        ca.dfsT[cb + c] = nT;
+        ca.dfsH[cb + c] = nH;
+      }
+      if (status)
+        atomicOr(&ca.cnt[e].status, status);
+    }
+
+    __global__ void __launch_bounds__(kEventBlock) ntupletScanKernel(CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t lds[kEventBlock / 64];
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t* t = ca.dfsT + cb;
+      uint32_t* h = ca.dfsH + cb;
+      uint32_t carryT = 0, carryH = 0;
+      for (uint32_t base = 0; base < nCells; base += kEventBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t vt = i < nCells ? t[i] : 0, vh = i < nCells ? h[i] : 0;
+        uint32_t totT, totH;
+        const uint32_t exT = wgExclusiveScan<kEventBlock>(vt, totT, lds);
+        const uint32_t exH = wgExclusiveScan<kEventBlock>(vh, totH, lds);
+        if (i < nCells) {
+          t[i] = carryT + exT;
+          h[i] = carryH + exH;
+        }
+        carryT += totT;
+        carryH += totH;
+      }
+      if (threadIdx.x == 0) {
+        CACounts& c = ca.cnt[e];
+        c.apcM = carryT;
+        c.apcN = carryH;
+        c.nTuples = min(carryT, ca.caps.maxTuples);
+        if (carryT > ca.caps.maxTuples)
+          c.status |= kCaTupleOverflow;
+      }
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) ntupletFillKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t status = 0;
+      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+        const uint32_t next = c + 1 < nCells ? ca.dfsT[cb + c + 1] : ca.cnt[e].apcM;
+        if (next == ca.dfsT[cb + c])
+          continue;  // starts no tuple
+        uint32_t nT, nH;
+        walk<true>(ca, e, c, nT, nH, status);
+      }
+    }
+
+    // HistoContainer.h:193-204 bulkFinalizeFill, and one track list slot per cell in a tuple
+    __global__ void __launch_bounds__(kCaBlock) trackListAllocKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const CACounts cnt = ca.cnt[e];
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      const uint32_t M = ca.caps.maxTuples;
+      if (blockIdx.x == 0) {
+        uint32_t* off = ca.tupleOff + ca.offBase(e);
+        if (cnt.apcM >= M) {
+          if (threadIdx.x == 0)
+            off[M] = off[M - 1];
+        } else {
+          for (uint32_t i = cnt.apcM + threadIdx.x; i <= M; i += blockDim.x)
+            off[i] = cnt.apcN;
+        }
+      }
+      uint32_t status = 0;
+      for (uint32_t c = gridStart(); c < cnt.nCells; c += gridStride()) {
+        const uint32_t n = ca.trCount[cb + c];
+        ca.dfsT[cb + c] = n;  // tracks of the cell (all of them)
+        ca.trCount[cb + c] = 0;
+        if (n == 0)
+          continue;
+        const uint32_t s = atomicAdd(&ca.cnt[e].trLists, 1u);
+        if (s + 1 < ca.caps.maxCellLists) {
+          ca.trSlot[cb + c] = int32_t(s);
+        } else {
+          ca.trSlot[cb + c] = -2;
+          status |= kCaTrackStoreOverflow;
+        }
+      }
+      if (status)
+        atomicOr(&ca.cnt[e].status, status);
+    }
+
+    __global__ void __launch_bounds__(kCaBlock) trackListFillKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nT = ca.cnt[e].nTuples;
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t c = ca.tupleCells[4 * tb + 4 * size_t(t) + k];
+          if (c == 0xffffffffu)
+            break;
+          const int32_t slot = ca.trSlot[cb + c];
+          const uint32_t pos = atomicAdd(&ca.trCount[cb + c], 1u);
+          if (slot >= 0 && pos < kCaMaxCellTracks)
+            pool[size_t(slot) * kCaMaxCellTracks + pos] = uint16_t(t);
+        }
+      }
+    }
+
+    // sorts each cell's track list (ascending tuple index, first 48 kept) and
+    // runs a16 kernel_earlyDuplicateRemover (CAHitNtupletGeneratorKernelsImpl.h:126-149)
+    // on it: order independent, it only marks shorter tuples as duplicates.
+    __global__ void __launch_bounds__(kCaBlock) trackListSortKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const uint32_t nT = ca.cnt[e].nTuples;
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      const uint32_t* off = ca.tupleOff + ca.offBase(e);
+      uint32_t status = 0;
+      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+        const uint32_t total = ca.dfsT[cb + c];
+        const int32_t slot = ca.trSlot[cb + c];
+        if (total == 0 || slot < 0) {
+          ca.trCount[cb + c] = 0;
+          continue;
+        }
+        uint16_t* list = pool + size_t(slot) * kCaMaxCellTracks;
+        uint32_t n = total;
+        if (total > kCaMaxCellTracks) {
+          // keep the 48 smallest tuple indices: rebuild from the tuples
+          status |= kCaTrackOverflow;
+          n = 0;
+          for (uint32_t t = 0; t < nT && n < kCaMaxCellTracks; ++t)
+            for (int k = 0; k < 4; ++k) {
+              const uint32_t x = ca.tupleCells[4 * tb + 4 * size_t(t) + k];
+              if (x == 0xffffffffu)
+                break;
+              if (x == c) {
+                list[n++] = uint16_t(t);
+                break;
+              }
+            }
+        } else {
+          for (uint32_t k = 1; k < n; ++k) {
+            const uint16_t v = list[k];
+            uint32_t m = k;
+            while (m > 0 && list[m - 1] > v) {
+              list[m] = list[m - 1];
+              --m;
+            }
+            list[m] = v;
+          }
+        }
+        ca.trCount[cb + c] = n;
+        if (n < 2)
+          continue;
+        uint32_t maxNh = 0;
+        for (uint32_t k = 0; k < n; ++k)
+          maxNh = max(maxNh, off[list[k] + 1] - off[list[k]]);
+        for (uint32_t k = 0; k < n; ++k)
+          if (off[list[k] + 1] - off[list[k]] != maxNh)
+            ca.quality[tb + list[k]] = kQualityDup;
+        const uint32_t at = atomicAdd(&ca.cnt[e].nMulti, 1u);
+        ca.multiCells[cb + at] = c;
+      }
+      if (status)
+        atomicOr(&ca.cnt[e].status, status);
+    }
+
+    // ---- a16 tuple multiplicity (kernel_countMultiplicity / fillMultiplicity) ----------
+    // One workgroup per event.  Within a size the serial fill leaves tuples in
+    // descending index order; each tuple's place is computed from its rank.
+    // Also writes the detector indices (a17) and clears the fit outputs.
+    __global__ void __launch_bounds__(kEventBlock) multiplicityKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t binCount[8], binEnd[8], carry[8];
+      __shared__ uint32_t waveCnt[8][kEventBlock / 64];
+      const uint32_t nT = ca.cnt[e].nTuples;
+      const size_t tb = ca.tupleBase(e);
+      const uint32_t* off = ca.tupleOff + ca.offBase(e);
+      const uint8_t* q = ca.quality + tb;
+      const uint32_t cap = ca.caps.maxTuplesMult;
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+      if (threadIdx.x < 8) {
+        binCount[threadIdx.x] = 0;
+        carry[threadIdx.x] = 0;
+      }
+      __syncthreads();
+      auto binOf = [&](uint32_t t) -> int {
+        const uint32_t nh = off[t + 1] - off[t];
+        return (nh < 3 || nh > 7 || q[t] == kQualityDup) ? -1 : int(nh);
+      };
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        const int b = binOf(t);
+        if (b >= 0)
+          atomicAdd(&binCount[b], 1u);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int b = 0; b < 8; ++b) {
+          s += binCount[b];
+          binEnd[b] = s;
+        }
+        uint32_t* mo = ca.multOff + size_t(e) * 16;
+        for (int b = 0; b < 8; ++b)
+          mo[b] = min(binEnd[b] - binCount[b], cap);
+        mo[8] = min(s, cap);
+        ca.cnt[e].nMult = min(s, cap);
+        if (s > cap)
+          atomicOr(&ca.cnt[e].status, kCaMultiplicityOverflow);
+      }
+      __syncthreads();
+      uint16_t* mb = ca.multBins + size_t(e) * cap;
+      for (uint32_t base = 0; base < nT; base += kEventBlock) {
+        const uint32_t t = base + threadIdx.x;
+        const int b = t < nT ? binOf(t) : -1;
+        uint32_t below = 0;
+        for (int bb = 3; bb < 8; ++bb) {
+          const uint64_t m = __ballot(b == bb);
+          if (lane == 0)
+            waveCnt[bb][wid] = __popcll(m);
+          if (b == bb)
+            below = __popcll(m & ((1ull << lane) - 1));
+        }
+        __syncthreads();
+        if (b >= 0) {
+          uint32_t rank = carry[b] + below;
+          for (int w = 0; w < wid; ++w)
+            rank += waveCnt[b][w];
+          const uint32_t pos = binEnd[b] - 1 - rank;
+          if (pos < cap)
+            mb[pos] = uint16_t(t);
+        }
+        __syncthreads();
+        if (threadIdx.x < 8 && threadIdx.x >= 3) {
+          uint32_t s = 0;
+          for (int w = 0; w < kEventBlock / 64; ++w)
+            s += waveCnt[threadIdx.x][w];
+          carry[threadIdx.x] += s;
+        }
+        __syncthreads();
+      }
+      // a17 kernel_fillHitDetIndices
+      const uint16_t* detInd = p.detInd + size_t(e) * kMaxNumClusters;
+      const uint32_t nEntries = off[ca.caps.maxTuples];
+      for (uint32_t i = threadIdx.x; i < nEntries; i += kEventBlock)
+        ca.detIndices[5 * tb + i] = detInd[ca.tupleBins[5 * tb + i]];
+      const uint32_t M = ca.caps.maxTuples;
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        for (int k = 0; k < 5; ++k)
+          ca.state[5 * tb + size_t(k) * M + t] = 0;
+        for (int k = 0; k < 15; ++k)
+          ca.cov[15 * tb + size_t(k) * M + t] = 0;
+        ca.pt[tb + t] = 0;
+        ca.eta[tb + t] = 0;
+        ca.chi2[tb + t] = 0;
+      }
+    }
+
+  }  // namespace
+
+#define PXG_K(NAME, KERNEL, GRID, BLOCK, ...) \
+  do {                                           \
+    l.t->b(NAME);                                \
+    KERNEL<<<GRID, BLOCK, 0, l.s>>>(__VA_ARGS__); \
+    l.t->e();                                    \
+  } while (0)
+
+  void launchDoublets(const CALaunch& l) {
+    const dim3 g(l.blocks, l.nEvents);
+    PXG_K("doubletCount", doubletCountKernel, g, kCaBlock, *l.p, *l.ca);
+    PXG_K("doubletScan", doubletScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
+    PXG_K("doubletFill", doubletFillKernel, g, kCaBlock, *l.p, *l.ca);
+  }
+
+  void launchOuterHitLists(const CALaunch& l) {
+    const dim3 g(l.blocks, l.nEvents);
+    PXG_K("outerHitScan", ohScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
+    PXG_K("outerHitFill", ohFillKernel, g, kCaBlock, *l.ca);
+    PXG_K("outerHitSort", ohSortKernel, g, kCaBlock, *l.p, *l.ca);
+  }
+
+  void launchConnect(const CALaunch& l) {
+    PXG_K("connect", connectKernel, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+  }
+
+  void launchFishbone(const CALaunch& l) {
+    PXG_K("fishbone", fishboneKernel, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+  }
+
+  void launchNtuplets(const CALaunch& l) {
+    const dim3 g(l.blocks, l.nEvents);
+    PXG_K("ntupletCount", ntupletCountKernel, g, kCaBlock, *l.ca);
+    PXG_K("ntupletScan", ntupletScanKernel, l.nEvents, kEventBlock, *l.ca);
+    PXG_K("ntupletFill", ntupletFillKernel, g, kCaBlock, *l.ca);
+    PXG_K("trackListAlloc", trackListAllocKernel, g, kCaBlock, *l.ca);
+    PXG_K("trackListFill", trackListFillKernel, g, kCaBlock, *l.ca);
+    PXG_K("trackListSort", trackListSortKernel, g, kCaBlock, *l.ca);
+  }
+
+  void launchMultiplicity(const CALaunch& l) {
+    PXG_K("multiplicity", multiplicityKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
+  }
+
+}  // namespace pxg

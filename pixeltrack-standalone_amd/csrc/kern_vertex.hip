@@ -1,0 +1,391 @@
+// Primary vertices on gfx950 (a23 loadTracks, a24 clusterTracksByDensity +
+// fitVertices + splitVertices, a25 sortByPt2: plugin-PixelVertexFinding
+// gpuVertexFinderImpl.h:10-111, gpuClusterTracksByDensity.h:20-207,
+// gpuFitVertices.h:15-98, gpuSplitVertices.h:15-125, gpuSortByPt2.h:15-55).
+//
+// One workgroup per event.  Every float sum of the serial program runs over
+// tracks in ascending order; here each vertex owns a list of its tracks in
+// ascending order (a stable counting sort) and one lane per vertex sums its
+// list, so the sums are the serial ones bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "ca_device.h"
+
+namespace pxg {
+
+  namespace {
+    constexpr int kVB = kEventBlock;
+    constexpr int kUnassigned = 9999;
+
+    struct VtxLds {
+      uint32_t counts[kMaxVertices];
+      uint32_t start[kMaxVertices + 1];
+      uint32_t running[kMaxVertices];
+      uint16_t chunkKey[kVB];
+      uint32_t scan[kVB / 64];
+      uint32_t nvIntermediate, nvFinal, found, nt;
+    };
+
+    // Stable counting sort of items 0..n-1 by key(i) in [0, K) (items with a
+    // key >= K are left out): list gets the items bin by bin, ascending item
+    // index within a bin (descending if `descending`); off[0..K] the bin starts.
+    template <typename Key>
+    __device__ void countingSort(uint32_t n, uint32_t K, Key key, bool descending, uint32_t* list, uint32_t* off,
+                                 VtxLds& L) {
+      for (uint32_t k = threadIdx.x; k < K; k += kVB) {
+        L.counts[k] = 0;
+        L.running[k] = 0;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n; i += kVB) {
+        const uint32_t k = key(i);
+        if (k < K)
+          atomicAdd(&L.counts[k], 1u);
+      }
+      __syncthreads();
+      uint32_t carry = 0;
+      for (uint32_t base = 0; base < K; base += kVB) {
+        const uint32_t k = base + threadIdx.x;
+        const uint32_t v = k < K ? L.counts[k] : 0;
+        uint32_t tot;
+        const uint32_t ex = wgExclusiveScan<kVB>(v, tot, L.scan);
+        if (k < K)
+          L.start[k] = carry + ex;
+        carry += tot;
+      }
+      if (threadIdx.x == 0)
+        L.start[K] = carry;
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k <= K; k += kVB)
+        off[k] = L.start[k];
+      for (uint32_t base = 0; base < n; base += kVB) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t k = i < n ? key(i) : 0xffffu;
+        L.chunkKey[threadIdx.x] = uint16_t(min(k, 0xffffu));
+        __syncthreads();
+        if (i < n && k < K) {
+          uint32_t rank = L.running[k];
+          for (uint32_t j = 0; j < threadIdx.x; ++j)
+            rank += L.chunkKey[j] == k;
+          const uint32_t pos = descending ? L.start[k] + L.counts[k] - 1 - rank : L.start[k] + rank;
+          list[pos] = i;
+        }
+        __syncthreads();
+        if (i < n && k < K)
+          atomicAdd(&L.running[k], 1u);
+        __syncthreads();
+      }
+    }
+
+    __global__ void __launch_bounds__(kVB) vertexKernel(CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ VtxLds L;
+      const size_t tb = ca.tupleBase(e);
+      const uint32_t M = ca.caps.maxTuples;
+      const uint32_t nT = ca.cnt[e].nTuples;
+      const uint32_t* toff = ca.tupleOff + ca.offBase(e);
+      const uint8_t* q = ca.quality + tb;
+      int16_t* idv = ca.idv + tb;
+      int32_t* nn = ca.ndof + tb;
+      uint16_t* itrk = ca.vItrk + tb;
+      float* zt = ca.vZt + tb;
+      float* ezt2 = ca.vEzt2 + tb;
+      float* ptt2 = ca.vPtt2 + tb;
+      int32_t* iv = ca.vIv + tb;
+      int32_t* tmp = ca.vTmp + tb;
+      uint8_t* izt = ca.vIzt + tb;
+      uint32_t* zbins = ca.vTrk + tb;  // tracks by z bin, then tracks by vertex
+      uint32_t* voff = ca.vOff + size_t(e) * (kMaxVertices + 64);
+      float* zv = ca.zv + size_t(e) * kMaxVertices;
+      float* wv = ca.wv + size_t(e) * kMaxVertices;
+      float* chi2 = ca.vchi2 + size_t(e) * kMaxVertices;
+      float* ptv2 = ca.ptv2 + size_t(e) * kMaxVertices;
+      uint16_t* sortInd = ca.sortInd + size_t(e) * kMaxVertices;
+      constexpr int minT = 2;
+      constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f, ptMin = 0.5f;
+
+      // ---- loadTracks: loose tuples with 4+ hits and pt >= 0.5 GeV, in tuple order
+      if (threadIdx.x == 0)
+        L.nt = 0;
+      __syncthreads();
+      for (uint32_t base = 0; base < nT; base += kVB) {
+        const uint32_t t = base + threadIdx.x;
+        bool take = false;
+        if (t < nT) {
+          const uint32_t nh = toff[t + 1] - toff[t];
+          idv[t] = nh == 0 ? 0 : -1;  // only the last stored tuple can be empty
+          take = nh >= 4 && q[t] == kQualityLoose && !(ca.pt[tb + t] < ptMin);
+        }
+        uint32_t tot;
+        const uint32_t ex = wgExclusiveScan<kVB>(take ? 1u : 0u, tot, L.scan);
+        if (take) {
+          const uint32_t k = L.nt + ex;
+          const float pt = ca.pt[tb + t];
+          itrk[k] = uint16_t(t);
+          zt[k] = ca.state[5 * tb + 4 * size_t(M) + t];
+          ezt2[k] = ca.cov[15 * tb + 14 * size_t(M) + t];
+          ptt2[k] = pt * pt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+          L.nt += tot;
+        __syncthreads();
+      }
+      const uint32_t nt = L.nt;
+
+      // ---- clusterTracksByDensity
+      const float er2mx = errmax * errmax;
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+        int iz = int(zt[i] * 10.);
+        iz = min(max(iz, -128), 127);
+        izt[i] = uint8_t(iz + 128);
+        iv[i] = int32_t(i);
+        nn[i] = 0;
+      }
+      __syncthreads();
+      countingSort(nt, 256, [&](uint32_t i) { return uint32_t(izt[i]); }, true, zbins, voff, L);
+      __syncthreads();
+      auto forEachNear = [&](uint32_t value, auto&& f) {
+        const int bs = max(0, int(value) - 1);
+        const int be = min(255, int(value) + 1);
+        for (uint32_t p = voff[bs]; p < voff[be + 1]; ++p)
+          f(zbins[p]);
+      };
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+        if (ezt2[i] > er2mx)
+          continue;
+        int32_t c = 0;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (i == j)
+            return;
+          const float dist = fabsf(zt[i] - zt[j]);
+          if (dist > eps)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          ++c;
+        });
+        nn[i] = c;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+        float mdist = eps;
+        int32_t best = int32_t(i);
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (nn[j] < nn[i])
+            return;
+          if (nn[j] == nn[i] && zt[j] >= zt[i])
+            return;
+          const float dist = fabsf(zt[i] - zt[j]);
+          if (dist > mdist)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          mdist = dist;
+          best = int32_t(j);
+        });
+        iv[i] = best;
+      }
+      __syncthreads();
+      // follow the links to the root of each tree
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+        // links go to strictly more neighbours, or as many at smaller z: no cycles
+        int32_t m = iv[i];
+        for (uint32_t s = 0; m != iv[m] && s < nt; ++s)
+          m = iv[m];
+        tmp[i] = m;
+      }
+      __syncthreads();
+      // roots with enough neighbours become vertices, numbered in track order
+      if (threadIdx.x == 0)
+        L.found = 0;
+      __syncthreads();
+      for (uint32_t base = 0; base < nt; base += kVB) {
+        const uint32_t i = base + threadIdx.x;
+        const bool root = i < nt && tmp[i] == int32_t(i);
+        const bool seed = root && nn[i] >= minT;
+        uint32_t tot;
+        const uint32_t ex = wgExclusiveScan<kVB>(seed ? 1u : 0u, tot, L.scan);
+        if (root)
+          iv[i] = seed ? -int32_t(L.found + ex + 1) : -9998;
+        __syncthreads();
+        if (threadIdx.x == 0)
+          L.found += tot;
+        __syncthreads();
+      }
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB)
+        if (tmp[i] != int32_t(i))
+          iv[i] = iv[tmp[i]];
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < nt; i += kVB)
+        iv[i] = -iv[i] - 1;
+      if (threadIdx.x == 0) {
+        L.nvIntermediate = L.found;
+        L.nvFinal = L.found;
+        ca.cnt[e].vstatus = L.found >= kMaxVertices ? 1 : 0;
+        ca.cnt[e].vtxTracks = nt;
+      }
+      __syncthreads();
+
+      // ---- fitVertices (twice) and splitVertices
+      auto fit = [&](float chi2Max) {
+        const uint32_t nvf = L.nvIntermediate;
+        const uint32_t nc = min(nvf, kMaxVertices);
+        __syncthreads();
+        if (threadIdx.x == 0)
+          L.nvFinal = nvf;
+        countingSort(nt, nc, [&](uint32_t i) { return uint32_t(iv[i]); }, false, zbins, voff, L);
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
+          float z = 0, w = 0;
+          for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
+            const uint32_t i = zbins[p];
+            const float wi = 1.f / ezt2[i];
+            z += zt[i] * wi;
+            w += wi;
+          }
+          zv[v] = z / w;
+          wv[v] = w;
+          nn[v] = -1;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+          if (iv[i] > 9990 || iv[i] >= int32_t(nc))
+            continue;
+          float c2 = zv[iv[i]] - zt[i];
+          c2 *= c2 / ezt2[i];
+          if (c2 > chi2Max)
+            iv[i] = kUnassigned;
+          else
+            tmp[i] = __float_as_int(c2);
+        }
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
+          float c = 0;
+          int32_t k = -1;
+          for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
+            const uint32_t i = zbins[p];
+            if (iv[i] != int32_t(v))
+              continue;
+            c += __int_as_float(tmp[i]);
+            ++k;
+          }
+          chi2[v] = c;
+          nn[v] = k;
+          if (k > 0)
+            wv[v] *= float(k) / c;
+        }
+        __syncthreads();
+      };
+      fit(50.f);
+
+      // splitVertices: each vertex's 2-means runs on its own tracks; new
+      // vertices are numbered in the order of the vertices they split from
+      {
+        const uint32_t nvf = L.nvFinal;
+        for (uint32_t base = 0; base < nvf; base += kVB) {
+          const uint32_t kv = base + threadIdx.x;
+          bool split = false;
+          if (kv < nvf && kv < kMaxVertices && nn[kv] >= 4 && !(chi2[kv] < 9.f * float(nn[kv])) && nn[kv] < 512) {
+            // tracks of the vertex in ascending order: zbins[voff[kv], voff[kv+1]) still assigned
+            for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
+              const uint32_t i = zbins[p];
+              if (iv[i] != int32_t(kv))
+                continue;
+              const float zz = zt[i] - zv[kv];
+              tmp[i] = __float_as_int(zz);
+              izt[i] = zz < 0 ? 0 : 1;
+            }
+            float znew[2], wnew[2];
+            int maxiter = 20;
+            bool more = true;
+            while (more) {
+              more = false;
+              znew[0] = znew[1] = 0;
+              wnew[0] = wnew[1] = 0;
+              for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
+                const uint32_t i = zbins[p];
+                if (iv[i] != int32_t(kv))
+                  continue;
+                const float ww = 1.f / ezt2[i];
+                znew[izt[i]] += __int_as_float(tmp[i]) * ww;
+                wnew[izt[i]] += ww;
+              }
+              znew[0] /= wnew[0];
+              znew[1] /= wnew[1];
+              for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
+                const uint32_t i = zbins[p];
+                if (iv[i] != int32_t(kv))
+                  continue;
+                const float zz = __int_as_float(tmp[i]);
+                const float d0 = fabsf(zz - znew[0]);
+                const float d1 = fabsf(zz - znew[1]);
+                const uint8_t newer = d0 < d1 ? 0 : 1;
+                more |= newer != izt[i];
+                izt[i] = newer;
+              }
+              --maxiter;
+              if (maxiter <= 0)
+                more = false;
+            }
+            if (!(0 == wnew[0] || 0 == wnew[1])) {
+              const float dist2 = (znew[0] - znew[1]) * (znew[0] - znew[1]);
+              const float chi2Dist = dist2 / (1.f / wnew[0] + 1.f / wnew[1]);
+              split = !(chi2Dist < 4);
+            }
+          }
+          uint32_t tot;
+          const uint32_t ex = wgExclusiveScan<kVB>(split ? 1u : 0u, tot, L.scan);
+          if (split) {
+            // the new id is >= nvf, so no other vertex of this pass sees the moved tracks
+            const int32_t igv = int32_t(L.nvIntermediate + ex);
+            for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
+              const uint32_t i = zbins[p];
+              if (iv[i] == int32_t(kv) && izt[i] == 1)
+                iv[i] = igv;
+            }
+          }
+          __syncthreads();
+          if (threadIdx.x == 0)
+            L.nvIntermediate += tot;
+          __syncthreads();
+        }
+      }
+      fit(5000.f);
+
+      // ---- sortByPt2
+      const uint32_t nvFinal = L.nvFinal;
+      if (threadIdx.x == 0)
+        ca.cnt[e].nVertices = nvFinal;
+      if (nvFinal >= 1) {
+        for (uint32_t i = threadIdx.x; i < nt; i += kVB)
+          idv[itrk[i]] = int16_t(iv[i]);
+        const uint32_t nc = min(nvFinal, kMaxVertices);
+        for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
+          float s = 0;
+          for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
+            const uint32_t i = zbins[p];
+            if (iv[i] == int32_t(v))
+              s += ptt2[i];
+          }
+          ptv2[v] = s;
+        }
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
+          uint32_t r = 0;
+          const float pv = ptv2[v];
+          for (uint32_t u = 0; u < nc; ++u)
+            r += (ptv2[u] < pv) || (ptv2[u] == pv && u < v);
+          sortInd[r] = uint16_t(v);
+        }
+      }
+    }
+  }  // namespace
+
+  void launchVertices(const CALaunch& l) {
+    l.t->b("vertices");
+    vertexKernel<<<l.nEvents, kVB, 0, l.s>>>(*l.ca);
+    l.t->e();
+  }
+
+}  // namespace pxg

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -65,6 +66,8 @@ struct pxg_stream {
   pxg_context* ctx = nullptr;
   hipStream_t stream = nullptr;
   Pool pool{};
+  CAPool ca{};
+  CACounts* hostCA = nullptr;  // pinned
   std::vector<void*> owned;
   EventDesc* hostEvents = nullptr;    // pinned
   EventCounts* hostCounts = nullptr;  // pinned
@@ -105,6 +108,79 @@ namespace {
     v.phiOff = p.phiOff + size_t(first) * kPhiOffStride;
     v.avg = p.avg + first;
     return v;
+  }
+
+  // ntuplet arrays of the pool shifted to event `first`
+  CAPool caViewFrom(const CAPool& c, uint32_t first) {
+    CAPool v = c;
+    const size_t E = first, D = size_t(first) * c.caps.maxDoublets, T = size_t(first) * c.caps.maxTuples;
+    const size_t H = size_t(first) * kHitStride, L = size_t(first) * c.caps.maxCellLists;
+    v.cnt += E;
+    v.dOff += size_t(first) * (kDoubletSlots + 64);
+    v.cellInner += D;
+    v.cellOuter += D;
+    v.cellPair += D;
+    v.cellFlags += D;
+    v.cellUsed += size_t(first) * (c.caps.maxDoublets / 32 + 64);
+    v.ohCount += H;
+    v.ohOff += H;
+    v.ohCells += D;
+    v.nbSlot += D;
+    v.nbCount += D;
+    v.nbPool += L * kCaMaxNeighbors;
+    v.dfsT += D;
+    v.dfsH += D;
+    v.trSlot += D;
+    v.trCount += D;
+    v.trPool += L * kCaMaxCellTracks;
+    v.multiCells += D;
+    v.tupleOff += c.offBase(first);
+    v.tupleBins += 5 * T;
+    v.tupleCells += 4 * T;
+    v.quality += T;
+    v.multOff += 16 * E;
+    v.multBins += size_t(first) * c.caps.maxTuplesMult;
+    v.detIndices += 5 * T;
+    v.state += 5 * T;
+    v.cov += 15 * T;
+    v.pt += T;
+    v.eta += T;
+    v.chi2 += T;
+    v.h2tCount += H;
+    v.h2tOff += H;
+    v.h2tBins += size_t(first) * c.caps.maxHitToTuple;
+    v.idv += T;
+    v.ndof += T;
+    v.zv += E * kMaxVertices;
+    v.wv += E * kMaxVertices;
+    v.vchi2 += E * kMaxVertices;
+    v.ptv2 += E * kMaxVertices;
+    v.sortInd += E * kMaxVertices;
+    v.vItrk += T;
+    v.vZt += T;
+    v.vEzt2 += T;
+    v.vPtt2 += T;
+    v.vIv += T;
+    v.vTmp += T;
+    v.vIzt += T;
+    v.vTrk += T;
+    v.vOff += E * (kMaxVertices + 64);
+    return v;
+  }
+
+  void timerBegin(void* ctx, const char* name) {
+    auto* s = static_cast<pxg_stream*>(ctx);
+    ProfileMark m{name, nullptr, nullptr};
+    if (hipEventCreate(&m.start) != hipSuccess || hipEventCreate(&m.stop) != hipSuccess)
+      return;
+    (void)hipEventRecord(m.start, s->stream);
+    s->marks.push_back(m);
+  }
+
+  void timerEnd(void* ctx) {
+    auto* s = static_cast<pxg_stream*>(ctx);
+    if (!s->marks.empty())
+      (void)hipEventRecord(s->marks.back().stop, s->stream);
   }
 
   template <typename F>
@@ -151,7 +227,39 @@ namespace {
       if ((rc = launchTimed(s, "phiBinner", [&] { launchPhiBinner(c, v, count, s->stream); })))
         return rc;
     }
+    const CAPool ca = caViewFrom(s->ca, first);
+    Timer timer;
+    if (s->profiling)
+      timer = {timerBegin, timerEnd, s};
+    const CALaunch l{&c, &v, &ca, count, std::max(4u, std::min(512u, 2048u / count)), s->stream, &timer};
+    if (from <= 4 && upto >= 4) {
+      launchDoublets(l);
+      launchOuterHitLists(l);
+      launchConnect(l);
+      launchFishbone(l);
+      launchNtuplets(l);
+      launchMultiplicity(l);
+      launchFit(l);
+      launchClassify(l);
+      launchHitToTuple(l);
+    }
+    if (from <= 5 && upto >= 5)
+      launchVertices(l);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return fail(PXG_ERR_HIP, "launch of the ntuplet or vertex stage failed: %s", hipGetErrorString(e));
     return 0;
+  }
+
+  void fillTracks(pxg_stream* s, uint32_t e, pxg_tracks* t, pxg_vertices* vx) {
+    const CAPool ca = caViewFrom(s->ca, e);
+    const CACounts& c = s->hostCA[e];
+    if (t) {
+      *t = {ca.tupleOff, ca.tupleBins, ca.detIndices, ca.quality, ca.chi2, ca.eta, ca.pt, ca.state, ca.cov,
+            ca.caps.maxTuples, c.nTuples};
+    }
+    if (vx)
+      *vx = {ca.idv, ca.zv, ca.wv, ca.vchi2, ca.ptv2, ca.ndof, ca.sortInd, c.nVertices};
   }
 
   void fillProducts(pxg_stream* s, uint32_t e, pxg_digis* d, pxg_clusters* cl, pxg_errors* er, pxg_hits* h) {
@@ -260,8 +368,15 @@ void pxg_finalize(pxg_context* ctx) {
 }
 
 int pxg_stream_create(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, pxg_stream** out) {
-  if (!ctx || !out || maxEvents == 0 || maxWords == 0)
+  return pxg_stream_create_profile(ctx, maxEvents, maxWords, PXG_PROFILE_REFERENCE, out);
+}
+
+int pxg_stream_create_profile(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, int profile,
+                              pxg_stream** out) {
+  if (!ctx || !out || maxEvents == 0 || maxWords == 0 || maxEvents > 65535)
     return fail(PXG_ERR_ARG, "pxg_stream_create: bad arguments");
+  if (profile != PXG_PROFILE_REFERENCE && profile != PXG_PROFILE_HIGH_OCCUPANCY)
+    return fail(PXG_ERR_ARG, "pxg_stream_create: unknown capacity profile %d", profile);
   auto* s = new pxg_stream;
   s->ctx = ctx;
   const uint32_t W = alignUp(maxWords, kWordAlign) + kWordAlign * maxEvents;
@@ -312,13 +427,71 @@ int pxg_stream_create(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, p
   ALLOC(p.phiOff, E * kPhiOffStride);
   ALLOC(p.phiBins, H);
   ALLOC(p.avg, E);
+  CAPool& ca = s->ca;
+  ca.caps = profile == PXG_PROFILE_HIGH_OCCUPANCY ? kCapsHighOccupancy : kCapsReference;
+  const size_t D = E * ca.caps.maxDoublets, T = E * ca.caps.maxTuples, HS = E * kHitStride;
+  const size_t L = E * ca.caps.maxCellLists;
+  ALLOC(ca.cnt, E);
+  ALLOC(ca.dOff, E * (kDoubletSlots + 64));
+  ALLOC(ca.cellInner, D);
+  ALLOC(ca.cellOuter, D);
+  ALLOC(ca.cellPair, D);
+  ALLOC(ca.cellFlags, D);
+  ALLOC(ca.cellUsed, E * (ca.caps.maxDoublets / 32 + 64));
+  ALLOC(ca.ohCount, HS);
+  ALLOC(ca.ohOff, HS);
+  ALLOC(ca.ohCells, D);
+  ALLOC(ca.nbSlot, D);
+  ALLOC(ca.nbCount, D);
+  ALLOC(ca.nbPool, L * kCaMaxNeighbors);
+  ALLOC(ca.dfsT, D);
+  ALLOC(ca.dfsH, D);
+  ALLOC(ca.trSlot, D);
+  ALLOC(ca.trCount, D);
+  ALLOC(ca.trPool, L * kCaMaxCellTracks);
+  ALLOC(ca.multiCells, D);
+  ALLOC(ca.tupleOff, E * (ca.caps.maxTuples + 64));
+  ALLOC(ca.tupleBins, 5 * T);
+  ALLOC(ca.tupleCells, 4 * T);
+  ALLOC(ca.quality, T);
+  ALLOC(ca.multOff, 16 * E);
+  ALLOC(ca.multBins, E * ca.caps.maxTuplesMult);
+  ALLOC(ca.detIndices, 5 * T);
+  ALLOC(ca.state, 5 * T);
+  ALLOC(ca.cov, 15 * T);
+  ALLOC(ca.pt, T);
+  ALLOC(ca.eta, T);
+  ALLOC(ca.chi2, T);
+  ALLOC(ca.h2tCount, HS);
+  ALLOC(ca.h2tOff, HS);
+  ALLOC(ca.h2tBins, E * ca.caps.maxHitToTuple);
+  ALLOC(ca.idv, T);
+  ALLOC(ca.ndof, T);
+  ALLOC(ca.zv, E * kMaxVertices);
+  ALLOC(ca.wv, E * kMaxVertices);
+  ALLOC(ca.vchi2, E * kMaxVertices);
+  ALLOC(ca.ptv2, E * kMaxVertices);
+  ALLOC(ca.sortInd, E * kMaxVertices);
+  ALLOC(ca.vItrk, T);
+  ALLOC(ca.vZt, T);
+  ALLOC(ca.vEzt2, T);
+  ALLOC(ca.vPtt2, T);
+  ALLOC(ca.vIv, T);
+  ALLOC(ca.vTmp, T);
+  ALLOC(ca.vIzt, T);
+  ALLOC(ca.vTrk, T);
+  ALLOC(ca.vOff, E * (kMaxVertices + 64));
 #undef ALLOC
   if (e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&s->hostEvents), E * sizeof(EventDesc), hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&s->hostCounts), E * sizeof(EventCounts), hipHostMallocDefault);
   if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->hostCA), E * sizeof(CACounts), hipHostMallocDefault);
+  if (e == hipSuccess)
     e = hipMemsetAsync(p.counts, 0, E * sizeof(EventCounts), s->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(ca.cnt, 0, E * sizeof(CACounts), s->stream);
   if (e == hipSuccess)
     e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
@@ -326,6 +499,7 @@ int pxg_stream_create(pxg_context* ctx, uint32_t maxEvents, uint32_t maxWords, p
     return fail(PXG_ERR_HIP, "pxg_stream_create: %s", hipGetErrorString(e));
   }
   std::memset(s->hostCounts, 0, E * sizeof(EventCounts));
+  std::memset(s->hostCA, 0, E * sizeof(CACounts));
   *out = s;
   return 0;
 }
@@ -345,6 +519,8 @@ void pxg_stream_destroy(pxg_stream* s) {
     (void)hipHostFree(s->hostEvents);
   if (s->hostCounts)
     (void)hipHostFree(s->hostCounts);
+  if (s->hostCA)
+    (void)hipHostFree(s->hostCA);
   if (s->stream)
     (void)hipStreamDestroy(s->stream);
   delete s;
@@ -392,7 +568,7 @@ int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const p
 }
 
 int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
-  if (!s || upto < 1 || upto > 3)
+  if (!s || upto < 1 || upto > 5)
     return fail(PXG_ERR_ARG, "pxg_run: bad arguments");
   return runStages(s, first, count, 1, upto);
 }
@@ -402,10 +578,15 @@ int pxg_fetch_counts(pxg_stream* s, uint32_t first, uint32_t count) {
     return fail(PXG_ERR_ARG, "pxg_fetch_counts: bad range");
   PXG_HIP(hipMemcpyAsync(s->hostCounts + first, s->pool.counts + first, count * sizeof(EventCounts),
                          hipMemcpyDeviceToHost, s->stream));
+  PXG_HIP(hipMemcpyAsync(s->hostCA + first, s->ca.cnt + first, count * sizeof(CACounts), hipMemcpyDeviceToHost,
+                         s->stream));
   PXG_HIP(hipStreamSynchronize(s->stream));
   uint32_t st = 0;
-  for (uint32_t e = first; e < first + count; ++e)
+  for (uint32_t e = first; e < first + count; ++e) {
     st |= s->hostCounts[e].status;
+    st |= s->hostCA[e].status << PXG_STATUS_CA_SHIFT;
+    st |= s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u;
+  }
   return int(st);
 }
 
@@ -424,6 +605,40 @@ int pxg_event_counts(pxg_stream* s, uint32_t e, uint32_t out[4]) {
   out[1] = c.nClusters;
   out[2] = c.nErrors;
   out[3] = c.status;
+  return 0;
+}
+
+int pxg_event_tracks(pxg_stream* s, uint32_t e, pxg_tracks* t, pxg_vertices* v) {
+  if (!s || e >= s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_event_tracks: event %u not resident", e);
+  fillTracks(s, e, t, v);
+  return 0;
+}
+
+int pxg_event_ca_internals(pxg_stream* s, uint32_t e, pxg_ca_internals* o) {
+  if (!s || !o || e >= s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_event_ca_internals: event %u not resident", e);
+  const CAPool ca = caViewFrom(s->ca, e);
+  *o = {};
+  o->cellInner = ca.cellInner;
+  o->cellOuter = ca.cellOuter;
+  o->cellPair = ca.cellPair;
+  o->cellFlags = ca.cellFlags;
+  o->cellUsed = ca.cellUsed;
+  o->outerHitOff = ca.ohOff;
+  o->outerHitCells = ca.ohCells;
+  o->neighborSlot = ca.nbSlot;
+  o->neighborCount = ca.nbCount;
+  o->neighborPool = ca.nbPool;
+  o->trackSlot = ca.trSlot;
+  o->trackCount = ca.trCount;
+  o->trackPool = ca.trPool;
+  o->multOff = ca.multOff;
+  o->multBins = ca.multBins;
+  o->hitToTupleOff = ca.h2tOff;
+  o->hitToTupleBins = ca.h2tBins;
+  static_assert(sizeof(CACounts) <= sizeof(o->counts), "counts");
+  std::memcpy(o->counts, &s->hostCA[e], sizeof(CACounts));
   return 0;
 }
 
@@ -484,6 +699,36 @@ int pxg_rechits(pxg_stream* s, const pxg_digis* d, const pxg_clusters* c, const 
   if (rc < 0)
     return rc;
   fillProducts(s, 0, nullptr, nullptr, nullptr, h);
+  return rc;
+}
+
+int pxg_ntuplets(pxg_stream* s, const pxg_hits* h, pxg_tracks* t) {
+  if (!s || !h || !t)
+    return fail(PXG_ERR_ARG, "pxg_ntuplets: null argument");
+  if (s->nLoaded < 1 || h->xg != s->pool.xg)
+    return fail(PXG_ERR_ARG, "pxg_ntuplets: hits are not the last products of this stream");
+  int rc = runStages(s, 0, 1, 4, 4);
+  if (rc)
+    return rc;
+  rc = pxg_fetch_counts(s, 0, 1);
+  if (rc < 0)
+    return rc;
+  fillTracks(s, 0, t, nullptr);
+  return rc;
+}
+
+int pxg_find_vertices(pxg_stream* s, const pxg_tracks* t, pxg_vertices* v) {
+  if (!s || !t || !v)
+    return fail(PXG_ERR_ARG, "pxg_find_vertices: null argument");
+  if (s->nLoaded < 1 || t->hitOff != s->ca.tupleOff)
+    return fail(PXG_ERR_ARG, "pxg_find_vertices: tracks are not the last products of this stream");
+  int rc = runStages(s, 0, 1, 5, 5);
+  if (rc)
+    return rc;
+  rc = pxg_fetch_counts(s, 0, 1);
+  if (rc < 0)
+    return rc;
+  fillTracks(s, 0, nullptr, v);
   return rc;
 }
 

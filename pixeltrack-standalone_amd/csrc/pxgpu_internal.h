@@ -100,6 +100,121 @@ namespace pxg {
   constexpr uint32_t kLayerStartStride = 16;
   constexpr uint32_t kPhiOffStride = 1288;
 
+  // ---- ntuplets and vertices ---------------------------------------------------------
+  constexpr int kNumPairs = 19;                              // layer pairs of the doublet search
+  constexpr uint32_t kDoubletSlots = 6 * kMaxNumClusters;    // (pair, inner hit) slots: layer 0 is inner in 6 pairs
+  constexpr uint32_t kHitStride = kMaxNumClusters + 64;      // per-event stride of per-hit offset arrays
+  constexpr uint32_t kCaBlock = 256;                         // workgroup of the grid-stride CA kernels
+  constexpr uint32_t kEventBlock = 1024;                     // workgroup of the per-event kernels
+
+  // Per-event counters of the ntuplet and vertex stages.
+  struct CACounts {
+    uint32_t nCells;       // cells stored (min(doublets found, maxDoublets))
+    uint32_t nDoublets;    // doublets found
+    uint32_t apcM, apcN;   // tuples and tuple hits found (the reference's atomic pair counter)
+    uint32_t nTuples;      // tuples stored (min(apcM, maxTuples))
+    uint32_t status;       // kCa* bits
+    uint32_t nbLists;      // neighbour lists requested
+    uint32_t trLists;      // track lists requested
+    uint32_t nMulti;       // cells in two or more tuples
+    uint32_t nMult;        // entries of the multiplicity container
+    uint32_t nVertices;    // vertices after splitting (nvFinal)
+    uint32_t vtxTracks;    // tracks given to the vertex finder
+    uint32_t vstatus;      // 1: vertex container full
+    uint32_t nSlots;       // (layer pair, inner hit) slots of the doublet search
+    uint32_t nNeighbors;   // neighbour entries stored
+    uint32_t pad;
+  };
+
+  // Device arrays of the ntuplet and vertex stages, all per event at e * stride
+  // (strides follow the capacity profile chosen at stream creation).
+  struct CAPool {
+    CACapacities caps;
+    CACounts* cnt;
+    uint32_t* dOff;          // [kDoubletSlots + 64]: doublets per (pair, inner hit), then offsets
+    uint16_t *cellInner, *cellOuter;  // [maxDoublets]
+    uint8_t* cellPair;
+    uint8_t* cellFlags;      // bit0 listed in its outer hit's list, bit1 killed, bit2 duplicate pass done
+    uint32_t* cellUsed;      // bit per cell [maxDoublets / 32 + 64]
+    uint32_t* ohCount;       // [kHitStride] cells per outer hit, then fill cursors
+    uint32_t* ohOff;         // [kHitStride]
+    uint32_t* ohCells;       // [maxDoublets] cells by outer hit, ascending within a hit
+    int32_t* nbSlot;         // [maxDoublets] neighbour list slot (-1 none, -2 store full)
+    uint8_t* nbCount;        // [maxDoublets] neighbours stored (<= 36)
+    uint32_t* nbPool;        // [maxCellLists * 36]
+    uint32_t *dfsT, *dfsH;   // [maxDoublets] tuples / hits started by a cell, then offsets
+    int32_t* trSlot;         // [maxDoublets] track list slot (-1 none, -2 store full)
+    uint32_t* trCount;       // [maxDoublets] tracks stored (<= 48)
+    uint16_t* trPool;        // [maxCellLists * 48]
+    uint32_t* multiCells;    // [maxDoublets] cells with two or more tracks (any order)
+    uint32_t* tupleOff;      // [maxTuples + 64]
+    uint16_t* tupleBins;     // [5 * maxTuples]
+    uint32_t* tupleCells;    // [4 * maxTuples] cells of a tuple (0xffffffff past the end)
+    uint8_t* quality;        // [maxTuples]
+    uint32_t* multOff;       // [16]
+    uint16_t* multBins;      // [maxTuplesMult]
+    uint16_t* detIndices;    // [5 * maxTuples]
+    float* state;            // [5][maxTuples]: phi, tip, q/pt, cot(theta), zip
+    float* cov;              // [15][maxTuples]: packed upper triangle, row by row
+    float *pt, *eta, *chi2;  // [maxTuples]
+    uint32_t* h2tCount;      // [kHitStride]
+    uint32_t* h2tOff;        // [kHitStride]
+    uint16_t* h2tBins;       // [maxHitToTuple]
+    // vertices
+    int16_t* idv;            // [maxTuples]
+    int32_t* ndof;           // [maxTuples]: per-track neighbour counts, then per-vertex ndof
+    float *zv, *wv, *vchi2, *ptv2;  // [kMaxVertices]
+    uint16_t* sortInd;       // [kMaxVertices]
+    uint16_t* vItrk;         // [maxTuples] tuple index of each loaded track
+    float *vZt, *vEzt2, *vPtt2;      // [maxTuples]
+    int32_t *vIv, *vTmp;     // [maxTuples]
+    uint8_t* vIzt;           // [maxTuples]
+    uint16_t* vBins;         // [maxTuples]: tracks by z bin
+    uint32_t* vTrk;          // [maxTuples]: tracks by vertex, ascending within a vertex
+    uint32_t* vOff;          // [kMaxVertices + 64]
+
+    // per-event strides
+    __host__ __device__ size_t cellBase(uint32_t e) const { return size_t(e) * caps.maxDoublets; }
+    __host__ __device__ size_t tupleBase(uint32_t e) const { return size_t(e) * caps.maxTuples; }
+    __host__ __device__ size_t offBase(uint32_t e) const { return size_t(e) * (caps.maxTuples + 64); }
+  };
+
+  // Brackets each kernel launch (HIP events on the launch stream when profiling).
+  struct Timer {
+    void (*begin)(void* ctx, const char* name) = nullptr;
+    void (*end)(void* ctx) = nullptr;
+    void* ctx = nullptr;
+    void b(const char* name) const {
+      if (begin)
+        begin(ctx, name);
+    }
+    void e() const {
+      if (end)
+        end(ctx);
+    }
+  };
+
+  // kernel launchers (stage C, D); `blocks` is the number of workgroups per event of the
+  // grid-stride kernels
+  struct CALaunch {
+    const DevConditions* cond;
+    const Pool* p;
+    const CAPool* ca;
+    uint32_t nEvents, blocks;
+    hipStream_t s;
+    const Timer* t;
+  };
+  void launchDoublets(const CALaunch& l);
+  void launchOuterHitLists(const CALaunch& l);
+  void launchConnect(const CALaunch& l);
+  void launchFishbone(const CALaunch& l);
+  void launchNtuplets(const CALaunch& l);
+  void launchMultiplicity(const CALaunch& l);
+  void launchFit(const CALaunch& l);
+  void launchClassify(const CALaunch& l);
+  void launchHitToTuple(const CALaunch& l);
+  void launchVertices(const CALaunch& l);
+
   // kernel launchers (stage A, B); nBlocks = total decode workgroups of the batch
   void launchDecode(const DevConditions& c, const Pool& p, uint32_t nEvents, uint32_t nBlocks, uint32_t blockBase,
                     hipStream_t s);

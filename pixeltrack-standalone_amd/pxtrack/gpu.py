@@ -47,6 +47,27 @@ class RawView(C.Structure):
     _fields_ = [("words", vp), ("fedIds", vp), ("nWords", u32)]
 
 
+class Tracks(C.Structure):
+    _fields_ = [(n, vp) for n in ("hitOff", "hitIndices", "detIndices", "quality", "chi2", "eta", "pt", "state",
+                                  "covariance")] + [("stride", u32), ("nTuples", u32)]
+
+
+class Vertices(C.Structure):
+    _fields_ = [(n, vp) for n in ("idv", "zv", "wv", "chi2", "ptv2", "ndof", "sortInd")] + [("nvFinal", u32)]
+
+
+class CAInternals(C.Structure):
+    _fields_ = [(n, vp) for n in ("cellInner", "cellOuter", "cellPair", "cellFlags", "cellUsed", "outerHitOff",
+                                  "outerHitCells", "neighborSlot", "neighborCount", "neighborPool", "trackSlot",
+                                  "trackCount", "trackPool", "multOff", "multBins", "hitToTupleOff",
+                                  "hitToTupleBins")] + [("counts", u32 * 16)]
+
+
+CA_COUNTS = ["nCells", "nDoublets", "apcM", "apcN", "nTuples", "caStatus", "nbLists", "trLists", "nMulti", "nMult",
+             "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors"]
+PROFILE_REFERENCE, PROFILE_HIGH_OCCUPANCY = 0, 1
+
+
 DIGI_FIELDS = {"xx": np.uint16, "yy": np.uint16, "adc": np.uint16, "moduleInd": np.uint16, "clus": np.int32,
                "pdigi": np.uint32, "rawIdArr": np.uint32}
 CLUSTER_FIELDS = {"moduleStart": 2001, "clusInModule": 2000, "moduleId": 2000, "clusModuleStart": 2001}
@@ -63,6 +84,11 @@ def _gpu():
         lib.pxg_finalize.argtypes = [vp]
         lib.pxg_device_name.argtypes = [C.c_char_p, C.c_size_t]
         lib.pxg_stream_create.argtypes = [vp, u32, u32, C.POINTER(vp)]
+        lib.pxg_stream_create_profile.argtypes = [vp, u32, u32, C.c_int, C.POINTER(vp)]
+        lib.pxg_ntuplets.argtypes = [vp, C.POINTER(Hits), C.POINTER(Tracks)]
+        lib.pxg_find_vertices.argtypes = [vp, C.POINTER(Tracks), C.POINTER(Vertices)]
+        lib.pxg_event_tracks.argtypes = [vp, u32, C.POINTER(Tracks), C.POINTER(Vertices)]
+        lib.pxg_event_ca_internals.argtypes = [vp, u32, C.POINTER(CAInternals)]
         lib.pxg_stream_destroy.argtypes = [vp]
         lib.pxg_sync.argtypes = [vp]
         lib.pxg_raw_to_digi.argtypes = [vp, vp, vp, u32, C.POINTER(Digis), C.POINTER(Errors)]
@@ -147,17 +173,18 @@ class Context:
     def __del__(self):
         self.close()
 
-    def stream(self, max_events=1, max_words=300000):
-        return Stream(self, max_events, max_words)
+    def stream(self, max_events=1, max_words=300000, profile=PROFILE_REFERENCE):
+        return Stream(self, max_events, max_words, profile)
 
 
 class Stream:
     """One HIP stream with its device-resident product pool."""
 
-    def __init__(self, ctx, max_events, max_words):
+    def __init__(self, ctx, max_events, max_words, profile=PROFILE_REFERENCE):
         self.ctx, self.lib = ctx, ctx.lib
         self._h = vp()
-        _check(self.lib.pxg_stream_create(ctx._h, max_events, max_words, C.byref(self._h)), "pxg_stream_create")
+        _check(self.lib.pxg_stream_create_profile(ctx._h, max_events, max_words, profile, C.byref(self._h)),
+               "pxg_stream_create_profile")
         self._inputs = []
 
     def close(self):
@@ -197,6 +224,75 @@ class Stream:
         out["nHits"] = h.nHits
         return out
 
+    def tracks_to_host(self, t):
+        """Track products in the oracle's layout (per-tuple rows for state and covariance)."""
+        S, n = t.stride, t.nTuples
+        out = {"tupleOff": self._copy(t.hitOff, np.uint32, S + 1)}
+        nb = int(out["tupleOff"][S])
+        out["tupleBins"] = self._copy(t.hitIndices, np.uint16, nb)
+        out["detIndices"] = self._copy(t.detIndices, np.uint16, nb)
+        out["quality"] = self._copy(t.quality, np.uint8, S)
+        for k in ("chi2", "eta", "pt"):
+            out[k] = self._copy(getattr(t, k), np.float32, S)
+        out["state"] = self._copy(t.state, np.float32, 5 * S).reshape(5, S).T.copy().ravel()
+        out["covariance"] = self._copy(t.covariance, np.float32, 15 * S).reshape(15, S).T.copy().ravel()
+        out["nTuples"] = n
+        return out
+
+    def vertices_to_host(self, v, stride):
+        out = {"idv": self._copy(v.idv, np.int16, stride), "ndof": self._copy(v.ndof, np.int32, stride)}
+        for k, name in (("zv", "zv"), ("wv", "wv"), ("chi2", "chi2v"), ("ptv2", "ptv2")):
+            out[name] = self._copy(getattr(v, k), np.float32, 1024)
+        out["sortInd"] = self._copy(v.sortInd, np.uint16, 1024)
+        out["nVertices"] = v.nvFinal
+        return out
+
+    def ca_counts(self, e):
+        """Counters of the ntuplet and vertex stages of resident event e (after fetch_counts)."""
+        ci = CAInternals()
+        _check(self.lib.pxg_event_ca_internals(self._h, e, C.byref(ci)), "pxg_event_ca_internals")
+        return dict(zip(CA_COUNTS, list(ci.counts)))
+
+    def ca_internals(self, e, n_hits):
+        """The ntuplet stage's containers of resident event e, in the oracle's layout."""
+        ci = CAInternals()
+        _check(self.lib.pxg_event_ca_internals(self._h, e, C.byref(ci)), "pxg_event_ca_internals")
+        cnt = dict(zip(CA_COUNTS, list(ci.counts)))
+        nc = cnt["nCells"]
+        out = {"cellInner": self._copy(ci.cellInner, np.uint16, nc), "cellOuter": self._copy(ci.cellOuter, np.uint16, nc)}
+        out["cellLayerPair"] = self._copy(ci.cellPair, np.uint8, nc).astype(np.int16)
+        flags = self._copy(ci.cellFlags, np.uint8, nc)
+        out["cellDoubletId"] = np.where(flags & 2, -1, np.arange(nc)).astype(np.int32)
+        used = self._copy(ci.cellUsed, np.uint32, (nc + 31) // 32)
+        bits = np.unpackbits(used.view(np.uint8), bitorder="little")[:nc] if nc else np.zeros(0, np.uint8)
+        out["cellUsed"] = bits.astype(np.uint16)
+        ohoff = self._copy(ci.outerHitOff, np.uint32, n_hits + 1)
+        ohc = self._copy(ci.outerHitCells, np.uint32, int(ohoff[-1]) if n_hits else 0)
+        lists = [ohc[ohoff[h]:ohoff[h + 1]][:128] for h in range(n_hits)]
+        out["outerHitOff"] = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.uint32)
+        out["outerHitCells"] = np.concatenate(lists).astype(np.uint32) if lists else np.zeros(0, np.uint32)
+
+        def csr(slot_p, count_p, pool_p, width, dtype, count_dtype):
+            slot = self._copy(slot_p, np.int32, nc)
+            count = self._copy(count_p, count_dtype, nc).astype(np.int64)
+            count[slot < 0] = 0
+            npool = int(slot.max()) + 1 if nc and slot.max() >= 0 else 0
+            pool = self._copy(pool_p, dtype, npool * width).reshape(npool, width) if npool else None
+            vals = [pool[slot[c], :count[c]] for c in range(nc) if count[c]]
+            off = np.concatenate([[0], np.cumsum(count)]).astype(np.uint32)
+            return off, (np.concatenate(vals).astype(dtype) if vals else np.zeros(0, dtype))
+
+        out["cellNeighborOff"], out["cellNeighbors"] = csr(ci.neighborSlot, ci.neighborCount, ci.neighborPool, 36,
+                                                           np.uint32, np.uint8)
+        out["cellTrackOff"], out["cellTracks"] = csr(ci.trackSlot, ci.trackCount, ci.trackPool, 48, np.uint16,
+                                                     np.uint32)
+        out["multOff"] = self._copy(ci.multOff, np.uint32, 9)
+        out["multBins"] = self._copy(ci.multBins, np.uint16, int(out["multOff"][8]))
+        out["hitToTupleOff"] = self._copy(ci.hitToTupleOff, np.uint32, n_hits + 1)
+        out["hitToTupleBins"] = self._copy(ci.hitToTupleBins, np.uint16, int(out["hitToTupleOff"][-1]) if n_hits else 0)
+        out.update(cnt)
+        return out
+
     # -- plain per-event path
     def raw_to_cluster(self, words, fed_ids):
         words = np.ascontiguousarray(words, dtype=np.uint32)
@@ -220,7 +316,17 @@ class Stream:
         st = _check(self.lib.pxg_rechits(self._h, C.byref(d), C.byref(c), C.byref(bs), C.byref(h)), "pxg_rechits")
         return st, h
 
-    def process(self, words, fed_ids, upto=3):
+    def ntuplets(self, h):
+        t = Tracks()
+        st = _check(self.lib.pxg_ntuplets(self._h, C.byref(h), C.byref(t)), "pxg_ntuplets")
+        return st, t
+
+    def find_vertices(self, t):
+        v = Vertices()
+        st = _check(self.lib.pxg_find_vertices(self._h, C.byref(t), C.byref(v)), "pxg_find_vertices")
+        return st, v
+
+    def process(self, words, fed_ids, upto=3, internals=False):
         """Plain path on one event; returns a dict of host arrays named like the oracle's fields."""
         if upto == 1:
             st, d, e = self.raw_to_digi(words, fed_ids)
@@ -233,6 +339,16 @@ class Stream:
                 st2, h = self.rechits(d, c)
                 st |= st2
                 out.update(self.hits_to_host(h))
+            if upto >= 4:
+                st3, t = self.ntuplets(h)
+                st |= st3
+                out.update(self.tracks_to_host(t))
+                if internals:
+                    out.update(self.ca_internals(0, h.nHits))
+            if upto >= 5:
+                st4, v = self.find_vertices(t)
+                st |= st4
+                out.update(self.vertices_to_host(v, t.stride))
         out["errors"] = self.errors_to_host(e)
         out["status"] = st
         return out
@@ -264,6 +380,12 @@ class Stream:
         if upto >= 3:
             out.update(self.hits_to_host(h))
         out["errors"] = self.errors_to_host(er)
+        if upto >= 4:
+            t, v = Tracks(), Vertices()
+            _check(self.lib.pxg_event_tracks(self._h, e, C.byref(t), C.byref(v)), "pxg_event_tracks")
+            out.update(self.tracks_to_host(t))
+            if upto >= 5:
+                out.update(self.vertices_to_host(v, t.stride))
         return out
 
     def event_counts(self, e):

@@ -1,0 +1,149 @@
+"""GPU ntuplet and vertex stages against the oracle (through the C ABI).
+
+Bit-exact: cells, outer-hit lists, neighbour lists, fishbone kills, tuples,
+per-cell track lists, multiplicity, detector indices, quality flags, hit ->
+tuple association, vertex assignment, ndof and ordering.  Fitted track
+parameters and vertex z / weight / chi2 are floats computed in double (fits)
+or float (vertices) with the serial operation order; the device and host
+math libraries may differ in the last bit of atan2/log/sin/cos, so they are
+compared with a relative tolerance of 1e-5 (float outputs) and an absolute
+floor of 1e-10 for values that cancel to zero.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_same, event
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+ATOL = 1e-10  # values that cancel to ~0 (e.g. cot(theta) of a track at eta 0) differ by double-precision noise
+
+CASES = [
+    ("serial100", 0, {}),
+    ("serial100", 7, {}),
+    ("pu50", 0, dict(pu_lo=5, pu_hi=5)),
+    ("pu50", 0, {}),
+    ("pu50", 3, {}),
+    ("pu20_80", 2, {}),
+    ("pu50", 5, {"error_rate": 0.1}),
+]
+
+EXACT_CA = ["cellInner", "cellOuter", "cellLayerPair", "cellDoubletId", "cellUsed", "outerHitOff", "outerHitCells",
+            "cellNeighborOff", "cellNeighbors", "cellTrackOff", "cellTracks", "multOff"]
+
+
+def _close(a, b, name, ctx):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    if a.shape != b.shape:
+        return f"{ctx} {name}: shape {a.shape} != {b.shape}"
+    same = (a == b) | (np.isnan(a) & np.isnan(b)) | \
+        (np.abs(a - b) <= RTOL * np.maximum(np.abs(a), np.abs(b)) + ATOL)
+    if not same.all():
+        bad = np.nonzero(~same)[0]
+        return (f"{ctx} {name}: {bad.size} of {a.size} differ beyond {RTOL}, first at {bad[:5]}: "
+                f"{a[bad[:5]]} vs {b[bad[:5]]}")
+    return None
+
+
+def _exact(a, b, name, ctx):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return f"{ctx} {name}: shape {a.shape} != {b.shape}"
+    if not np.array_equal(a, b):
+        bad = np.nonzero(a != b)[0]
+        return f"{ctx} {name}: {bad.size} of {a.size} differ, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+    return None
+
+
+def compare_ca(o, g, ctx, internals=True):
+    """Collects every mismatching product of one event, then fails with all of them."""
+    errs = []
+    n_t = o["nTuples"]
+    if g["nTuples"] != n_t:
+        errs.append(f"{ctx}: tuples {g['nTuples']} != {n_t}")
+    if internals:
+        for k in ("nCells", "caStatus"):
+            if g[k] != o[k]:
+                errs.append(f"{ctx}: {k} {g[k]} != {o[k]}")
+        for k in EXACT_CA:
+            errs.append(_exact(o[k], g[k], k, ctx))
+        nm = int(o["multOff"][8])
+        errs.append(_exact(o["multBins"][:nm], g["multBins"], "multBins", ctx))
+        nh = o["nHits"]
+        errs.append(_exact(o["hitToTupleOff"][:nh + 1], g["hitToTupleOff"], "hitToTupleOff", ctx))
+        ne = int(o["hitToTupleOff"][nh])
+        errs.append(_exact(o["hitToTupleBins"][:ne], g["hitToTupleBins"], "hitToTupleBins", ctx))
+    for k in ("tupleOff", "tupleBins", "detIndices"):
+        errs.append(_exact(o[k], g[k], k, ctx))
+    errs.append(_exact(o["quality"][:n_t], g["quality"][:n_t], "quality", ctx))
+    for k in ("pt", "eta", "chi2"):
+        errs.append(_close(o[k][:n_t], g[k][:n_t], k, ctx))
+    errs.append(_close(o["state"][:5 * n_t], g["state"][:5 * n_t], "state", ctx))
+    errs.append(_close(o["covariance"][:15 * n_t], g["covariance"][:15 * n_t], "covariance", ctx))
+    if "nVertices" in g:
+        nv = o["nVertices"]
+        if g["nVertices"] != nv:
+            errs.append(f"{ctx}: vertices {g['nVertices']} != {nv}")
+        else:
+            errs.append(_exact(o["idv"][:n_t], g["idv"][:n_t], "idv", ctx))
+            for k in ("ndof", "sortInd"):
+                errs.append(_exact(o[k][:nv], g[k][:nv], k, ctx))
+            for k in ("zv", "wv", "chi2v", "ptv2"):
+                errs.append(_close(o[k][:nv], g[k][:nv], k, ctx))
+    errs = [e for e in errs if e]
+    assert not errs, "\n".join(errs)
+
+
+def _gpu_event(ctx_gpu, stream, raw, upto=5, internals=True):
+    from pxtrack import gpu
+    words, fed = gpu.unpack_raw(raw, ctx_gpu.fed_ids)
+    return stream.process(words, fed, upto=upto, internals=internals)
+
+
+@pytest.mark.parametrize("preset,index,kw", CASES)
+def test_plain_ntuplets_and_vertices_match_oracle(detector, oracle, gpu_ctx, gpu_stream, preset, index, kw):
+    raw, _ = event(detector, preset, index, **kw)
+    o = oracle.process(raw, upto=5)
+    print(f"{preset}[{index}] hits={o['nHits']} cells={o['nCells']} tuples={o['nTuples']} "
+          f"vertices={o['nVertices']}", flush=True)
+    g = _gpu_event(gpu_ctx, gpu_stream, raw)
+    compare_ca(o, g, f"{preset}[{index}]")
+
+
+def test_empty_event(detector, oracle, gpu_ctx, gpu_stream):
+    g = gpu_stream.process(np.zeros(0, np.uint32), np.zeros(0, np.uint8), upto=5, internals=True)
+    assert g["nTuples"] == 0 and g["nCells"] == 0 and g["nVertices"] == 0
+
+
+def test_batched_matches_plain(detector, oracle, gpu_ctx):
+    from pxtrack import gpu
+    evs = [event(detector, "pu20_80", i)[0] for i in range(6)]
+    unpacked = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in evs]
+    s = gpu_ctx.stream(max_events=len(evs), max_words=sum(w.size for w, _ in unpacked) + 256 * len(evs))
+    s.load(unpacked)
+    s.run(0, len(evs), upto=5)
+    s.fetch_counts(0, len(evs))
+    for e, raw in enumerate(evs):
+        print(f"batched event {e}", flush=True)
+        o = oracle.process(raw, upto=5)
+        g = s.event_products(e, upto=5)
+        g.update(s.ca_internals(e, int(g["nHits"])))
+        compare_ca(o, g, f"batch[{e}]")
+    # a sub-range of the resident events gives the same products
+    s.run(2, 3, upto=5)
+    s.fetch_counts(2, 3)
+    for e in range(2, 5):
+        o = oracle.process(evs[e], upto=5)
+        compare_ca(o, s.event_products(e, upto=5), f"sub-batch[{e}]", internals=False)
+
+
+def test_high_occupancy_profile(detector, conditions, gpu_ctx):
+    import pyoracle
+    from pxtrack import gpu
+    orc = pyoracle.Oracle(conditions, profile=1)
+    s = gpu_ctx.stream(max_events=1, max_words=300000, profile=gpu.PROFILE_HIGH_OCCUPANCY)
+    raw, _ = event(detector, "pu200", 1)
+    o = orc.process(raw, upto=5)
+    print(f"pu200 hits={o['nHits']} cells={o['nCells']} tuples={o['nTuples']} status={o['caStatus']:#x}", flush=True)
+    compare_ca(o, _gpu_event(gpu_ctx, s, raw), "pu200/high-occupancy")
