@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -127,6 +128,13 @@ def main():
         for first in range(0, n, B):
             stream.run(first, min(B, n - first), args.upto)
         return stream.fetch_counts(0, n)
+
+    # ---- launch shape chosen by the autotuner on the first batch
+    tuned = None
+    if not args.no_tune:
+        lanes, grid, tms = stream.autotune(0, B, args.upto)
+        tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": round(tms, 4)}
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
 
     # ---- batched path: warmup, then K timed steps (one step = all events)
     for _ in range(args.warmup):
@@ -226,7 +234,7 @@ def main():
                    "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
                    "stages": STAGES[args.upto]},
         "batched": {"events_per_s": round(evps, 1), "batch_latency_ms": round(batch_lat_ms, 4),
-                    "status": status},
+                    "status": status, "tuning": tuned},
         "plain": plain,
         "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
         "kernel_ms_per_launch": kernel_ms,

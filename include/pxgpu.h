@@ -132,7 +132,7 @@ typedef struct {
   uint32_t* multOff; uint16_t* multBins;   /* tuple multiplicity */
   uint32_t *hitToTupleOff; uint16_t* hitToTupleBins;
   uint32_t counts[16];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,
-                            nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors */
+                            nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors, nStart */
 } pxg_ca_internals;
 
 /* Context: device + conditions uploaded once, immutable afterwards (pxg_init is the
@@ -151,6 +151,8 @@ int pxg_stream_create(pxg_context* ctx, uint32_t max_events, uint32_t max_words,
 enum { PXG_PROFILE_REFERENCE = 0, PXG_PROFILE_HIGH_OCCUPANCY = 1 };
 int pxg_stream_create_profile(pxg_context* ctx, uint32_t max_events, uint32_t max_words, int profile,
                               pxg_stream** out);
+/* Device bytes a stream's pool of max_events events and max_words raw words takes. */
+size_t pxg_pool_bytes(uint32_t max_events, uint32_t max_words, int profile);
 void pxg_stream_destroy(pxg_stream* s);
 int pxg_sync(pxg_stream* s);
 
@@ -220,6 +222,59 @@ int pxg_event_ca_internals(pxg_stream* s, uint32_t e, pxg_ca_internals* out);
 
 /* Device -> host copy of product memory (validation, --transfer). */
 int pxg_copy_to_host(pxg_stream* s, void* dst, const void* src, size_t bytes);
+
+/* ---- serving loop (continuous batching) ---------------------------------------------- */
+
+/* No counterpart in the reference (the framework runs one event per EDM stream).  Events are
+ * submitted one by one from any host thread; a dispatcher thread admits waiting events into a
+ * batch whenever one of n_streams slots is free, as long as the batch's events (<= max_batch)
+ * and raw words fit the slot's device pool, and runs the chain on the batch as one launch per
+ * stage; slots run concurrently.  The pools are sized from hbm_budget (bytes over all slots;
+ * 0: max_batch events of 300000 words each).  Completed events carry their products packed in
+ * one host buffer (layout: 16 u32 header | tupleOff[nT+1] u32 | tupleBins u16 | quality u8 |
+ * state[5][nT] f32 | idv[nT] i16 | zv[nV] f32 | sortInd[nV] u16, parts 4-byte aligned, nBins
+ * tuple hits reserved as 5 nT). */
+typedef struct pxg_server pxg_server;
+typedef struct {
+  uint32_t max_batch;      /* events per batch (one launch per stage) */
+  uint32_t n_streams;      /* batches in flight */
+  uint64_t hbm_budget;     /* device bytes for all slots; 0 = no limit */
+  int profile;             /* PXG_PROFILE_* */
+  int upto;                /* last stage, 1..5 */
+  uint32_t doublet_lanes;  /* launch shape (0: default) */
+  uint32_t grid_target;
+} pxg_server_config;
+
+typedef struct {
+  uint64_t id;
+  int status;              /* PXG_STATUS_* bits of the event */
+  uint64_t t_submit_ns, t_launch_ns, t_done_ns;  /* steady clock */
+  uint32_t batch_size;     /* events in the batch that carried it */
+  uint32_t n_words, n_hits, n_tuples, n_vertices;
+} pxg_server_result;
+
+int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server** out);
+void pxg_server_destroy(pxg_server* sv); /* finishes the events already submitted */
+int pxg_server_submit(pxg_server* sv, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords,
+                      const pxg_beamspot* bs, uint64_t* id);
+/* Completed events (up to max), waiting up to wait_ms for the first; returns the number. */
+int pxg_server_poll(pxg_server* sv, pxg_server_result* out, int max, int wait_ms);
+/* Packed products of a polled event, valid until pxg_server_release. */
+int pxg_server_products(pxg_server* sv, uint64_t id, const void** packed, uint32_t* bytes);
+int pxg_server_release(pxg_server* sv, uint64_t id);
+int pxg_server_info(pxg_server* sv, uint32_t* max_batch_words, uint64_t* batches, uint64_t* events);
+
+/* ---- tuning ---------------------------------------------------------------------------- */
+
+/* Launch shape of the stream's kernels (products do not depend on it): lanes that scan one
+ * (layer pair, inner hit) slot of the doublet search together (power of two, 1..64), and
+ * the number of workgroups of a grid-stride launch, shared by the events of the launch. */
+int pxg_set_tuning(pxg_stream* s, uint32_t doublet_lanes, uint32_t grid_target);
+int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
+/* Times the chain on resident events [first, first+count) for each launch shape of a fixed
+ * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */
+int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* best_lanes,
+                 uint32_t* best_grid, float* best_ms);
 
 /* ---- profiling ------------------------------------------------------------------------ */
 

@@ -42,73 +42,125 @@ namespace pxg {
     __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
     __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
+    __device__ __forceinline__ uint32_t laneId() { return threadIdx.x & 63; }
+    __device__ __forceinline__ uint64_t lanesBelow() { return (1ull << laneId()) - 1; }
+    // wave index within the event's workgroups, uniform over the wave
+    __device__ __forceinline__ uint32_t waveStart() {
+      return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    }
+    __device__ __forceinline__ uint32_t waveStride() { return gridDim.x * (blockDim.x >> 6); }
+
     // ---- a11/a12 doublets: gpuPixelDoubletsAlgos.h:25-238 ----------------------------
-    // Visits the accepted outer hits of inner slot j (pair, inner hit) in the
-    // serial order: phi bins from the window's low edge upwards, each bin's
-    // hits in histogram order.
-    template <typename F>
-    __device__ __forceinline__ void forEachDoublet(const HitView& H, const PairTable& T, uint32_t j, F&& f) {
+    // One wave per inner slot j = (layer pair, inner hit); the lanes test the
+    // outer-layer hits of the phi window in parallel.  The window's hits are
+    // visited in the serial order (phi bins from the low edge upwards, each
+    // bin in histogram order), which is one or (across phi = pi) two
+    // contiguous ranges of the histogram contents; ballots compact the
+    // accepted ones in that order.
+    struct SlotScan {
+      bool live;            // the inner hit passes its own cuts
+      int pairId, inner, outer;
+      uint32_t i;           // inner hit
+      uint32_t s1, n1, s2, n;  // candidate ranges: [s1, s1+n1) then [s2, s2+n-n1)
+      float mez, mer;
+      int16_t mes, mep, iphicut;
+      float maxr;
+    };
+
+    __device__ __forceinline__ SlotScan slotOf(const HitView& H, const PairTable& T, uint32_t j) {
+      SlotScan q{};
       int pairId = 0;
       while (j >= T.cum[pairId])
         ++pairId;
-      const int inner = kPairInner[pairId], outer = kPairOuter[pairId];
-      const uint32_t i = j - (pairId ? T.cum[pairId - 1] : 0) + T.layer[inner];
-      if (H.detInd[i] > 2000)
-        return;
-      const float mez = H.zg[i];
-      if (mez < kPairMinZ[pairId] || mez > kPairMaxZ[pairId])
-        return;
-      constexpr int minYsizeB1 = 36, minYsizeB2 = 28, maxDYsize12 = 28, maxDYsize = 20, maxDYPred = 20;
+      q.pairId = pairId;
+      q.inner = kPairInner[pairId];
+      q.outer = kPairOuter[pairId];
+      q.i = j - (pairId ? T.cum[pairId - 1] : 0) + T.layer[q.inner];
+      q.live = false;
+      if (H.detInd[q.i] > 2000)
+        return q;
+      q.mez = H.zg[q.i];
+      if (q.mez < kPairMinZ[pairId] || q.mez > kPairMaxZ[pairId])
+        return q;
+      constexpr int minYsizeB1 = 36, minYsizeB2 = 28;
+      q.mes = H.ysize[q.i];  // ideal conditions: every ladder counts as an outer ladder
+      if (q.inner == 0 && q.outer > 3 && q.mes > 0 && q.mes < minYsizeB1)
+        return q;
+      if (q.inner == 1 && q.outer > 3 && q.mes > 0 && q.mes < minYsizeB2)
+        return q;
+      q.live = true;
+      q.mep = H.iphi[q.i];
+      q.mer = H.rg[q.i];
+      q.iphicut = kPairPhiCut[pairId];
+      q.maxr = kPairMaxR[pairId];
+      const uint32_t hoff = kPhiBins * q.outer;
+      const int kl = phiBinOf(int16_t(q.mep - q.iphicut));
+      const int kh = phiBinOf(int16_t(q.mep + q.iphicut));
+      if (kl <= kh) {
+        q.s1 = H.phiOff[hoff + kl];
+        q.n1 = H.phiOff[hoff + kh + 1] - q.s1;
+        q.s2 = 0;
+        q.n = q.n1;
+      } else {  // the window wraps through phi = pi
+        q.s1 = H.phiOff[hoff + kl];
+        q.n1 = H.phiOff[hoff + kPhiBins] - q.s1;
+        q.s2 = H.phiOff[hoff];
+        q.n = q.n1 + (H.phiOff[hoff + kh + 1] - q.s2);
+      }
+      return q;
+    }
+
+    // the cuts of one (inner, outer) hit pair: gpuPixelDoubletsAlgos.h:141-215
+    __device__ __forceinline__ bool acceptDoublet(const HitView& H, const SlotScan& q, uint32_t oi) {
+      constexpr int maxDYsize12 = 28, maxDYsize = 20, maxDYPred = 20;
       constexpr float dzdrFact = 8 * 0.0285 / 0.015;
       constexpr float z0cut = 12.f;
       constexpr float minRadius = 0.5f * 87.78f;
       constexpr float minRadius2T4 = 4.f * minRadius * minRadius;
       constexpr float i2p = 3.14159265358979323846 / 32768;
-      const int16_t mes = H.ysize[i];  // ideal conditions: every ladder counts as an outer ladder
-      if (inner == 0 && outer > 3 && mes > 0 && mes < minYsizeB1)
-        return;
-      if (inner == 1 && outer > 3 && mes > 0 && mes < minYsizeB2)
-        return;
-      const int16_t mep = H.iphi[i];
-      const float mer = H.rg[i];
-      const int16_t iphicut = kPairPhiCut[pairId];
-      const float maxr = kPairMaxR[pairId];
-      const bool onlyBarrel = outer < 4;
-      const int dy = inner == 0 ? maxDYsize12 : maxDYsize;
-      const uint32_t hoff = kPhiBins * outer;
-      const int kl = phiBinOf(int16_t(mep - iphicut));
-      const int kh = phiBinOf(int16_t(mep + iphicut));
-      const int khh = (kh + 1) % int(kPhiBins);
-      for (int kk = kl; kk != khh; kk = (kk + 1) % int(kPhiBins)) {
-        const uint32_t b = kk + hoff;
-        const uint32_t pe = H.phiOff[b + 1];
-        for (uint32_t pp = H.phiOff[b]; pp < pe; ++pp) {
-          const uint32_t oi = H.phiBins[pp];
-          if (H.detInd[oi] > 2000)
-            continue;
-          const float zo = H.zg[oi], ro = H.rg[oi];
-          const float dr = ro - mer;
-          if (dr > maxr || dr < 0 || fabsf(mez * ro - mer * zo) > z0cut * dr)
-            continue;
-          const int16_t mop = H.iphi[oi];
-          const int a1 = abs(int(int16_t(mop - mep))), a2 = abs(int(int16_t(mep - mop)));
-          const uint16_t idphi = uint16_t(a1 < a2 ? a1 : a2);
-          if (idphi > iphicut)
-            continue;
-          const int16_t so = H.ysize[oi];
-          const bool zsize =
-              onlyBarrel ? (mes > 0 && so > 0 && abs(so - mes) > dy)
-                         : (inner < 4) && mes > 0 && abs(mes - int(fabsf((mez - zo) / (mer - ro)) * dzdrFact + 0.5f)) > maxDYPred;
-          if (zsize)
-            continue;
-          const float dphi = float(int16_t(idphi)) * i2p;
-          if (dphi * dphi * (minRadius2T4 - mer * ro) > (ro - mer) * (ro - mer))
-            continue;
-          f(i, oi, pairId);
-        }
-      }
+      if (H.detInd[oi] > 2000)
+        return false;
+      const float zo = H.zg[oi], ro = H.rg[oi];
+      const float dr = ro - q.mer;
+      if (dr > q.maxr || dr < 0 || fabsf(q.mez * ro - q.mer * zo) > z0cut * dr)
+        return false;
+      const int16_t mop = H.iphi[oi];
+      const int a1 = abs(int(int16_t(mop - q.mep))), a2 = abs(int(int16_t(q.mep - mop)));
+      const uint16_t idphi = uint16_t(a1 < a2 ? a1 : a2);
+      if (idphi > q.iphicut)
+        return false;
+      const int16_t so = H.ysize[oi];
+      const int dy = q.inner == 0 ? maxDYsize12 : maxDYsize;
+      const bool zsize = q.outer < 4
+                             ? (q.mes > 0 && so > 0 && abs(so - q.mes) > dy)
+                             : (q.inner < 4) && q.mes > 0 &&
+                                   abs(q.mes - int(fabsf((q.mez - zo) / (q.mer - ro)) * dzdrFact + 0.5f)) > maxDYPred;
+      if (zsize)
+        return false;
+      const float dphi = float(int16_t(idphi)) * i2p;
+      return !(dphi * dphi * (minRadius2T4 - q.mer * ro) > (ro - q.mer) * (ro - q.mer));
     }
 
+    __device__ __forceinline__ uint32_t candidate(const HitView& H, const SlotScan& q, uint32_t k) {
+      return H.phiBins[k < q.n1 ? q.s1 + k : q.s2 + (k - q.n1)];
+    }
+
+    // G lanes per slot (G divides 64): 64 / G slots per wave
+    template <int G>
+    __device__ __forceinline__ uint32_t groupSlotStart() {
+      return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / G) + laneId() / G;
+    }
+    template <int G>
+    __device__ __forceinline__ uint32_t groupSlotStride() {
+      return gridDim.x * (blockDim.x >> 6) * (64 / G);
+    }
+    template <int G>
+    __device__ __forceinline__ uint64_t groupBits(uint64_t m) {
+      const uint32_t g = laneId() / G;
+      return G == 64 ? m : (m >> (g * G)) & ((1ull << G) - 1);
+    }
+
+    template <int G>
     __global__ void __launch_bounds__(kCaBlock) doubletCountKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.y;
       __shared__ PairTable T;
@@ -118,10 +170,19 @@ namespace pxg {
       __syncthreads();
       uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
       const uint32_t ntot = T.cum[kNumPairs - 1];
-      for (uint32_t j = gridStart(); j < ntot; j += gridStride()) {
+      const uint32_t sub = laneId() % G;
+      for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
+        const SlotScan q = slotOf(H, T, j);
         uint32_t n = 0;
-        forEachDoublet(H, T, j, [&](uint32_t, uint32_t, int) { ++n; });
-        dOff[j] = n;
+        if (q.live) {
+          for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
+            const uint32_t k = k0 + sub;
+            const bool ok = k < q.n && acceptDoublet(H, q, candidate(H, q, k));
+            n += __popcll(groupBits<G>(__ballot(ok)));
+          }
+        }
+        if (sub == 0)
+          dOff[j] = n;
       }
     }
 
@@ -159,6 +220,7 @@ namespace pxg {
       }
     }
 
+    template <int G>
     __global__ void __launch_bounds__(kCaBlock) doubletFillKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.y;
       __shared__ PairTable T;
@@ -171,24 +233,32 @@ namespace pxg {
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       uint32_t* ohc = ca.ohCount + size_t(e) * kHitStride;
-      for (uint32_t j = gridStart(); j < ntot; j += gridStride()) {
-        uint32_t c = dOff[j];
-        if (c >= nCells)
+      const uint32_t sub = laneId() % G;
+      const uint64_t below = (1ull << sub) - 1;
+      for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
+        const uint32_t base = dOff[j];
+        if (base >= nCells || dOff[j + 1] == base)
           continue;
-        forEachDoublet(H, T, j, [&](uint32_t i, uint32_t oi, int pairId) {
-          if (c < nCells) {
-            ca.cellInner[cb + c] = uint16_t(i);
+        const SlotScan q = slotOf(H, T, j);
+        uint32_t run = 0;
+        for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
+          const uint32_t k = k0 + sub;
+          uint32_t oi = 0;
+          bool ok = false;
+          if (k < q.n) {
+            oi = candidate(H, q, k);
+            ok = acceptDoublet(H, q, oi);
+          }
+          const uint64_t m = groupBits<G>(__ballot(ok));
+          const uint32_t c = base + run + __popcll(m & below);
+          if (ok && c < nCells) {
+            ca.cellInner[cb + c] = uint16_t(q.i);
             ca.cellOuter[cb + c] = uint16_t(oi);
-            ca.cellPair[cb + c] = uint8_t(pairId);
-            ca.cellFlags[cb + c] = 0;
-            ca.nbSlot[cb + c] = -1;
-            ca.nbCount[cb + c] = 0;
-            ca.trSlot[cb + c] = -1;
-            ca.trCount[cb + c] = 0;
+            ca.cellPair[cb + c] = uint8_t(q.pairId);
             atomicAdd(&ohc[oi], 1u);
           }
-          ++c;
-        });
+          run += __popcll(m);
+        }
       }
     }
 
@@ -225,40 +295,30 @@ namespace pxg {
         const uint32_t h = ca.cellOuter[cb + c];
         const uint32_t pos = atomicAdd(&cur[h], 1u);
         ca.ohCells[cb + off[h] + pos] = c;
+        ca.cellFlags[cb + c] = 0;
+        ca.nbSlot[cb + c] = -1;
+        ca.nbCount[cb + c] = 0;
+        ca.trSlot[cb + c] = -1;
+        ca.trCount[cb + c] = 0;
       }
     }
 
-    __global__ void __launch_bounds__(kCaBlock) ohSortKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
-      const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
-      const size_t cb = ca.cellBase(e);
-      const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
-      uint32_t* cells = ca.ohCells + cb;
-      bool over = false;
-      for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
-        const uint32_t b = off[h], n = off[h + 1] - b;
-        for (uint32_t k = 1; k < n; ++k) {  // insertion sort, ascending cell index
-          const uint32_t v = cells[b + k];
-          uint32_t m = k;
-          while (m > 0 && cells[b + m - 1] > v) {
-            cells[b + m] = cells[b + m - 1];
-            --m;
-          }
-          cells[b + m] = v;
-        }
-        for (uint32_t k = 0; k < n; ++k)
-          ca.cellFlags[cb + cells[b + k]] = k < kCaMaxCellsPerHit ? 1 : 0;
-        over |= n > kCaMaxCellsPerHit;
-      }
-      if (over)
-        atomicOr(&ca.cnt[e].status, kCaOuterHitOverflow);
+    // rank of cell c in the outer-hit list of h (ascending cell order); only
+    // needed when the list is longer than the 128 cells the serial container keeps
+    __device__ __forceinline__ bool listedInOuterHit(const uint32_t* cells, uint32_t b, uint32_t n, uint32_t c) {
+      if (n <= kCaMaxCellsPerHit)
+        return true;
+      uint32_t rank = 0;
+      for (uint32_t k = 0; k < n; ++k)
+        rank += cells[b + k] < c;
+      return rank < kCaMaxCellsPerHit;
     }
 
     // ---- a13 kernel_connect: CAHitNtupletGeneratorKernelsImpl.h:191-262 ----------------
     // Gathered per outer cell: the neighbours of cell `oc` are the cells `me`
     // whose inner hit is oc's outer hit and which pass the cuts, ascending in
     // `me` (the serial program's append order); `oc` must be among the first
-    // 128 cells of its outer hit's list.
+    // 128 cells of its outer hit's list (GPUCACell::OuterHitOfCell).
     __global__ void __launch_bounds__(kCaBlock) connectKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.y;
       __shared__ PairTable T;
@@ -269,13 +329,17 @@ namespace pxg {
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       const uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
-      uint32_t status = 0;
+      uint32_t status = 0, stored = 0;
       for (uint32_t oc = gridStart(); oc < nCells; oc += gridStride()) {
-        if (!(ca.cellFlags[cb + oc] & 1))
-          continue;
         const uint32_t hi = ca.cellInner[cb + oc], hm = ca.cellOuter[cb + oc];
+        const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
+        if (ln > kCaMaxCellsPerHit)
+          status |= kCaOuterHitOverflow;
+        if (!listedInOuterHit(ca.ohCells + cb, lb, ln, oc))
+          continue;
         const int lo = kPairOuter[ca.cellPair[cb + oc]];
         const float r1 = H.rg[hi], z1 = H.zg[hi], x1 = H.xg[hi], y1 = H.yg[hi];
         const float ri = H.rg[hm], zi = H.zg[hm], x2 = H.xg[hm], y2 = H.yg[hm];
@@ -320,57 +384,86 @@ namespace pxg {
         if (slot >= 0) {
           ca.nbSlot[cb + oc] = slot;
           ca.nbCount[cb + oc] = uint8_t(min(n, kCaMaxNeighbors));
-          atomicAdd(&ca.cnt[e].nNeighbors, min(n, kCaMaxNeighbors));
+          stored += min(n, kCaMaxNeighbors);
         }
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
+      if (stored)
+        atomicAdd(&ca.cnt[e].nNeighbors, stored);
     }
 
-    // ---- a14 fishbone: gpuFishbone.h:21-90 (per hit: independent of the others) --------
+    // ---- a14 fishbone: gpuFishbone.h:21-90 --------------------------------------------
+    // The serial loop visits the used cells of each outer-hit list (first 128,
+    // ascending) in rows: row ci kills the later aligned cells until the first
+    // aligned later cell that is closer, which kills ci and ends the row.  The
+    // rows do not read the kills, so each row runs on its own (one thread per
+    // cell), finding its first killer as the smallest such cell index.
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.y;
       const HitView H = hitsOf(p, e);
-      const uint32_t nHits = H.layerStart[10];
-      if (nHits < 2)
+      if (H.layerStart[10] < 2)
         return;
+      const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       const uint32_t* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
-      for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
-        const uint32_t b = off[h];
-        const uint32_t n = min(off[h + 1] - b, kCaMaxCellsPerHit);
+      for (uint32_t ci = gridStart(); ci < nCells; ci += gridStride()) {
+        if (!isUsed(ci))
+          continue;
+        const uint32_t h = ca.cellOuter[cb + ci];
+        const uint32_t b = off[h], n = off[h + 1] - b;
         if (n < 2)
           continue;
-        const float xo = H.xg[h], yo = H.yg[h], zo = H.zg[h];
-        for (uint32_t ic = 0; ic + 1 < n; ++ic) {
-          const uint32_t ci = cells[b + ic];
-          if (!isUsed(ci))
+        // with more than 128 cells only the 128 lowest indices take part
+        uint32_t limit = 0xffffffffu;
+        if (n > kCaMaxCellsPerHit) {
+          if (!listedInOuterHit(cells, b, n, ci))
             continue;
-          const uint32_t hi = ca.cellInner[cb + ci];
-          const uint16_t di = H.detInd[hi];
-          const float xi = H.xg[hi] - xo, yi = H.yg[hi] - yo, zi = H.zg[hi] - zo;
-          const float ni = xi * xi + yi * yi + zi * zi;
-          for (uint32_t jc = ic + 1; jc < n; ++jc) {
-            const uint32_t cj = cells[b + jc];
-            if (!isUsed(cj))
-              continue;
-            const uint32_t hj = ca.cellInner[cb + cj];
-            const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
-            const float nj = xj * xj + yj * yj + zj * zj;
-            const float cos12 = xi * xj + yi * yj + zi * zj;
-            if (di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj) {
-              if (ni > nj) {
-                ca.cellFlags[cb + ci] |= 2;
-                break;
-              } else {
-                ca.cellFlags[cb + cj] |= 2;
-              }
-            }
+          for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t c = cells[b + k];
+            uint32_t rank = 0;
+            for (uint32_t m = 0; m < n; ++m)
+              rank += cells[b + m] < c;
+            if (rank == kCaMaxCellsPerHit)
+              limit = c;
           }
         }
+        const float xo = H.xg[h], yo = H.yg[h], zo = H.zg[h];
+        const uint32_t hi = ca.cellInner[cb + ci];
+        const uint16_t di = H.detInd[hi];
+        const float xi = H.xg[hi] - xo, yi = H.yg[hi] - yo, zi = H.zg[hi] - zo;
+        const float ni = xi * xi + yi * yi + zi * zi;
+        auto aligned = [&](uint32_t cj, float& nj) {
+          const uint32_t hj = ca.cellInner[cb + cj];
+          const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
+          nj = xj * xj + yj * yj + zj * zj;
+          const float cos12 = xi * xj + yi * yj + zi * zj;
+          return di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj;
+        };
+        // first killer: the smallest later used cell, aligned and closer
+        uint32_t killer = limit;
+        for (uint32_t k = 0; k < n; ++k) {
+          const uint32_t cj = cells[b + k];
+          if (cj <= ci || cj >= killer || !isUsed(cj))
+            continue;
+          float nj;
+          if (aligned(cj, nj) && ni > nj)
+            killer = cj;
+        }
+        // the aligned later cells before it are killed
+        for (uint32_t k = 0; k < n; ++k) {
+          const uint32_t cj = cells[b + k];
+          if (cj <= ci || cj >= killer || !isUsed(cj))
+            continue;
+          float nj;
+          if (aligned(cj, nj))
+            ca.cellFlags[cb + cj] |= 2;  // several rows may kill a cell: same byte stored
+        }
+        if (killer != limit)
+          ca.cellFlags[cb + ci] |= 2;
       }
     }
 
@@ -379,7 +472,8 @@ namespace pxg {
     // order: tuples come out in the serial program's order.  Count mode counts
     // tuples and hits; fill mode writes them from the cell's scanned offsets.
     template <bool kFill>
-    __device__ void walk(const CAPool& ca, uint32_t e, uint32_t c, uint32_t& nT, uint32_t& nH, uint32_t& status) {
+    __device__ void walk(const CAPool& ca, uint32_t e, uint32_t c, uint32_t tBase, uint32_t hBase, uint32_t& nT,
+                         uint32_t& nH, uint32_t& status) {
       const size_t cb = ca.cellBase(e);
       const uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
       const size_t tb = ca.tupleBase(e);
@@ -390,11 +484,6 @@ namespace pxg {
       stack[0] = c;
       pos[0] = 0;
       any[0] = false;
-      uint32_t tBase = 0, hBase = 0;
-      if (kFill) {
-        tBase = ca.dfsT[cb + c];
-        hBase = ca.dfsH[cb + c];
-      }
       nT = nH = 0;
       while (depth > 0) {
         const int top = depth - 1;
@@ -447,24 +536,64 @@ namespace pxg {
       }
     }
 
-    __device__ __forceinline__ bool isStartCell(const CAPool& ca, size_t cb, uint32_t c) {
-      if (ca.cellFlags[cb + c] & 2)
-        return false;
-      const int pid = ca.cellPair[cb + c];
-      return pid < 8 || pid > 12;
+    // The walk starts from every live cell of a starting layer pair that has
+    // neighbours (a cell without neighbours makes no tuple).  These start cells
+    // are compacted in ascending order first: a bit per cell, a scan of the
+    // words' popcounts, and each word's cells written at its offset.
+    __global__ void __launch_bounds__(kCaBlock) startMaskKernel(CAPool ca) {
+      const uint32_t e = blockIdx.y;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint64_t* mask = ca.startMask + size_t(e) * (ca.caps.maxDoublets / 64 + 64);
+      const uint32_t words = (nCells + 63) / 64;
+      for (uint32_t c = gridStart(); c < words * 64; c += gridStride()) {
+        bool start = false;
+        if (c < nCells) {
+          const int pid = ca.cellPair[cb + c];
+          start = !(ca.cellFlags[cb + c] & 2) && (pid < 8 || pid > 12) && ca.nbSlot[cb + c] >= 0;
+        }
+        const uint64_t m = __ballot(start);
+        if ((threadIdx.x & 63) == 0)
+          mask[c / 64] = m;
+      }
+    }
+
+    __global__ void __launch_bounds__(kEventBlock) startCompactKernel(CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t lds[kEventBlock / 64];
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      const uint64_t* mask = ca.startMask + size_t(e) * (ca.caps.maxDoublets / 64 + 64);
+      uint32_t* list = ca.startCells + cb;
+      const uint32_t words = (nCells + 63) / 64;
+      uint32_t carry = 0;
+      for (uint32_t base = 0; base < words; base += kEventBlock) {
+        const uint32_t w = base + threadIdx.x;
+        uint64_t m = w < words ? mask[w] : 0;
+        uint32_t tot;
+        uint32_t at = carry + wgExclusiveScan<kEventBlock>(uint32_t(__popcll(m)), tot, lds);
+        while (m) {
+          const uint32_t bit = __ffsll((unsigned long long)m) - 1;
+          list[at++] = w * 64 + bit;
+          m &= m - 1;
+        }
+        carry += tot;
+      }
+      if (threadIdx.x == 0)
+        ca.cnt[e].nStart = carry;
     }
 
     __global__ void __launch_bounds__(kCaBlock) ntupletCountKernel(CAPool ca) {
       const uint32_t e = blockIdx.y;
-      const uint32_t nCells = ca.cnt[e].nCells;
+      const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
+      const uint32_t* list = ca.startCells + cb;
       uint32_t status = 0;
-      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+      for (uint32_t k = gridStart(); k < nStart; k += gridStride()) {
         uint32_t nT = 0, nH = 0;
-        if (isStartCell(ca, cb, c) && ca.nbSlot[cb + c] >= 0)
-          walk<false>(ca, e, c, nT, nH, status);
-        ca.dfsT[cb + c] = nT;
-        ca.dfsH[cb + c] = nH;
+        walk<false>(ca, e, list[k], 0, 0, nT, nH, status);
+        ca.dfsT[cb + k] = nT;
+        ca.dfsH[cb + k] = nH;
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
@@ -473,18 +602,18 @@ namespace pxg {
     __global__ void __launch_bounds__(kEventBlock) ntupletScanKernel(CAPool ca) {
       const uint32_t e = blockIdx.x;
       __shared__ uint32_t lds[kEventBlock / 64];
-      const uint32_t nCells = ca.cnt[e].nCells;
+      const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
       uint32_t* t = ca.dfsT + cb;
       uint32_t* h = ca.dfsH + cb;
       uint32_t carryT = 0, carryH = 0;
-      for (uint32_t base = 0; base < nCells; base += kEventBlock) {
+      for (uint32_t base = 0; base < nStart; base += kEventBlock) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t vt = i < nCells ? t[i] : 0, vh = i < nCells ? h[i] : 0;
+        const uint32_t vt = i < nStart ? t[i] : 0, vh = i < nStart ? h[i] : 0;
         uint32_t totT, totH;
         const uint32_t exT = wgExclusiveScan<kEventBlock>(vt, totT, lds);
         const uint32_t exH = wgExclusiveScan<kEventBlock>(vh, totH, lds);
-        if (i < nCells) {
+        if (i < nStart) {
           t[i] = carryT + exT;
           h[i] = carryH + exH;
         }
@@ -503,15 +632,17 @@ namespace pxg {
 
     __global__ void __launch_bounds__(kCaBlock) ntupletFillKernel(CAPool ca) {
       const uint32_t e = blockIdx.y;
-      const uint32_t nCells = ca.cnt[e].nCells;
+      const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
+      const uint32_t* list = ca.startCells + cb;
       uint32_t status = 0;
-      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
-        const uint32_t next = c + 1 < nCells ? ca.dfsT[cb + c + 1] : ca.cnt[e].apcM;
-        if (next == ca.dfsT[cb + c])
-          continue;  // starts no tuple
+      for (uint32_t k = gridStart(); k < nStart; k += gridStride()) {
+        const uint32_t tBase = ca.dfsT[cb + k];
+        const uint32_t next = k + 1 < nStart ? ca.dfsT[cb + k + 1] : ca.cnt[e].apcM;
+        if (next == tBase || tBase >= ca.caps.maxTuples)
+          continue;  // no tuple stored from this cell
         uint32_t nT, nH;
-        walk<true>(ca, e, c, nT, nH, status);
+        walk<true>(ca, e, list[k], tBase, ca.dfsH[cb + k], nT, nH, status);
       }
     }
 
@@ -732,18 +863,30 @@ namespace pxg {
     l.t->e();                                    \
   } while (0)
 
-  void launchDoublets(const CALaunch& l) {
+  template <int G>
+  void doubletKernels(const CALaunch& l) {
     const dim3 g(l.blocks, l.nEvents);
-    PXG_K("doubletCount", doubletCountKernel, g, kCaBlock, *l.p, *l.ca);
+    PXG_K("doubletCount", doubletCountKernel<G>, g, kCaBlock, *l.p, *l.ca);
     PXG_K("doubletScan", doubletScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
-    PXG_K("doubletFill", doubletFillKernel, g, kCaBlock, *l.p, *l.ca);
+    PXG_K("doubletFill", doubletFillKernel<G>, g, kCaBlock, *l.p, *l.ca);
+  }
+
+  void launchDoublets(const CALaunch& l) {
+    switch (l.doubletLanes) {
+      case 1: doubletKernels<1>(l); break;
+      case 2: doubletKernels<2>(l); break;
+      case 4: doubletKernels<4>(l); break;
+      case 8: doubletKernels<8>(l); break;
+      case 16: doubletKernels<16>(l); break;
+      case 32: doubletKernels<32>(l); break;
+      default: doubletKernels<64>(l); break;
+    }
   }
 
   void launchOuterHitLists(const CALaunch& l) {
     const dim3 g(l.blocks, l.nEvents);
     PXG_K("outerHitScan", ohScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
     PXG_K("outerHitFill", ohFillKernel, g, kCaBlock, *l.ca);
-    PXG_K("outerHitSort", ohSortKernel, g, kCaBlock, *l.p, *l.ca);
   }
 
   void launchConnect(const CALaunch& l) {
@@ -756,6 +899,8 @@ namespace pxg {
 
   void launchNtuplets(const CALaunch& l) {
     const dim3 g(l.blocks, l.nEvents);
+    PXG_K("startMask", startMaskKernel, g, kCaBlock, *l.ca);
+    PXG_K("startCompact", startCompactKernel, l.nEvents, kEventBlock, *l.ca);
     PXG_K("ntupletCount", ntupletCountKernel, g, kCaBlock, *l.ca);
     PXG_K("ntupletScan", ntupletScanKernel, l.nEvents, kEventBlock, *l.ca);
     PXG_K("ntupletFill", ntupletFillKernel, g, kCaBlock, *l.ca);

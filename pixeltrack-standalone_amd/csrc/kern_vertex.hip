@@ -21,7 +21,6 @@ namespace pxg {
       uint32_t counts[kMaxVertices];
       uint32_t start[kMaxVertices + 1];
       uint32_t running[kMaxVertices];
-      uint16_t chunkKey[kVB];
       uint32_t scan[kVB / 64];
       uint32_t nvIntermediate, nvFinal, found, nt;
     };
@@ -58,22 +57,28 @@ namespace pxg {
       __syncthreads();
       for (uint32_t k = threadIdx.x; k <= K; k += kVB)
         off[k] = L.start[k];
+      // rank of an item among the earlier items of its bin: within its wave by
+      // shuffles, across waves by visiting the waves in order
+      const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
       for (uint32_t base = 0; base < n; base += kVB) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t k = i < n ? key(i) : 0xffffu;
-        L.chunkKey[threadIdx.x] = uint16_t(min(k, 0xffffu));
-        __syncthreads();
-        if (i < n && k < K) {
-          uint32_t rank = L.running[k];
-          for (uint32_t j = 0; j < threadIdx.x; ++j)
-            rank += L.chunkKey[j] == k;
-          const uint32_t pos = descending ? L.start[k] + L.counts[k] - 1 - rank : L.start[k] + rank;
-          list[pos] = i;
+        const uint32_t k = i < n ? key(i) : 0xffffffffu;
+        uint32_t below = 0, same = 0;
+        for (uint32_t src = 0; src < 64; ++src) {
+          const uint32_t ks = __shfl(k, src);
+          same += ks == k;
+          below += ks == k && src < lane;
         }
-        __syncthreads();
-        if (i < n && k < K)
-          atomicAdd(&L.running[k], 1u);
-        __syncthreads();
+        for (uint32_t w = 0; w < kVB / 64; ++w) {
+          if (wid == w && i < n && k < K) {
+            const uint32_t rank = L.running[k] + below;
+            const uint32_t pos = descending ? L.start[k] + L.counts[k] - 1 - rank : L.start[k] + rank;
+            list[pos] = i;
+            if (below == same - 1)
+              L.running[k] += same;
+          }
+          __syncthreads();
+        }
       }
     }
 

@@ -12,6 +12,7 @@
 
 #include "../../include/pxgpu.h"
 #include "pxgpu_internal.h"
+#include "pxgpu_stream.h"
 
 using namespace pxg;
 
@@ -19,7 +20,9 @@ static_assert(sizeof(pxg_beamspot) == sizeof(BeamSpotPOD), "beam spot layout");
 
 namespace {
   thread_local std::string g_lastError;
+}  // namespace
 
+namespace pxg::detail {
   int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -29,6 +32,11 @@ namespace {
     g_lastError = buf;
     return code;
   }
+}  // namespace pxg::detail
+
+using pxg::detail::fail;
+
+namespace {
 
 #define PXG_HIP(call)                                                                          \
   do {                                                                                         \
@@ -51,34 +59,115 @@ namespace {
   uint32_t alignUp(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
 }  // namespace
 
-struct pxg_context {
-  int device = 0;
-  DevConditions cond{};
-  std::vector<void*> owned;
-};
+namespace pxg::detail {
+  // every device array of a pool, with its element count (shared by the
+  // allocation and by the memory budget of the serving loop)
+  template <typename F>
+  void forEachArray(Pool& p, CAPool& ca, uint32_t maxEvents, uint32_t W, F&& f) {
+    const size_t E = maxEvents, H = size_t(maxEvents) * kMaxNumClusters, M = size_t(maxEvents) * kMaxModuleRanges;
+    f(p.words, W);
+    f(p.fedIds, W / 2);
+    f(p.events, E);
+    f(p.counts, E);
+    f(p.xx, W);
+    f(p.yy, W);
+    f(p.adc, W);
+    f(p.moduleInd, W);
+    f(p.clus, W);
+    f(p.pdigi, W);
+    f(p.rawIdArr, W);
+    f(p.boundaryMask, 2 * (W / 64));
+    f(p.errorsSparse, W);
+    f(p.errors, W);
+    f(p.work, 64);
+    f(p.bigList, M);
+    f(p.moduleStart, M);
+    f(p.clusInModule, M);
+    f(p.moduleId, M);
+    f(p.clusModuleStart, M);
+    f(p.xl, H);
+    f(p.yl, H);
+    f(p.xerr, H);
+    f(p.yerr, H);
+    f(p.xg, H);
+    f(p.yg, H);
+    f(p.zg, H);
+    f(p.rg, H);
+    f(p.iphi, H);
+    f(p.charge, H);
+    f(p.xsize, H);
+    f(p.ysize, H);
+    f(p.detInd, H);
+    f(p.hitsLayerStart, E * kLayerStartStride);
+    f(p.phiOff, E * kPhiOffStride);
+    f(p.phiBins, H);
+    f(p.avg, E);
+    const size_t D = E * ca.caps.maxDoublets, T = E * ca.caps.maxTuples, HS = E * kHitStride;
+    const size_t L = E * ca.caps.maxCellLists;
+    f(ca.cnt, E);
+    f(ca.dOff, E * (kDoubletSlots + 64));
+    f(ca.cellInner, D);
+    f(ca.cellOuter, D);
+    f(ca.cellPair, D);
+    f(ca.cellFlags, D);
+    f(ca.cellUsed, E * (ca.caps.maxDoublets / 32 + 64));
+    f(ca.ohCount, HS);
+    f(ca.ohOff, HS);
+    f(ca.ohCells, D);
+    f(ca.startCells, D);
+    f(ca.nbSlot, D);
+    f(ca.nbCount, D);
+    f(ca.nbPool, L * kCaMaxNeighbors);
+    f(ca.startMask, E * (ca.caps.maxDoublets / 64 + 64));
+    f(ca.dfsT, D);
+    f(ca.dfsH, D);
+    f(ca.trSlot, D);
+    f(ca.trCount, D);
+    f(ca.trPool, L * kCaMaxCellTracks);
+    f(ca.multiCells, D);
+    f(ca.tupleOff, E * (ca.caps.maxTuples + 64));
+    f(ca.tupleBins, 5 * T);
+    f(ca.tupleCells, 4 * T);
+    f(ca.quality, T);
+    f(ca.multOff, 16 * E);
+    f(ca.multBins, E * ca.caps.maxTuplesMult);
+    f(ca.detIndices, 5 * T);
+    f(ca.state, 5 * T);
+    f(ca.cov, 15 * T);
+    f(ca.pt, T);
+    f(ca.eta, T);
+    f(ca.chi2, T);
+    f(ca.h2tCount, HS);
+    f(ca.h2tOff, HS);
+    f(ca.h2tBins, E * ca.caps.maxHitToTuple);
+    f(ca.idv, T);
+    f(ca.ndof, T);
+    f(ca.zv, E * kMaxVertices);
+    f(ca.wv, E * kMaxVertices);
+    f(ca.vchi2, E * kMaxVertices);
+    f(ca.ptv2, E * kMaxVertices);
+    f(ca.sortInd, E * kMaxVertices);
+    f(ca.vItrk, T);
+    f(ca.vZt, T);
+    f(ca.vEzt2, T);
+    f(ca.vPtt2, T);
+    f(ca.vIv, T);
+    f(ca.vTmp, T);
+    f(ca.vIzt, T);
+    f(ca.vTrk, T);
+    f(ca.vOff, E * (kMaxVertices + 64));
+  }
 
-struct ProfileMark {
-  const char* name;
-  hipEvent_t start, stop;
-};
+  size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps) {
+    Pool p{};
+    CAPool ca{};
+    ca.caps = caps;
+    const uint32_t W = alignUp(maxWords, kWordAlign) + kWordAlign * maxEvents;
+    size_t total = 0;
+    forEachArray(p, ca, maxEvents, W, [&](auto* ptr, size_t n) { total += n * sizeof(*ptr) + 256; });
+    return total;
+  }
 
-struct pxg_stream {
-  pxg_context* ctx = nullptr;
-  hipStream_t stream = nullptr;
-  Pool pool{};
-  CAPool ca{};
-  CACounts* hostCA = nullptr;  // pinned
-  std::vector<void*> owned;
-  EventDesc* hostEvents = nullptr;    // pinned
-  EventCounts* hostCounts = nullptr;  // pinned
-  uint32_t nLoaded = 0;
-  uint32_t wordCapacity = 0;
-  bool profiling = false;
-  std::vector<ProfileMark> marks;
-  std::map<std::string, std::pair<uint32_t, double>> profile;
-};
-
-namespace {
   // per-event arrays of the pool shifted to event `first`
   Pool viewFrom(const Pool& p, uint32_t first) {
     Pool v = p;
@@ -125,9 +214,11 @@ namespace {
     v.ohCount += H;
     v.ohOff += H;
     v.ohCells += D;
+    v.startCells += D;
     v.nbSlot += D;
     v.nbCount += D;
     v.nbPool += L * kCaMaxNeighbors;
+    v.startMask += size_t(first) * (c.caps.maxDoublets / 64 + 64);
     v.dfsT += D;
     v.dfsH += D;
     v.trSlot += D;
@@ -231,7 +322,8 @@ namespace {
     Timer timer;
     if (s->profiling)
       timer = {timerBegin, timerEnd, s};
-    const CALaunch l{&c, &v, &ca, count, std::max(4u, std::min(512u, 2048u / count)), s->stream, &timer};
+    const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
+                     s->stream, &timer, s->doubletLanes};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -299,7 +391,44 @@ namespace {
             cnt.nClusters};
     }
   }
-}  // namespace
+}  // namespace pxg::detail
+
+using namespace pxg::detail;
+
+int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync) {
+  if (!s || !raws || !bs || n == 0)
+    return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
+  if (n > s->pool.maxEvents)
+    return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
+  uint32_t off = 0, blocks = 0;
+  for (uint32_t e = 0; e < n; ++e) {
+    if (raws[e].nWords % 2)
+      return fail(PXG_ERR_ARG, "event %u: odd number of words %u", e, raws[e].nWords);
+    if (raws[e].nWords > kMaxFedWords)
+      return fail(PXG_ERR_CAPACITY, "event %u: %u words > %u", e, raws[e].nWords, kMaxFedWords);
+    if (uint64_t(off) + raws[e].nWords > s->wordCapacity)
+      return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u words exceed the pool", off + raws[e].nWords);
+    EventDesc& d = s->hostEvents[e];
+    d.wordOffset = off;
+    d.nWords = raws[e].nWords;
+    d.blockOffset = blocks;
+    d.pad = 0;
+    std::memcpy(&d.bs, bs, sizeof(BeamSpotPOD));
+    if (raws[e].nWords) {
+      PXG_HIP(hipMemcpyAsync(s->pool.words + off, raws[e].words, size_t(raws[e].nWords) * 4, hipMemcpyHostToDevice,
+                             s->stream));
+      PXG_HIP(hipMemcpyAsync(s->pool.fedIds + off / 2, raws[e].fedIds, raws[e].nWords / 2, hipMemcpyHostToDevice,
+                             s->stream));
+    }
+    blocks += (raws[e].nWords + kDecodeBlock - 1) / kDecodeBlock;
+    off = alignUp(off + raws[e].nWords, kWordAlign);
+  }
+  PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, n * sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
+  if (sync)
+    PXG_HIP(hipStreamSynchronize(s->stream));
+  s->nLoaded = n;
+  return 0;
+}
 
 extern "C" {
 
@@ -387,101 +516,12 @@ int pxg_stream_create_profile(pxg_context* ctx, uint32_t maxEvents, uint32_t max
   const size_t E = maxEvents, H = size_t(maxEvents) * kMaxNumClusters, M = size_t(maxEvents) * kMaxModuleRanges;
   auto& o = s->owned;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-#define ALLOC(ptr, n) \
-  if (e == hipSuccess) \
-  e = devAlloc(ptr, n, o)
-  ALLOC(p.words, W);
-  ALLOC(p.fedIds, W / 2);
-  ALLOC(p.events, E);
-  ALLOC(p.counts, E);
-  ALLOC(p.xx, W);
-  ALLOC(p.yy, W);
-  ALLOC(p.adc, W);
-  ALLOC(p.moduleInd, W);
-  ALLOC(p.clus, W);
-  ALLOC(p.pdigi, W);
-  ALLOC(p.rawIdArr, W);
-  ALLOC(p.boundaryMask, 2 * (W / 64));
-  ALLOC(p.errorsSparse, W);
-  ALLOC(p.errors, W);
-  ALLOC(p.work, 64);
-  ALLOC(p.bigList, M);
-  ALLOC(p.moduleStart, M);
-  ALLOC(p.clusInModule, M);
-  ALLOC(p.moduleId, M);
-  ALLOC(p.clusModuleStart, M);
-  ALLOC(p.xl, H);
-  ALLOC(p.yl, H);
-  ALLOC(p.xerr, H);
-  ALLOC(p.yerr, H);
-  ALLOC(p.xg, H);
-  ALLOC(p.yg, H);
-  ALLOC(p.zg, H);
-  ALLOC(p.rg, H);
-  ALLOC(p.iphi, H);
-  ALLOC(p.charge, H);
-  ALLOC(p.xsize, H);
-  ALLOC(p.ysize, H);
-  ALLOC(p.detInd, H);
-  ALLOC(p.hitsLayerStart, E * kLayerStartStride);
-  ALLOC(p.phiOff, E * kPhiOffStride);
-  ALLOC(p.phiBins, H);
-  ALLOC(p.avg, E);
   CAPool& ca = s->ca;
   ca.caps = profile == PXG_PROFILE_HIGH_OCCUPANCY ? kCapsHighOccupancy : kCapsReference;
-  const size_t D = E * ca.caps.maxDoublets, T = E * ca.caps.maxTuples, HS = E * kHitStride;
-  const size_t L = E * ca.caps.maxCellLists;
-  ALLOC(ca.cnt, E);
-  ALLOC(ca.dOff, E * (kDoubletSlots + 64));
-  ALLOC(ca.cellInner, D);
-  ALLOC(ca.cellOuter, D);
-  ALLOC(ca.cellPair, D);
-  ALLOC(ca.cellFlags, D);
-  ALLOC(ca.cellUsed, E * (ca.caps.maxDoublets / 32 + 64));
-  ALLOC(ca.ohCount, HS);
-  ALLOC(ca.ohOff, HS);
-  ALLOC(ca.ohCells, D);
-  ALLOC(ca.nbSlot, D);
-  ALLOC(ca.nbCount, D);
-  ALLOC(ca.nbPool, L * kCaMaxNeighbors);
-  ALLOC(ca.dfsT, D);
-  ALLOC(ca.dfsH, D);
-  ALLOC(ca.trSlot, D);
-  ALLOC(ca.trCount, D);
-  ALLOC(ca.trPool, L * kCaMaxCellTracks);
-  ALLOC(ca.multiCells, D);
-  ALLOC(ca.tupleOff, E * (ca.caps.maxTuples + 64));
-  ALLOC(ca.tupleBins, 5 * T);
-  ALLOC(ca.tupleCells, 4 * T);
-  ALLOC(ca.quality, T);
-  ALLOC(ca.multOff, 16 * E);
-  ALLOC(ca.multBins, E * ca.caps.maxTuplesMult);
-  ALLOC(ca.detIndices, 5 * T);
-  ALLOC(ca.state, 5 * T);
-  ALLOC(ca.cov, 15 * T);
-  ALLOC(ca.pt, T);
-  ALLOC(ca.eta, T);
-  ALLOC(ca.chi2, T);
-  ALLOC(ca.h2tCount, HS);
-  ALLOC(ca.h2tOff, HS);
-  ALLOC(ca.h2tBins, E * ca.caps.maxHitToTuple);
-  ALLOC(ca.idv, T);
-  ALLOC(ca.ndof, T);
-  ALLOC(ca.zv, E * kMaxVertices);
-  ALLOC(ca.wv, E * kMaxVertices);
-  ALLOC(ca.vchi2, E * kMaxVertices);
-  ALLOC(ca.ptv2, E * kMaxVertices);
-  ALLOC(ca.sortInd, E * kMaxVertices);
-  ALLOC(ca.vItrk, T);
-  ALLOC(ca.vZt, T);
-  ALLOC(ca.vEzt2, T);
-  ALLOC(ca.vPtt2, T);
-  ALLOC(ca.vIv, T);
-  ALLOC(ca.vTmp, T);
-  ALLOC(ca.vIzt, T);
-  ALLOC(ca.vTrk, T);
-  ALLOC(ca.vOff, E * (kMaxVertices + 64));
-#undef ALLOC
+  forEachArray(p, ca, maxEvents, W, [&](auto*& ptr, size_t n) {
+    if (e == hipSuccess)
+      e = devAlloc(ptr, n, o);
+  });
   if (e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&s->hostEvents), E * sizeof(EventDesc), hipHostMallocDefault);
   if (e == hipSuccess)
@@ -534,37 +574,7 @@ int pxg_sync(pxg_stream* s) {
 }
 
 int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs) {
-  if (!s || !raws || !bs || n == 0)
-    return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
-  if (n > s->pool.maxEvents)
-    return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
-  uint32_t off = 0, blocks = 0;
-  for (uint32_t e = 0; e < n; ++e) {
-    if (raws[e].nWords % 2)
-      return fail(PXG_ERR_ARG, "event %u: odd number of words %u", e, raws[e].nWords);
-    if (raws[e].nWords > kMaxFedWords)
-      return fail(PXG_ERR_CAPACITY, "event %u: %u words > %u", e, raws[e].nWords, kMaxFedWords);
-    if (uint64_t(off) + raws[e].nWords > s->wordCapacity)
-      return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u words exceed the pool", off + raws[e].nWords);
-    EventDesc& d = s->hostEvents[e];
-    d.wordOffset = off;
-    d.nWords = raws[e].nWords;
-    d.blockOffset = blocks;
-    d.pad = 0;
-    std::memcpy(&d.bs, bs, sizeof(BeamSpotPOD));
-    if (raws[e].nWords) {
-      PXG_HIP(hipMemcpyAsync(s->pool.words + off, raws[e].words, size_t(raws[e].nWords) * 4, hipMemcpyHostToDevice,
-                             s->stream));
-      PXG_HIP(hipMemcpyAsync(s->pool.fedIds + off / 2, raws[e].fedIds, raws[e].nWords / 2, hipMemcpyHostToDevice,
-                             s->stream));
-    }
-    blocks += (raws[e].nWords + kDecodeBlock - 1) / kDecodeBlock;
-    off = alignUp(off + raws[e].nWords, kWordAlign);
-  }
-  PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, n * sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
-  PXG_HIP(hipStreamSynchronize(s->stream));
-  s->nLoaded = n;
-  return 0;
+  return loadEvents(s, n, raws, bs, true);
 }
 
 int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
@@ -730,6 +740,77 @@ int pxg_find_vertices(pxg_stream* s, const pxg_tracks* t, pxg_vertices* v) {
     return rc;
   fillTracks(s, 0, nullptr, v);
   return rc;
+}
+
+size_t pxg_pool_bytes(uint32_t maxEvents, uint32_t maxWords, int profile) {
+  return poolBytes(maxEvents, maxWords, profile == PXG_PROFILE_HIGH_OCCUPANCY ? kCapsHighOccupancy : kCapsReference);
+}
+
+// ---- tuning ---------------------------------------------------------------------------
+int pxg_set_tuning(pxg_stream* s, uint32_t doubletLanes, uint32_t gridTarget) {
+  if (!s || doubletLanes == 0 || doubletLanes > 64 || (doubletLanes & (doubletLanes - 1)) || gridTarget < 4)
+    return fail(PXG_ERR_ARG, "pxg_set_tuning: lanes must be a power of two in [1, 64], grid target >= 4");
+  s->doubletLanes = doubletLanes;
+  s->gridTarget = gridTarget;
+  return 0;
+}
+
+int pxg_get_tuning(pxg_stream* s, uint32_t* doubletLanes, uint32_t* gridTarget) {
+  if (!s || !doubletLanes || !gridTarget)
+    return fail(PXG_ERR_ARG, "pxg_get_tuning: null argument");
+  *doubletLanes = s->doubletLanes;
+  *gridTarget = s->gridTarget;
+  return 0;
+}
+
+int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* bestLanes,
+                 uint32_t* bestGrid, float* bestMs) {
+  if (!s || upto < 1 || upto > 5 || count == 0 || first + count > s->nLoaded)
+    return fail(PXG_ERR_ARG, "pxg_autotune: bad arguments");
+  static const uint32_t lanes[] = {2, 4, 8, 16, 32, 64};
+  static const uint32_t grids[] = {1024, 2048, 4096, 8192};
+  const bool wasProfiling = s->profiling;
+  s->profiling = false;
+  hipEvent_t a, b;
+  PXG_HIP(hipEventCreate(&a));
+  PXG_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  uint32_t bl = s->doubletLanes, bg = s->gridTarget;
+  for (uint32_t g : grids)
+    for (uint32_t ln : lanes) {
+      s->doubletLanes = ln;
+      s->gridTarget = g;
+      int rc = runStages(s, first, count, 1, upto);  // warm
+      float ms = 0;
+      if (!rc) {
+        (void)hipEventRecord(a, s->stream);
+        rc = runStages(s, first, count, 1, upto);
+        (void)hipEventRecord(b, s->stream);
+      }
+      if (rc || hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        s->profiling = wasProfiling;
+        return rc ? rc : fail(PXG_ERR_HIP, "pxg_autotune: timing failed");
+      }
+      if (ms < best) {
+        best = ms;
+        bl = ln;
+        bg = g;
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  s->doubletLanes = bl;
+  s->gridTarget = bg;
+  s->profiling = wasProfiling;
+  if (bestLanes)
+    *bestLanes = bl;
+  if (bestGrid)
+    *bestGrid = bg;
+  if (bestMs)
+    *bestMs = best;
+  return 0;
 }
 
 // ---- profiling ------------------------------------------------------------------------

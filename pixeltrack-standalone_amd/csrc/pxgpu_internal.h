@@ -123,7 +123,7 @@ namespace pxg {
     uint32_t vstatus;      // 1: vertex container full
     uint32_t nSlots;       // (layer pair, inner hit) slots of the doublet search
     uint32_t nNeighbors;   // neighbour entries stored
-    uint32_t pad;
+    uint32_t nStart;       // cells the ntuplet walk starts from
   };
 
   // Device arrays of the ntuplet and vertex stages, all per event at e * stride
@@ -138,11 +138,13 @@ namespace pxg {
     uint32_t* cellUsed;      // bit per cell [maxDoublets / 32 + 64]
     uint32_t* ohCount;       // [kHitStride] cells per outer hit, then fill cursors
     uint32_t* ohOff;         // [kHitStride]
-    uint32_t* ohCells;       // [maxDoublets] cells by outer hit, ascending within a hit
+    uint32_t* ohCells;       // [maxDoublets] cells by outer hit (fill order)
+    uint32_t* startCells;    // [maxDoublets] start cells of the ntuplet walk, ascending
     int32_t* nbSlot;         // [maxDoublets] neighbour list slot (-1 none, -2 store full)
     uint8_t* nbCount;        // [maxDoublets] neighbours stored (<= 36)
     uint32_t* nbPool;        // [maxCellLists * 36]
-    uint32_t *dfsT, *dfsH;   // [maxDoublets] tuples / hits started by a cell, then offsets
+    uint64_t* startMask;     // [maxDoublets / 64 + 64] start cells of the ntuplet walk
+    uint32_t *dfsT, *dfsH;   // [maxDoublets] tuples / hits per start cell, then offsets
     int32_t* trSlot;         // [maxDoublets] track list slot (-1 none, -2 store full)
     uint32_t* trCount;       // [maxDoublets] tracks stored (<= 48)
     uint16_t* trPool;        // [maxCellLists * 48]
@@ -203,6 +205,7 @@ namespace pxg {
     uint32_t nEvents, blocks;
     hipStream_t s;
     const Timer* t;
+    uint32_t doubletLanes;  // lanes per (layer pair, inner hit) slot of the doublet search: 1..64
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);

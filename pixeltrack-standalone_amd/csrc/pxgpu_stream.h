@@ -1,0 +1,57 @@
+// Host-side state of libpxgpu.so shared by the C ABI (pxgpu_api.hip) and the
+// serving loop (pxgpu_server.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pxgpu.h"
+#include "pxgpu_internal.h"
+
+struct pxg_context {
+  int device = 0;
+  pxg::DevConditions cond{};
+  std::vector<void*> owned;
+};
+
+struct ProfileMark {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct pxg_stream {
+  pxg_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  pxg::Pool pool{};
+  pxg::CAPool ca{};
+  pxg::CACounts* hostCA = nullptr;  // pinned
+  std::vector<void*> owned;
+  pxg::EventDesc* hostEvents = nullptr;    // pinned
+  pxg::EventCounts* hostCounts = nullptr;  // pinned
+  uint32_t nLoaded = 0;
+  uint32_t wordCapacity = 0;
+  bool profiling = false;
+  uint32_t doubletLanes = 8;   // tuning: lanes per doublet slot
+  uint32_t gridTarget = 2048;  // tuning: workgroups per grid-stride launch (divided among the events)
+  std::vector<ProfileMark> marks;
+  std::map<std::string, std::pair<uint32_t, double>> profile;
+};
+
+
+namespace pxg::detail {
+  // records a message for pxg_last_error and returns code
+  int fail(int code, const char* fmt, ...);
+  // per-event arrays shifted to event `first`
+  Pool viewFrom(const Pool& p, uint32_t first);
+  CAPool caViewFrom(const CAPool& c, uint32_t first);
+  // enqueues stages [from, upto] on resident events [first, first + count)
+  int runStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto);
+  // copies n events into the pool; host arrays must stay valid until the copies
+  // complete (pinned and kept alive by the caller when sync is false)
+  int loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync);
+  // bytes of device memory a pool of maxEvents events and maxWords words takes
+  size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps);
+}  // namespace pxg::detail

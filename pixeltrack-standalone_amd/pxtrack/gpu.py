@@ -63,8 +63,19 @@ class CAInternals(C.Structure):
                                   "hitToTupleBins")] + [("counts", u32 * 16)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", u32), ("n_streams", u32), ("hbm_budget", C.c_uint64), ("profile", C.c_int),
+                ("upto", C.c_int), ("doublet_lanes", u32), ("grid_target", u32)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("t_submit_ns", C.c_uint64), ("t_launch_ns", C.c_uint64),
+                ("t_done_ns", C.c_uint64), ("batch_size", u32), ("n_words", u32), ("n_hits", u32),
+                ("n_tuples", u32), ("n_vertices", u32)]
+
+
 CA_COUNTS = ["nCells", "nDoublets", "apcM", "apcN", "nTuples", "caStatus", "nbLists", "trLists", "nMulti", "nMult",
-             "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors"]
+             "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors", "nStart"]
 PROFILE_REFERENCE, PROFILE_HIGH_OCCUPANCY = 0, 1
 
 
@@ -89,6 +100,19 @@ def _gpu():
         lib.pxg_find_vertices.argtypes = [vp, C.POINTER(Tracks), C.POINTER(Vertices)]
         lib.pxg_event_tracks.argtypes = [vp, u32, C.POINTER(Tracks), C.POINTER(Vertices)]
         lib.pxg_event_ca_internals.argtypes = [vp, u32, C.POINTER(CAInternals)]
+        lib.pxg_pool_bytes.restype = C.c_size_t
+        lib.pxg_pool_bytes.argtypes = [u32, u32, C.c_int]
+        lib.pxg_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+        lib.pxg_server_destroy.argtypes = [vp]
+        lib.pxg_server_submit.argtypes = [vp, vp, vp, u32, C.POINTER(BeamSpot), C.POINTER(C.c_uint64)]
+        lib.pxg_server_poll.argtypes = [vp, C.POINTER(ServerResult), C.c_int, C.c_int]
+        lib.pxg_server_products.argtypes = [vp, C.c_uint64, C.POINTER(vp), C.POINTER(u32)]
+        lib.pxg_server_release.argtypes = [vp, C.c_uint64]
+        lib.pxg_server_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        lib.pxg_set_tuning.argtypes = [vp, u32, u32]
+        lib.pxg_get_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
+        lib.pxg_autotune.argtypes = [vp, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32),
+                                     C.POINTER(C.c_float)]
         lib.pxg_stream_destroy.argtypes = [vp]
         lib.pxg_sync.argtypes = [vp]
         lib.pxg_raw_to_digi.argtypes = [vp, vp, vp, u32, C.POINTER(Digis), C.POINTER(Errors)]
@@ -268,7 +292,9 @@ class Stream:
         out["cellUsed"] = bits.astype(np.uint16)
         ohoff = self._copy(ci.outerHitOff, np.uint32, n_hits + 1)
         ohc = self._copy(ci.outerHitCells, np.uint32, int(ohoff[-1]) if n_hits else 0)
-        lists = [ohc[ohoff[h]:ohoff[h + 1]][:128] for h in range(n_hits)]
+        # the device keeps each hit's cells in fill order; the serial container holds the
+        # first 128 in ascending order
+        lists = [np.sort(ohc[ohoff[h]:ohoff[h + 1]])[:128] for h in range(n_hits)]
         out["outerHitOff"] = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.uint32)
         out["outerHitCells"] = np.concatenate(lists).astype(np.uint32) if lists else np.zeros(0, np.uint32)
 
@@ -394,6 +420,21 @@ class Stream:
         _check(self.lib.pxg_event_counts(self._h, e, out), "pxg_event_counts")
         return tuple(out)
 
+    def set_tuning(self, doublet_lanes, grid_target):
+        _check(self.lib.pxg_set_tuning(self._h, doublet_lanes, grid_target), "pxg_set_tuning")
+
+    def tuning(self):
+        a, b = u32(), u32()
+        _check(self.lib.pxg_get_tuning(self._h, C.byref(a), C.byref(b)), "pxg_get_tuning")
+        return a.value, b.value
+
+    def autotune(self, first, count, upto=5):
+        """Chooses the fastest launch shape on resident events; returns (lanes, grid, ms per pass)."""
+        a, b, ms = u32(), u32(), C.c_float()
+        _check(self.lib.pxg_autotune(self._h, first, count, upto, C.byref(a), C.byref(b), C.byref(ms)),
+               "pxg_autotune")
+        return a.value, b.value, ms.value
+
     def sync(self):
         _check(self.lib.pxg_sync(self._h), "pxg_sync")
 
@@ -405,3 +446,86 @@ class Stream:
         names, calls, ms = (C.c_char_p * n)(), (u32 * n)(), (C.c_double * n)()
         k = _check(self.lib.pxg_profile_read(self._h, names, calls, ms, n), "pxg_profile_read")
         return {names[i].decode(): (calls[i], ms[i]) for i in range(min(k, n))}
+
+
+def unpack_products(buf):
+    """Splits one event's packed products (pxg_server_products layout) into arrays."""
+    b = np.frombuffer(buf, dtype=np.uint8)
+    h = b[:64].view(np.uint32)
+    nT, nV = int(h[5]), int(min(h[6], 1024))
+
+    def a4(v):
+        return (v + 3) & ~3
+
+    o = 64
+    out = {"nWords": int(h[0]), "nModules": int(h[1]), "nHits": int(h[2]), "status": int(h[3]),
+           "caStatus": int(h[4]), "nTuples": nT, "nVertices": int(h[6]), "nCells": int(h[8])}
+    out["tupleOff"] = b[o:o + 4 * (nT + 1)].view(np.uint32).copy()
+    o += a4(4 * (nT + 1))
+    nb = int(h[7])
+    out["tupleBins"] = b[o:o + 2 * nb].view(np.uint16).copy()
+    o += a4(2 * 5 * nT)
+    out["quality"] = b[o:o + nT].copy()
+    o += a4(nT)
+    out["state"] = b[o:o + 20 * nT].view(np.float32).reshape(5, nT).T.copy().ravel()
+    o += 20 * nT
+    out["idv"] = b[o:o + 2 * nT].view(np.int16).copy()
+    o += a4(2 * nT)
+    out["zv"] = b[o:o + 4 * nV].view(np.float32).copy()
+    o += 4 * nV
+    out["sortInd"] = b[o:o + 2 * nV].view(np.uint16).copy()
+    return out
+
+
+class Server:
+    """Continuous-batching serving loop of the library (pxg_server_*)."""
+
+    def __init__(self, ctx, max_batch=64, n_streams=3, hbm_budget=0, profile=PROFILE_REFERENCE, upto=5,
+                 doublet_lanes=0, grid_target=0):
+        self.ctx, self.lib = ctx, ctx.lib
+        cfg = ServerConfig(max_batch, n_streams, hbm_budget, profile, upto, doublet_lanes, grid_target)
+        self._h = vp()
+        _check(self.lib.pxg_server_create(ctx._h, C.byref(cfg), C.byref(self._h)), "pxg_server_create")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.pxg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, words, fed_ids, beamspot=None):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        fed_ids = np.ascontiguousarray(fed_ids, dtype=np.uint8)
+        rid = C.c_uint64()
+        bs = beamspot if beamspot is not None else self.ctx.beamspot
+        _check(self.lib.pxg_server_submit(self._h, words.ctypes.data, fed_ids.ctypes.data, words.size, C.byref(bs),
+                                          C.byref(rid)), "pxg_server_submit")
+        return rid.value
+
+    def poll(self, max_results=1024, wait_ms=0):
+        out = (ServerResult * max_results)()
+        n = _check(self.lib.pxg_server_poll(self._h, out, max_results, wait_ms), "pxg_server_poll")
+        return [{f: getattr(out[i], f) for f, _ in ServerResult._fields_} for i in range(n)]
+
+    def products(self, rid, release=True):
+        ptr, n = vp(), u32()
+        _check(self.lib.pxg_server_products(self._h, rid, C.byref(ptr), C.byref(n)), "pxg_server_products")
+        buf = bytes((C.c_uint8 * n.value).from_address(ptr.value))
+        if release:
+            self.lib.pxg_server_release(self._h, rid)
+        return unpack_products(buf)
+
+    def release(self, rid):
+        self.lib.pxg_server_release(self._h, rid)
+
+    def info(self):
+        w, b, e = u32(), C.c_uint64(), C.c_uint64()
+        _check(self.lib.pxg_server_info(self._h, C.byref(w), C.byref(b), C.byref(e)), "pxg_server_info")
+        return {"max_batch_words": w.value, "batches": b.value, "events": e.value}
+
+
+def pool_bytes(max_events, max_words, profile=PROFILE_REFERENCE):
+    """Device memory of a stream pool (pxg_pool_bytes)."""
+    return int(_gpu().pxg_pool_bytes(max_events, max_words, profile))

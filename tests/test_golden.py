@@ -50,5 +50,8 @@ def test_gpu_matches_golden(conditions, gpu_ctx, gpu_stream, path):
     n = want["nDigis"]
     got_digis = {k: got[k][:n] for k in ("xx", "yy", "adc", "moduleInd", "clus", "pdigi", "rawIdArr")}
     got.update(got_digis)
-    assert_same(want, got, FIELDS, os.path.basename(path))
+    # module starts are defined up to the number of module runs
+    m = want["nModules"] + 1
+    assert np.array_equal(want["moduleStart"][:m], got["moduleStart"][:m]), "moduleStart"
+    assert_same(want, got, [f for f in FIELDS if f != "moduleStart"], os.path.basename(path))
     assert_same_errors(want["errors"], got["errors"], os.path.basename(path))

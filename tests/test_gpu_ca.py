@@ -147,3 +147,31 @@ def test_high_occupancy_profile(detector, conditions, gpu_ctx):
     o = orc.process(raw, upto=5)
     print(f"pu200 hits={o['nHits']} cells={o['nCells']} tuples={o['nTuples']} status={o['caStatus']:#x}", flush=True)
     compare_ca(o, _gpu_event(gpu_ctx, s, raw), "pu200/high-occupancy")
+
+
+def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
+    """Every launch shape the autotuner may pick gives the same products."""
+    from pxtrack import gpu
+    evs = [event(detector, "pu20_80", 10 + i)[0] for i in range(3)]
+    unpacked = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in evs]
+    s = gpu_ctx.stream(max_events=3, max_words=sum(w.size for w, _ in unpacked) + 1024)
+    s.load(unpacked)
+    ref = None
+    for lanes, grid in ((1, 64), (8, 2048), (64, 8192), (4, 333)):
+        s.set_tuning(lanes, grid)
+        s.run(0, 3, upto=5)
+        s.fetch_counts(0, 3)
+        got = [s.event_products(e, upto=5) for e in range(3)]
+        for e in range(3):
+            got[e].update(s.ca_internals(e, int(got[e]["nHits"])))
+        if ref is None:
+            ref = got
+            for e in range(3):
+                compare_ca(oracle.process(evs[e], upto=5), got[e], f"shape {lanes}/{grid} event {e}")
+        else:
+            for e in range(3):
+                for k in ("tupleOff", "tupleBins", "quality", "state", "cellInner", "cellOuter", "cellNeighbors",
+                          "cellTracks", "idv", "zv"):
+                    assert np.array_equal(ref[e][k], got[e][k]), f"shape {lanes}/{grid} event {e}: {k}"
+    lanes, grid, ms = s.autotune(0, 3, upto=5)
+    assert lanes in (2, 4, 8, 16, 32, 64) and grid >= 1024 and ms > 0
