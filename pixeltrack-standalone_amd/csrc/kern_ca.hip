@@ -169,20 +169,30 @@ namespace pxg {
         loadPairTable(H.layerStart, T);
       __syncthreads();
       uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
+      uint64_t* dMask = ca.dMask + size_t(e) * 2 * kDoubletSlots;
       const uint32_t ntot = T.cum[kNumPairs - 1];
       const uint32_t sub = laneId() % G;
       for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
         const SlotScan q = slotOf(H, T, j);
         uint32_t n = 0;
+        uint64_t m0 = 0, m1 = 0;  // accepted candidates among the first 128 of the window
         if (q.live) {
           for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
             const uint32_t k = k0 + sub;
             const bool ok = k < q.n && acceptDoublet(H, q, candidate(H, q, k));
-            n += __popcll(groupBits<G>(__ballot(ok)));
+            const uint64_t bits = groupBits<G>(__ballot(ok));
+            n += __popcll(bits);
+            if (k0 < 64)
+              m0 |= bits << k0;
+            else if (k0 < 128)
+              m1 |= bits << (k0 - 64);
           }
         }
-        if (sub == 0)
+        if (sub == 0) {
           dOff[j] = n;
+          dMask[2 * j] = m0;
+          dMask[2 * j + 1] = m1;
+        }
       }
     }
 
@@ -233,6 +243,7 @@ namespace pxg {
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       uint32_t* ohc = ca.ohCount + size_t(e) * kHitStride;
+      const uint64_t* dMask = ca.dMask + size_t(e) * 2 * kDoubletSlots;
       const uint32_t sub = laneId() % G;
       const uint64_t below = (1ull << sub) - 1;
       for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
@@ -240,14 +251,17 @@ namespace pxg {
         if (base >= nCells || dOff[j + 1] == base)
           continue;
         const SlotScan q = slotOf(H, T, j);
+        const uint64_t m0 = dMask[2 * j], m1 = dMask[2 * j + 1];
         uint32_t run = 0;
         for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
           const uint32_t k = k0 + sub;
           uint32_t oi = 0;
           bool ok = false;
           if (k < q.n) {
-            oi = candidate(H, q, k);
-            ok = acceptDoublet(H, q, oi);
+            // the count pass recorded the first 128 candidates' verdicts
+            ok = k < 64 ? (m0 >> k) & 1 : k < 128 ? (m1 >> (k - 64)) & 1 : acceptDoublet(H, q, candidate(H, q, k));
+            if (ok)
+              oi = candidate(H, q, k);
           }
           const uint64_t m = groupBits<G>(__ballot(ok));
           const uint32_t c = base + run + __popcll(m & below);
@@ -333,7 +347,10 @@ namespace pxg {
       uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
       uint32_t status = 0, stored = 0;
-      for (uint32_t oc = gridStart(); oc < nCells; oc += gridStride()) {
+      // cells in outer-hit list order: neighbouring lanes mostly share their
+      // outer hit, hence the candidate cells and their hits (shared loads)
+      for (uint32_t pos = gridStart(); pos < nCells; pos += gridStride()) {
+        const uint32_t oc = ca.ohCells[cb + pos];
         const uint32_t hi = ca.cellInner[cb + oc], hm = ca.cellOuter[cb + oc];
         const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
         if (ln > kCaMaxCellsPerHit)
@@ -398,7 +415,7 @@ namespace pxg {
     // ascending) in rows: row ci kills the later aligned cells until the first
     // aligned later cell that is closer, which kills ci and ends the row.  The
     // rows do not read the kills, so each row runs on its own (one thread per
-    // cell), finding its first killer as the smallest such cell index.
+    // cell, in list order), finding its first killer as the smallest such cell.
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.y;
       const HitView H = hitsOf(p, e);
@@ -410,7 +427,8 @@ namespace pxg {
       const uint32_t* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
-      for (uint32_t ci = gridStart(); ci < nCells; ci += gridStride()) {
+      for (uint32_t pos = gridStart(); pos < nCells; pos += gridStride()) {
+        const uint32_t ci = cells[pos];
         if (!isUsed(ci))
           continue;
         const uint32_t h = ca.cellOuter[cb + ci];
@@ -436,31 +454,43 @@ namespace pxg {
         const uint16_t di = H.detInd[hi];
         const float xi = H.xg[hi] - xo, yi = H.yg[hi] - yo, zi = H.zg[hi] - zo;
         const float ni = xi * xi + yi * yi + zi * zi;
-        auto aligned = [&](uint32_t cj, float& nj) {
+        // one pass: the aligned later cells, and the first closer one among them
+        uint32_t killer = limit;
+        uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
+        for (uint32_t k = 0; k < n; ++k) {
+          const uint32_t cj = cells[b + k];
+          if (cj <= ci || cj >= limit || !isUsed(cj))
+            continue;
           const uint32_t hj = ca.cellInner[cb + cj];
           const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
-          nj = xj * xj + yj * yj + zj * zj;
+          const float nj = xj * xj + yj * yj + zj * zj;
           const float cos12 = xi * xj + yi * yj + zi * zj;
-          return di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj;
-        };
-        // first killer: the smallest later used cell, aligned and closer
-        uint32_t killer = limit;
-        for (uint32_t k = 0; k < n; ++k) {
-          const uint32_t cj = cells[b + k];
-          if (cj <= ci || cj >= killer || !isUsed(cj))
+          if (!(di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj))
             continue;
-          float nj;
-          if (aligned(cj, nj) && ni > nj)
+          if (ni > nj && cj < killer)
             killer = cj;
+          if (k < 64)
+            alignedMask |= 1ull << k;
         }
-        // the aligned later cells before it are killed
-        for (uint32_t k = 0; k < n; ++k) {
-          const uint32_t cj = cells[b + k];
-          if (cj <= ci || cj >= killer || !isUsed(cj))
-            continue;
-          float nj;
-          if (aligned(cj, nj))
-            ca.cellFlags[cb + cj] |= 2;  // several rows may kill a cell: same byte stored
+        // the aligned cells before the killer are killed (several rows may kill
+        // a cell: they store the same byte)
+        for (uint64_t m = alignedMask; m; m &= m - 1) {
+          const uint32_t cj = cells[b + __ffsll((unsigned long long)m) - 1];
+          if (cj < killer)
+            ca.cellFlags[cb + cj] |= 2;
+        }
+        if (n > 64) {  // positions past 64: second pass
+          for (uint32_t k = 64; k < n; ++k) {
+            const uint32_t cj = cells[b + k];
+            if (cj <= ci || cj >= killer || !isUsed(cj))
+              continue;
+            const uint32_t hj = ca.cellInner[cb + cj];
+            const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
+            const float nj = xj * xj + yj * yj + zj * zj;
+            const float cos12 = xi * xj + yi * yj + zi * zj;
+            if (di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj)
+              ca.cellFlags[cb + cj] |= 2;
+          }
         }
         if (killer != limit)
           ca.cellFlags[cb + ci] |= 2;

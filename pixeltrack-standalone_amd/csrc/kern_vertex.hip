@@ -16,6 +16,7 @@ namespace pxg {
   namespace {
     constexpr int kVB = kEventBlock;
     constexpr int kUnassigned = 9999;
+    constexpr uint32_t kLdsTracks = 2048;
 
     struct VtxLds {
       uint32_t counts[kMaxVertices];
@@ -27,10 +28,9 @@ namespace pxg {
 
     // Stable counting sort of items 0..n-1 by key(i) in [0, K) (items with a
     // key >= K are left out): list gets the items bin by bin, ascending item
-    // index within a bin (descending if `descending`); off[0..K] the bin starts.
+    // index within a bin (descending if `descending`); L.start[0..K] the bin starts.
     template <typename Key>
-    __device__ void countingSort(uint32_t n, uint32_t K, Key key, bool descending, uint32_t* list, uint32_t* off,
-                                 VtxLds& L) {
+    __device__ void countingSort(uint32_t n, uint32_t K, Key key, bool descending, uint32_t* list, VtxLds& L) {
       for (uint32_t k = threadIdx.x; k < K; k += kVB) {
         L.counts[k] = 0;
         L.running[k] = 0;
@@ -55,8 +55,6 @@ namespace pxg {
       if (threadIdx.x == 0)
         L.start[K] = carry;
       __syncthreads();
-      for (uint32_t k = threadIdx.x; k <= K; k += kVB)
-        off[k] = L.start[k];
       // rank of an item among the earlier items of its bin: within its wave by
       // shuffles, across waves by visiting the waves in order
       const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -91,7 +89,8 @@ namespace pxg {
       const uint32_t* toff = ca.tupleOff + ca.offBase(e);
       const uint8_t* q = ca.quality + tb;
       int16_t* idv = ca.idv + tb;
-      int32_t* nn = ca.ndof + tb;
+      int32_t* const nnOut = ca.ndof + tb;
+      int32_t* nn = nnOut;
       uint16_t* itrk = ca.vItrk + tb;
       float* zt = ca.vZt + tb;
       float* ezt2 = ca.vEzt2 + tb;
@@ -100,7 +99,7 @@ namespace pxg {
       int32_t* tmp = ca.vTmp + tb;
       uint8_t* izt = ca.vIzt + tb;
       uint32_t* zbins = ca.vTrk + tb;  // tracks by z bin, then tracks by vertex
-      uint32_t* voff = ca.vOff + size_t(e) * (kMaxVertices + 64);
+      const uint32_t* voff = L.start;  // bin starts of the last counting sort
       float* zv = ca.zv + size_t(e) * kMaxVertices;
       float* wv = ca.wv + size_t(e) * kMaxVertices;
       float* chi2 = ca.vchi2 + size_t(e) * kMaxVertices;
@@ -137,6 +136,26 @@ namespace pxg {
         __syncthreads();
       }
       const uint32_t nt = L.nt;
+      // events with up to kLdsTracks tracks keep the per-track arrays in LDS
+      __shared__ float sZt[kLdsTracks], sEzt2[kLdsTracks];
+      __shared__ int32_t sIv[kLdsTracks], sTmp[kLdsTracks], sNn[kLdsTracks];
+      __shared__ uint32_t sBins[kLdsTracks];
+      __shared__ uint8_t sIzt[kLdsTracks];
+      const bool inLds = nt <= kLdsTracks;
+      if (inLds) {
+        for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+          sZt[i] = zt[i];
+          sEzt2[i] = ezt2[i];
+        }
+        zt = sZt;
+        ezt2 = sEzt2;
+        iv = sIv;
+        tmp = sTmp;
+        nn = sNn;
+        zbins = sBins;
+        izt = sIzt;
+        __syncthreads();
+      }
 
       // ---- clusterTracksByDensity
       const float er2mx = errmax * errmax;
@@ -148,7 +167,7 @@ namespace pxg {
         nn[i] = 0;
       }
       __syncthreads();
-      countingSort(nt, 256, [&](uint32_t i) { return uint32_t(izt[i]); }, true, zbins, voff, L);
+      countingSort(nt, 256, [&](uint32_t i) { return uint32_t(izt[i]); }, true, zbins, L);
       __syncthreads();
       auto forEachNear = [&](uint32_t value, auto&& f) {
         const int bs = max(0, int(value) - 1);
@@ -239,7 +258,7 @@ namespace pxg {
         __syncthreads();
         if (threadIdx.x == 0)
           L.nvFinal = nvf;
-        countingSort(nt, nc, [&](uint32_t i) { return uint32_t(iv[i]); }, false, zbins, voff, L);
+        countingSort(nt, nc, [&](uint32_t i) { return uint32_t(iv[i]); }, false, zbins, L);
         __syncthreads();
         for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
           float z = 0, w = 0;
@@ -360,6 +379,9 @@ namespace pxg {
 
       // ---- sortByPt2
       const uint32_t nvFinal = L.nvFinal;
+      if (inLds)
+        for (uint32_t i = threadIdx.x; i < nt; i += kVB)
+          nnOut[i] = sNn[i];
       if (threadIdx.x == 0)
         ca.cnt[e].nVertices = nvFinal;
       if (nvFinal >= 1) {

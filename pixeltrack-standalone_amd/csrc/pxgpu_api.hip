@@ -106,6 +106,7 @@ namespace pxg::detail {
     const size_t L = E * ca.caps.maxCellLists;
     f(ca.cnt, E);
     f(ca.dOff, E * (kDoubletSlots + 64));
+    f(ca.dMask, E * 2 * kDoubletSlots);
     f(ca.cellInner, D);
     f(ca.cellOuter, D);
     f(ca.cellPair, D);
@@ -206,6 +207,7 @@ namespace pxg::detail {
     const size_t H = size_t(first) * kHitStride, L = size_t(first) * c.caps.maxCellLists;
     v.cnt += E;
     v.dOff += size_t(first) * (kDoubletSlots + 64);
+    v.dMask += size_t(first) * 2 * kDoubletSlots;
     v.cellInner += D;
     v.cellOuter += D;
     v.cellPair += D;

@@ -132,6 +132,7 @@ namespace pxg {
     CACapacities caps;
     CACounts* cnt;
     uint32_t* dOff;          // [kDoubletSlots + 64]: doublets per (pair, inner hit), then offsets
+    uint64_t* dMask;         // [2 * kDoubletSlots]: accepted candidates of a slot's first 128
     uint16_t *cellInner, *cellOuter;  // [maxDoublets]
     uint8_t* cellPair;
     uint8_t* cellFlags;      // bit0 listed in its outer hit's list, bit1 killed, bit2 duplicate pass done
