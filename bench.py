@@ -86,6 +86,83 @@ def gen_events(detector, cfg, n, threads):
         return list(ex.map(lambda i: detector.event(cfg, i)[0], range(n)))
 
 
+def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
+    """Submits `inputs` with Poisson arrivals at `rate` events/s (all at once if rate is None)
+    to a serving loop and returns throughput and latency percentiles (ms)."""
+    from pxtrack import gpu
+    sv = gpu.Server(ctx, **server_kw)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(inputs)) if rate else np.zeros(len(inputs))
+    ids = {}
+    t0 = time.perf_counter()
+    t = t0
+    for i, (w, f) in enumerate(inputs):
+        t += gaps[i]
+        while True:  # spin until the arrival time (sleep when far)
+            d = t - time.perf_counter()
+            if d <= 0:
+                break
+            if d > 2e-4:
+                time.sleep(d - 1e-4)
+        ids[sv.submit(w, f)] = i
+    res = []
+    deadline = time.time() + 120
+    while len(res) < len(ids) and time.time() < deadline:
+        res += sv.poll(wait_ms=50)
+    info = sv.info()
+    sv.close()
+    if len(res) < len(ids):
+        return {"rate": rate, "error": f"{len(res)} of {len(ids)} completed"}
+    lat = np.array([(r["t_done_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])
+    span = (max(r["t_done_ns"] for r in res) - min(r["t_submit_ns"] for r in res)) * 1e-9
+    out = {"rate": rate, "events": len(res), "events_per_s": round(len(res) / span, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
+           "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 1),
+           "batches": info["batches"], "status_or": int(np.bitwise_or.reduce([r["status"] for r in res]))}
+    if classes is not None:
+        for name in sorted(set(classes)):
+            lc = np.array([(r["t_done_ns"] - r["t_submit_ns"]) * 1e-6 for r in res if classes[ids[r["id"]]] == name])
+            out[f"{name}_p50_ms"] = round(float(np.percentile(lc, 50)), 3)
+            out[f"{name}_p99_ms"] = round(float(np.percentile(lc, 99)), 3)
+    return out
+
+
+def serving_sweep(ctx, det, cond, args):
+    """Continuous-batching serving loop vs the plain per-event path at the same arrival rates."""
+    from pxtrack import gen, gpu
+    threads = min(16, os.cpu_count() or 1)
+    raws = gen_events(det, gen.preset("pu_poisson60"), args.serve_events, threads)
+    ev = [gpu.unpack_raw(r, cond["fed_ids"]) for r in raws]
+    budget = int(args.serve_budget_gb * 1e9)
+    batched_kw = dict(max_batch=args.batch, n_streams=3, hbm_budget=budget, upto=args.upto)
+    plain_kw = dict(max_batch=1, n_streams=1, upto=args.upto)
+    rates = [1000, 3000, 10000, 30000, None]
+    out = {"workload": f"{len(ev)} events, pileup Poisson(60) clipped to [10, 200], Poisson arrivals",
+           "hbm_budget_gb": args.serve_budget_gb, "batched": [], "plain": []}
+    for r in rates:
+        out["batched"].append(serve_at(ctx, ev, r, batched_kw))
+        print(f"[bench] serving batched rate {r}: {out['batched'][-1]}", file=sys.stderr, flush=True)
+    for r in rates[:2] + [None]:
+        sub = ev[: min(len(ev), 300)]
+        out["plain"].append(serve_at(ctx, sub, r, plain_kw))
+        print(f"[bench] serving plain rate {r}: {out['plain'][-1]}", file=sys.stderr, flush=True)
+    # 80/20 mix of PU50 and PU200 with the high-occupancy capacities, latency per class
+    n50 = int(0.8 * args.mix_events)
+    r50 = gen_events(det, gen.preset("pu50"), n50, threads)
+    r200 = gen_events(det, gen.preset("pu200"), args.mix_events - n50, threads)
+    mix = [(gpu.unpack_raw(r, cond["fed_ids"]), "pu50") for r in r50] + \
+          [(gpu.unpack_raw(r, cond["fed_ids"]), "pu200") for r in r200]
+    order = np.random.default_rng(7).permutation(len(mix))
+    mix = [mix[i] for i in order]
+    mix_kw = dict(max_batch=args.batch // 2, n_streams=3, hbm_budget=budget, upto=args.upto,
+                  profile=gpu.PROFILE_HIGH_OCCUPANCY)
+    out["mix_pu50_pu200"] = [serve_at(ctx, [m[0] for m in mix], r, mix_kw, classes=[m[1] for m in mix])
+                             for r in (2000, None)]
+    print(f"[bench] serving mix: {out['mix_pu50_pu200']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +174,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--streams", type=int, default=3, help="HIP streams the batches are spread over")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
+    ap.add_argument("--serve-events", type=int, default=600)
+    ap.add_argument("--mix-events", type=int, default=200)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,20 +201,36 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = gpu.Context(cond)
-    stream = ctx.stream(max_events=len(inputs), max_words=int(W.sum()))
-    stream.load(inputs)
     n = len(inputs)
     B = max(1, min(args.batch, n))
+    # the events are split over S HIP streams, each with its own resident pool;
+    # a pass enqueues every stream's batches, which then overlap on the device
+    S = max(1, min(args.streams, (n + B - 1) // B))
+    bounds = [round(k * n / S) for k in range(S + 1)]
+    streams = []
+    for k in range(S):
+        part = inputs[bounds[k]:bounds[k + 1]]
+        st = ctx.stream(max_events=len(part), max_words=int(sum(len(w) for w, _ in part)))
+        st.load(part)
+        streams.append(st)
+    stream = streams[0]
 
     def batched_pass():
-        for first in range(0, n, B):
-            stream.run(first, min(B, n - first), args.upto)
-        return stream.fetch_counts(0, n)
+        for k, st in enumerate(streams):
+            m = bounds[k + 1] - bounds[k]
+            for first in range(0, m, B):
+                st.run(first, min(B, m - first), args.upto)
+        status = 0
+        for k, st in enumerate(streams):
+            status |= st.fetch_counts(0, bounds[k + 1] - bounds[k])
+        return status
 
     # ---- launch shape chosen by the autotuner on the first batch
     tuned = None
     if not args.no_tune:
-        lanes, grid, tms = stream.autotune(0, B, args.upto)
+        lanes, grid, tms = stream.autotune(0, min(B, bounds[1]), args.upto)
+        for st in streams[1:]:
+            st.set_tuning(lanes, grid)
         tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": round(tms, 4)}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
 
@@ -140,26 +238,33 @@ def main():
     for _ in range(args.warmup):
         batched_pass()
     torch.cuda.synchronize()
-    stream.profile(True)
-    stream.profile_read()
+    for st in streams:
+        st.profile(True)
+        st.profile_read()
+        st.sync()
     torch.cuda.synchronize()
-    stream.sync()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         status = batched_pass()
-    stream.sync()
+    for st in streams:
+        st.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    prof = stream.profile_read()
-    stream.profile(False)
+    prof = {}
+    for st in streams:
+        for k, (c, ms) in st.profile_read().items():
+            a = prof.get(k, (0, 0.0))
+            prof[k] = (a[0] + c, a[1] + ms)
+        st.profile(False)
     ms_per_step = 1e3 * elapsed / args.steps
     evps = n * args.steps / elapsed
 
     # per-event counters for the algorithmic bytes
-    cnts = np.array([stream.event_counts(e) for e in range(n)], dtype=np.float64)
+    where = [(st, e) for k, st in enumerate(streams) for e in range(bounds[k + 1] - bounds[k])]
+    cnts = np.array([st.event_counts(e) for st, e in where], dtype=np.float64)
     mean = {"W": float(W.mean()), "D": float(W.mean()), "M": float(cnts[:, 0].mean()), "H": float(cnts[:, 1].mean())}
     if args.upto >= 4:
-        cac = [stream.ca_counts(e) for e in range(n)]
+        cac = [st.ca_counts(e) for st, e in where]
         for key, name in (("S", "nSlots"), ("C", "nCells"), ("N", "nNeighbors"), ("T", "nTuples"), ("TH", "apcN"),
                           ("F", "nMult"), ("V", "vtxTracks")):
             mean[key] = float(np.mean([c[name] for c in cac]))
@@ -181,20 +286,25 @@ def main():
     plain = None
     if not args.no_plain:
         lat = []
-        for e in range(min(n, 50)):  # warmup
-            stream.run(e, 1, args.upto)
-            stream.fetch_counts(e, 1)
+        for st, e in where[:50]:  # warmup
+            st.run(e, 1, args.upto)
+            st.fetch_counts(e, 1)
         t1 = time.perf_counter()
-        for e in range(n):
+        for st, e in where:
             a = time.perf_counter()
-            stream.run(e, 1, args.upto)
-            stream.fetch_counts(e, 1)
+            st.run(e, 1, args.upto)
+            st.fetch_counts(e, 1)
             lat.append(time.perf_counter() - a)
         tp = time.perf_counter() - t1
         lat = np.array(lat) * 1e3
         plain = {"events_per_s": round(n / tp, 1), "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
                  "latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
                  "latency_ms_p999": round(float(np.percentile(lat, 99.9)), 4)}
+
+    # ---- serving loop: continuous batching under Poisson arrivals
+    serving = None
+    if not args.no_serve:
+        serving = serving_sweep(ctx, det, cond, args)
 
     # ---- CPU baseline: the oracle (serial restatement) on the host cores, bounded sample
     cpu = None
@@ -229,14 +339,16 @@ def main():
         "vs_baseline": None,
         "dtype": "int32/f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"{n} PU50 events ({STAGES[args.upto]}), batched path, {B} events per launch",
-                   "events": n, "batch": B, "mean_words": round(float(W.mean()), 1),
+        "config": {"workload": f"{n} PU50 events ({STAGES[args.upto]}), batched path, {B} events per launch, "
+                               f"{S} streams",
+                   "events": n, "batch": B, "streams": S, "mean_words": round(float(W.mean()), 1),
                    "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
                    "stages": STAGES[args.upto]},
         "batched": {"events_per_s": round(evps, 1), "batch_latency_ms": round(batch_lat_ms, 4),
                     "status": status, "tuning": tuned},
         "plain": plain,
         "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
+        "serving": serving,
         "kernel_ms_per_launch": kernel_ms,
         "kernel_time_share": share,
         "launches": launches_per_kernel,

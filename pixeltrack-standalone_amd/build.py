@@ -51,11 +51,20 @@ def build_gpu():
     _run([HIPCC] + HIP_FLAGS + srcs + ["-o", os.path.join(LIB, "libpxgpu.so")])
 
 
+def build_framework():
+    """The job driver with the reference's module interface, linked against the C ABI."""
+    srcs = sorted(glob.glob(os.path.join(PKG, "framework", "*.cc")))
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-pthread", "-I" + os.path.join(REPO, "include")] + srcs +
+         ["-o", os.path.join(LIB, "pixeltrack-amd"), "-L" + LIB, "-lpxgpu", "-lpxhost",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"])
+
+
 def build_all():
     os.makedirs(LIB, exist_ok=True)
     build_gen()
     build_host()
     build_gpu()
+    build_framework()
 
 
 if __name__ == "__main__":

@@ -44,6 +44,33 @@ namespace pxg {
 
     __device__ __forceinline__ uint32_t laneId() { return threadIdx.x & 63; }
     __device__ __forceinline__ uint64_t lanesBelow() { return (1ull << laneId()) - 1; }
+    // Counter allocation with one atomic per wave: the active lanes asking for
+    // a slot get consecutive values (lane order).  Lanes that diverged away do
+    // not take part, so it may be called in divergent code.
+    __device__ __forceinline__ uint32_t waveAlloc(uint32_t* counter, bool need) {
+      const uint64_t m = __ballot(need);
+      if (!m)
+        return 0;
+      const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+      uint32_t base = 0;
+      if (laneId() == leader)
+        base = atomicAdd(counter, uint32_t(__popcll(m)));
+      base = __shfl(base, leader);
+      return base + __popcll(m & lanesBelow());
+    }
+
+    // sum over the active lanes, added to *counter once per wave
+    __device__ __forceinline__ void waveAdd(uint32_t* counter, uint32_t v) {
+      const uint64_t m = __ballot(v != 0);
+      if (!m)
+        return;
+      uint32_t sum = v;
+      for (int o = 32; o > 0; o >>= 1)
+        sum += __shfl_xor(sum, o);
+      if (laneId() == __ffsll((unsigned long long)m) - 1)
+        atomicAdd(counter, sum);
+    }
+
     // wave index within the event's workgroups, uniform over the wave
     __device__ __forceinline__ uint32_t waveStart() {
       return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -382,7 +409,7 @@ namespace pxg {
               atomicOr(&used[oc >> 5], 1u << (oc & 31));
             if (n < kCaMaxNeighbors) {
               if (slot == -1) {
-                const uint32_t s = atomicAdd(&ca.cnt[e].nbLists, 1u);
+                const uint32_t s = waveAlloc(&ca.cnt[e].nbLists, true);
                 if (s + 1 < ca.caps.maxCellLists) {
                   slot = int32_t(s);
                 } else {
@@ -406,8 +433,7 @@ namespace pxg {
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
-      if (stored)
-        atomicAdd(&ca.cnt[e].nNeighbors, stored);
+      waveAdd(&ca.cnt[e].nNeighbors, stored);
     }
 
     // ---- a14 fishbone: gpuFishbone.h:21-90 --------------------------------------------
@@ -698,9 +724,9 @@ namespace pxg {
         const uint32_t n = ca.trCount[cb + c];
         ca.dfsT[cb + c] = n;  // tracks of the cell (all of them)
         ca.trCount[cb + c] = 0;
+        const uint32_t s = waveAlloc(&ca.cnt[e].trLists, n > 0);
         if (n == 0)
           continue;
-        const uint32_t s = atomicAdd(&ca.cnt[e].trLists, 1u);
         if (s + 1 < ca.caps.maxCellLists) {
           ca.trSlot[cb + c] = int32_t(s);
         } else {
@@ -786,7 +812,7 @@ namespace pxg {
         for (uint32_t k = 0; k < n; ++k)
           if (off[list[k] + 1] - off[list[k]] != maxNh)
             ca.quality[tb + list[k]] = kQualityDup;
-        const uint32_t at = atomicAdd(&ca.cnt[e].nMulti, 1u);
+        const uint32_t at = waveAlloc(&ca.cnt[e].nMulti, true);
         ca.multiCells[cb + at] = c;
       }
       if (status)

@@ -238,7 +238,7 @@ void pxg_server::finish(Slot& sl) {
 }
 
 void pxg_server::loop() {
-  hipSetDevice(ctx->device);
+  (void)hipSetDevice(ctx->device);
   while (true) {
     bool progressed = false;
     for (Slot& sl : slots) {
