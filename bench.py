@@ -119,6 +119,8 @@ def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
            "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 1),
+           "mean_queue_ms": round(float(np.mean([(r["t_launch_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])), 3),
+           "mean_batch_ms": round(float(np.mean([(r["t_done_ns"] - r["t_launch_ns"]) * 1e-6 for r in res])), 3),
            "batches": info["batches"], "status_or": int(np.bitwise_or.reduce([r["status"] for r in res]))}
     if classes is not None:
         for name in sorted(set(classes)):
@@ -128,15 +130,17 @@ def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
     return out
 
 
-def serving_sweep(ctx, det, cond, args):
+def serving_sweep(ctx, det, cond, args, tuned):
     """Continuous-batching serving loop vs the plain per-event path at the same arrival rates."""
     from pxtrack import gen, gpu
     threads = min(16, os.cpu_count() or 1)
     raws = gen_events(det, gen.preset("pu_poisson60"), args.serve_events, threads)
     ev = [gpu.unpack_raw(r, cond["fed_ids"]) for r in raws]
     budget = int(args.serve_budget_gb * 1e9)
-    batched_kw = dict(max_batch=args.batch, n_streams=3, hbm_budget=budget, upto=args.upto)
-    plain_kw = dict(max_batch=1, n_streams=1, upto=args.upto)
+    shape = dict(doublet_lanes=tuned["doublet_lanes"], grid_target=tuned["grid_target"]) if tuned else {}
+    # products stay on the device (no --transfer, the reference's default); counters come back
+    batched_kw = dict(max_batch=args.batch, n_streams=3, hbm_budget=budget, upto=args.upto, transfer=False, **shape)
+    plain_kw = dict(max_batch=1, n_streams=1, upto=args.upto, transfer=False, **shape)
     rates = [1000, 3000, 10000, 30000, None]
     out = {"workload": f"{len(ev)} events, pileup Poisson(60) clipped to [10, 200], Poisson arrivals",
            "hbm_budget_gb": args.serve_budget_gb, "batched": [], "plain": []}
@@ -156,7 +160,7 @@ def serving_sweep(ctx, det, cond, args):
     order = np.random.default_rng(7).permutation(len(mix))
     mix = [mix[i] for i in order]
     mix_kw = dict(max_batch=args.batch // 2, n_streams=3, hbm_budget=budget, upto=args.upto,
-                  profile=gpu.PROFILE_HIGH_OCCUPANCY)
+                  profile=gpu.PROFILE_HIGH_OCCUPANCY, transfer=False, **shape)
     out["mix_pu50_pu200"] = [serve_at(ctx, [m[0] for m in mix], r, mix_kw, classes=[m[1] for m in mix])
                              for r in (2000, None)]
     print(f"[bench] serving mix: {out['mix_pu50_pu200']}", file=sys.stderr, flush=True)
@@ -304,7 +308,7 @@ def main():
     # ---- serving loop: continuous batching under Poisson arrivals
     serving = None
     if not args.no_serve:
-        serving = serving_sweep(ctx, det, cond, args)
+        serving = serving_sweep(ctx, det, cond, args, tuned)
 
     # ---- CPU baseline: the oracle (serial restatement) on the host cores, bounded sample
     cpu = None

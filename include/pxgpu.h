@@ -243,6 +243,8 @@ typedef struct {
   int upto;                /* last stage, 1..5 */
   uint32_t doublet_lanes;  /* launch shape (0: default) */
   uint32_t grid_target;
+  int transfer;            /* 1: copy each event's products to the host (pxg_server_products);
+                              0: only counters (the products stay on the device and are reused) */
 } pxg_server_config;
 
 typedef struct {

@@ -111,11 +111,17 @@ namespace {
     }
   }
 
+  // Pinned host chunks holding one event's words and FED ids, so that the
+  // device copies read them directly (sized for the largest event).
+  constexpr size_t kChunkWords = kMaxFedWords;
+  constexpr size_t kChunkBytes = kChunkWords * 4 + kChunkWords / 2 + 256;
+
   struct Request {
     uint64_t id;
-    std::vector<uint32_t> words;
-    std::vector<uint8_t> fedIds;
+    uint8_t* chunk = nullptr;  // words at chunk, FED ids at chunk + 4 * kChunkWords
+    uint32_t nWords = 0;
     pxg_beamspot bs;
+    uint32_t counts[4] = {0, 0, 0, 0};  // words, hits, tuples, vertices
     uint64_t tSubmit = 0, tLaunch = 0, tDone = 0;
     int status = 0;
     uint32_t batchSize = 0;
@@ -128,8 +134,6 @@ namespace {
     pxg_stream* s = nullptr;
     SlotState state = SlotState::Idle;
     std::vector<std::unique_ptr<Request>> batch;
-    uint32_t* hWords = nullptr;  // pinned staging
-    uint8_t* hFed = nullptr;
     PackLayout* hLayout = nullptr;
     PackLayout* dLayout = nullptr;
     uint8_t* dPack = nullptr;
@@ -143,6 +147,7 @@ namespace {
 struct pxg_server {
   pxg_server_config cfg{};
   pxg_context* ctx = nullptr;
+  std::vector<uint8_t*> freeChunks, allChunks;  // guarded by mu
   uint32_t maxBatchWords = 0;
   std::vector<Slot> slots;
   std::mutex mu;
@@ -166,15 +171,10 @@ struct pxg_server {
 int pxg_server::launch(Slot& sl) {
   const uint32_t n = uint32_t(sl.batch.size());
   std::vector<pxg_raw_view> views(n);
-  uint32_t off = 0;
   const uint64_t t = nowNs();
   for (uint32_t e = 0; e < n; ++e) {
     Request& r = *sl.batch[e];
-    const uint32_t w = uint32_t(r.words.size());
-    std::memcpy(sl.hWords + off, r.words.data(), size_t(w) * 4);
-    std::memcpy(sl.hFed + off / 2, r.fedIds.data(), w / 2);
-    views[e] = {sl.hWords + off, sl.hFed + off / 2, w};
-    off += (w + 255) / 256 * 256;
+    views[e] = {reinterpret_cast<const uint32_t*>(r.chunk), r.chunk + 4 * kChunkWords, r.nWords};
     r.tLaunch = t;
     r.batchSize = n;
   }
@@ -220,11 +220,21 @@ int pxg_server::pack(Slot& sl) {
 void pxg_server::finish(Slot& sl) {
   const uint64_t t = nowNs();
   pxg_stream* s = sl.s;
+  // products copied out of the pinned buffer before taking the lock
+  if (cfg.transfer)
+    for (uint32_t e = 0; e < sl.batch.size(); ++e) {
+      const PackLayout& L = sl.hLayout[e];
+      sl.batch[e]->packed.assign(sl.hPack + L.base, sl.hPack + L.base + L.size);
+    }
   std::lock_guard<std::mutex> g(mu);
   for (uint32_t e = 0; e < sl.batch.size(); ++e) {
     auto& r = sl.batch[e];
-    const PackLayout& L = sl.hLayout[e];
-    r->packed.assign(sl.hPack + L.base, sl.hPack + L.base + L.size);
+    r->counts[0] = r->nWords;
+    r->counts[1] = s->hostCounts[e].nClusters;
+    r->counts[2] = cfg.upto >= 4 ? s->hostCA[e].nTuples : 0;
+    r->counts[3] = cfg.upto >= 5 ? s->hostCA[e].nVertices : 0;
+    freeChunks.push_back(r->chunk);
+    r->chunk = nullptr;
     r->status = int(s->hostCounts[e].status | (s->hostCA[e].status << PXG_STATUS_CA_SHIFT) |
                     (s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u));
     r->tDone = t;
@@ -244,7 +254,10 @@ void pxg_server::loop() {
     for (Slot& sl : slots) {
       int rc = 0;
       if (sl.state == SlotState::Running && hipEventQuery(sl.evRun) == hipSuccess) {
-        rc = pack(sl);
+        if (cfg.transfer)
+          rc = pack(sl);
+        else
+          finish(sl);
         progressed = true;
       } else if (sl.state == SlotState::Packing && hipEventQuery(sl.evPack) == hipSuccess) {
         finish(sl);
@@ -255,7 +268,7 @@ void pxg_server::loop() {
         std::unique_lock<std::mutex> g(mu);
         uint32_t words = 0;
         while (!waiting.empty() && sl.batch.size() < cfg.max_batch) {
-          const uint32_t w = (uint32_t(waiting.front()->words.size()) + 255) / 256 * 256;
+          const uint32_t w = (waiting.front()->nWords + 255) / 256 * 256;
           if (!sl.batch.empty() && words + w > maxBatchWords)
             break;
           words += w;
@@ -324,12 +337,7 @@ int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server
     sl.packCapacity = B * packPerEvent;
     hipError_t e = hipSuccess;
     if (!rc) {
-      const size_t Wp = size_t(sv->maxBatchWords) + 256 * B;
-      e = hipHostMalloc(reinterpret_cast<void**>(&sl.hWords), Wp * 4, hipHostMallocDefault);
-      if (!e)
-        e = hipHostMalloc(reinterpret_cast<void**>(&sl.hFed), Wp / 2 + 64, hipHostMallocDefault);
-      if (!e)
-        e = hipHostMalloc(reinterpret_cast<void**>(&sl.hLayout), B * sizeof(PackLayout), hipHostMallocDefault);
+      e = hipHostMalloc(reinterpret_cast<void**>(&sl.hLayout), B * sizeof(PackLayout), hipHostMallocDefault);
       if (!e)
         e = hipHostMalloc(reinterpret_cast<void**>(&sl.hPack), sl.packCapacity, hipHostMallocDefault);
       if (!e)
@@ -345,6 +353,17 @@ int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server
       pxg_server_destroy(sv);
       return rc ? rc : fail(PXG_ERR_HIP, "pxg_server_create: %s", hipGetErrorString(e));
     }
+  }
+  // pinned chunks for the events of all slots, a batch's worth waiting, and a margin
+  const size_t nChunks = size_t(cfg->n_streams + 1) * cfg->max_batch + 256;
+  for (size_t k = 0; k < nChunks; ++k) {
+    uint8_t* c = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&c), kChunkBytes, hipHostMallocDefault) != hipSuccess) {
+      pxg_server_destroy(sv);
+      return fail(PXG_ERR_HIP, "pxg_server_create: pinned host memory for %zu events unavailable", nChunks);
+    }
+    sv->allChunks.push_back(c);
+    sv->freeChunks.push_back(c);
   }
   sv->thread = std::thread([sv] { sv->loop(); });
   *out = sv;
@@ -372,16 +391,14 @@ void pxg_server_destroy(pxg_server* sv) {
       (void)hipFree(sl.dLayout);
     if (sl.dPack)
       (void)hipFree(sl.dPack);
-    if (sl.hWords)
-      (void)hipHostFree(sl.hWords);
-    if (sl.hFed)
-      (void)hipHostFree(sl.hFed);
     if (sl.hLayout)
       (void)hipHostFree(sl.hLayout);
     if (sl.hPack)
       (void)hipHostFree(sl.hPack);
     pxg_stream_destroy(sl.s);
   }
+  for (uint8_t* c : sv->allChunks)
+    (void)hipHostFree(c);
   delete sv;
 }
 
@@ -402,15 +419,25 @@ int pxg_server_submit(pxg_server* sv, const uint32_t* words, const uint8_t* fedI
                       const pxg_beamspot* bs, uint64_t* id) {
   if (!sv || !bs || !id || (nWords && (!words || !fedIds)) || nWords % 2)
     return fail(PXG_ERR_ARG, "pxg_server_submit: bad arguments");
-  if (nWords > kMaxFedWords || nWords > sv->maxBatchWords)
+  if (nWords > kChunkWords || nWords > sv->maxBatchWords)
     return fail(PXG_ERR_CAPACITY, "pxg_server_submit: %u words exceed the capacity", nWords);
   if (sv->error)
     return fail(sv->error, "pxg_server_submit: server stopped: %s", sv->errorText.c_str());
   auto r = std::make_unique<Request>();
-  r->words.assign(words, words + nWords);
-  r->fedIds.assign(fedIds, fedIds + nWords / 2);
-  r->bs = *bs;
   r->tSubmit = nowNs();
+  {
+    // a free pinned chunk; when all are taken the caller waits (back-pressure)
+    std::unique_lock<std::mutex> g(sv->mu);
+    sv->cv.wait(g, [&] { return !sv->freeChunks.empty() || sv->error; });
+    if (sv->error)
+      return fail(sv->error, "pxg_server_submit: server stopped: %s", sv->errorText.c_str());
+    r->chunk = sv->freeChunks.back();
+    sv->freeChunks.pop_back();
+  }
+  std::memcpy(r->chunk, words, size_t(nWords) * 4);
+  std::memcpy(r->chunk + 4 * kChunkWords, fedIds, nWords / 2);
+  r->nWords = nWords;
+  r->bs = *bs;
   std::lock_guard<std::mutex> g(sv->mu);
   r->id = sv->nextId++;
   *id = r->id;
@@ -431,8 +458,8 @@ int pxg_server_poll(pxg_server* sv, pxg_server_result* out, int max, int wait_ms
   while (n < max && !sv->done.empty()) {
     auto r = std::move(sv->done.front());
     sv->done.pop_front();
-    const uint32_t* h = reinterpret_cast<const uint32_t*>(r->packed.data());
-    out[n] = {r->id, r->status, r->tSubmit, r->tLaunch, r->tDone, r->batchSize, h[0], h[2], h[5], h[6]};
+    out[n] = {r->id,        r->status,   r->tSubmit,  r->tLaunch,  r->tDone,
+              r->batchSize, r->counts[0], r->counts[1], r->counts[2], r->counts[3]};
     sv->kept[r->id] = std::move(r);
     ++n;
   }
@@ -446,6 +473,8 @@ int pxg_server_products(pxg_server* sv, uint64_t id, const void** packed, uint32
   auto it = sv->kept.find(id);
   if (it == sv->kept.end())
     return fail(PXG_ERR_ARG, "pxg_server_products: request %llu not completed or released", (unsigned long long)id);
+  if (!sv->cfg.transfer)
+    return fail(PXG_ERR_ARG, "pxg_server_products: the server was created without transfer");
   *packed = it->second->packed.data();
   *bytes = uint32_t(it->second->packed.size());
   return 0;

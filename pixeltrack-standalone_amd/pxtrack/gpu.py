@@ -65,7 +65,7 @@ class CAInternals(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", u32), ("n_streams", u32), ("hbm_budget", C.c_uint64), ("profile", C.c_int),
-                ("upto", C.c_int), ("doublet_lanes", u32), ("grid_target", u32)]
+                ("upto", C.c_int), ("doublet_lanes", u32), ("grid_target", u32), ("transfer", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -481,9 +481,10 @@ class Server:
     """Continuous-batching serving loop of the library (pxg_server_*)."""
 
     def __init__(self, ctx, max_batch=64, n_streams=3, hbm_budget=0, profile=PROFILE_REFERENCE, upto=5,
-                 doublet_lanes=0, grid_target=0):
+                 doublet_lanes=0, grid_target=0, transfer=True):
         self.ctx, self.lib = ctx, ctx.lib
-        cfg = ServerConfig(max_batch, n_streams, hbm_budget, profile, upto, doublet_lanes, grid_target)
+        cfg = ServerConfig(max_batch, n_streams, hbm_budget, profile, upto, doublet_lanes, grid_target,
+                           1 if transfer else 0)
         self._h = vp()
         _check(self.lib.pxg_server_create(ctx._h, C.byref(cfg), C.byref(self._h)), "pxg_server_create")
 
