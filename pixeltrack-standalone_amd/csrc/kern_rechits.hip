@@ -1,9 +1,10 @@
 // Stage B: clusters -> rec-hits, layer offsets and the per-layer phi histogram.
 //
-//   hitsKernel      one workgroup per (module run, event): cluster extents and
-//                   integer charges accumulated with LDS atomics (exact in any
-//                   order), then one cluster per lane for the cluster position
-//                   estimate and the local -> global transform.  The float
+//   hitsKernel      one wavefront per (module run, event), four per workgroup:
+//                   cluster extents and integer charges accumulated with LDS
+//                   atomics (exact in any order), then one cluster per lane for
+//                   the cluster position estimate and the local -> global
+//                   transform.  The float
 //                   arithmetic keeps the serial program's operation order and
 //                   the library is compiled without FMA contraction, so the
 //                   positions are bit-identical to the serial ones.
@@ -24,7 +25,15 @@ namespace pxg {
 
   namespace {
     constexpr int kBlock = 256;
-    constexpr int kChunk = 256;  // clusters per pass
+    constexpr int kWaves = kBlock / 64;  // module runs per workgroup
+    constexpr int kChunk = 64;           // clusters per pass of a wavefront
+
+    // orders the wavefront's LDS accesses (lanes read what other lanes wrote)
+    __device__ __forceinline__ void waveSync() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 
     __device__ __forceinline__ uint32_t div52(uint32_t n) { return n / 52; }
     __device__ __forceinline__ bool bigX(uint32_t px) { return px == 79 || px == 80; }
@@ -80,7 +89,8 @@ namespace pxg {
   }  // namespace
 
   __global__ void __launch_bounds__(kBlock) hitsKernel(DevConditions c, Pool p) {
-    const uint32_t k = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t k = blockIdx.x * kWaves + wv;
     const uint32_t e = blockIdx.y;
     const EventDesc& ev = p.events[e];
     const BeamSpotPOD bs = ev.bs;
@@ -89,7 +99,7 @@ namespace pxg {
     if (k == 0) {  // average geometry corrected for the beam spot
       const AverageGeometry& ag = *c.cpeAverage;
       AverageGeometry& agc = p.avg[e];
-      for (uint32_t il = threadIdx.x; il < kLaddersInBarrel; il += kBlock) {
+      for (uint32_t il = lane; il < kLaddersInBarrel; il += 64) {
         const float lx = ag.ladderX[il] - bs.x, ly = ag.ladderY[il] - bs.y;
         agc.ladderZ[il] = ag.ladderZ[il] - bs.z;
         agc.ladderX[il] = lx;
@@ -98,7 +108,7 @@ namespace pxg {
         agc.ladderMinZ[il] = ag.ladderMinZ[il] - bs.z;
         agc.ladderMaxZ[il] = ag.ladderMaxZ[il] - bs.z;
       }
-      if (threadIdx.x == 0) {
+      if (lane == 0) {
         agc.endCapZ[0] = ag.endCapZ[0] - bs.z;
         agc.endCapZ[1] = ag.endCapZ[1] - bs.z;
       }
@@ -121,18 +131,22 @@ namespace pxg {
     const CPEDetParams& det = c.cpeDet[me];
     const CPECommonParams com = *c.cpeCommon;
 
-    __shared__ uint32_t minRow[kChunk], maxRow[kChunk], minCol[kChunk], maxCol[kChunk];
-    __shared__ int32_t qsum[kChunk], qfx[kChunk], qlx[kChunk], qfy[kChunk], qly[kChunk];
+    __shared__ uint32_t sMinRow[kWaves][kChunk], sMaxRow[kWaves][kChunk], sMinCol[kWaves][kChunk],
+        sMaxCol[kWaves][kChunk];
+    __shared__ int32_t sQsum[kWaves][kChunk], sQfx[kWaves][kChunk], sQlx[kWaves][kChunk], sQfy[kWaves][kChunk],
+        sQly[kWaves][kChunk];
+    uint32_t *minRow = sMinRow[wv], *maxRow = sMaxRow[wv], *minCol = sMinCol[wv], *maxCol = sMaxCol[wv];
+    int32_t *qsum = sQsum[wv], *qfx = sQfx[wv], *qlx = sQlx[wv], *qfy = sQfy[wv], *qly = sQly[wv];
 
     for (uint32_t start = 0; start < nclus; start += kChunk) {
       const uint32_t nIn = min(uint32_t(kChunk), nclus - start);
-      for (uint32_t ic = threadIdx.x; ic < nIn; ic += kBlock) {
-        minRow[ic] = minCol[ic] = 0xffffffffu;
-        maxRow[ic] = maxCol[ic] = 0;
-        qsum[ic] = qfx[ic] = qlx[ic] = qfy[ic] = qly[ic] = 0;
+      if (lane < nIn) {
+        minRow[lane] = minCol[lane] = 0xffffffffu;
+        maxRow[lane] = maxCol[lane] = 0;
+        qsum[lane] = qfx[lane] = qlx[lane] = qfy[lane] = qly[lane] = 0;
       }
-      __syncthreads();
-      for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
+      waveSync();
+      for (uint32_t i = first + lane; i < end; i += 64) {
         const uint32_t g = wo + i;
         if (p.moduleInd[g] != me)
           continue;  // invalid (or dropped) pixel
@@ -146,8 +160,8 @@ namespace pxg {
         atomicMin(&minCol[j], y);
         atomicMax(&maxCol[j], y);
       }
-      __syncthreads();
-      for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
+      waveSync();
+      for (uint32_t i = first + lane; i < end; i += 64) {
         const uint32_t g = wo + i;
         if (p.moduleInd[g] != me)
           continue;
@@ -167,8 +181,8 @@ namespace pxg {
         if (maxCol[j] == y)
           atomicAdd(&qly[j], ch);
       }
-      __syncthreads();
-      for (uint32_t ic = threadIdx.x; ic < nIn; ic += kBlock) {
+      waveSync();
+      for (uint32_t ic = lane; ic < nIn; ic += 64) {
         const uint32_t h = hitBase + start + ic;
         if (h >= kMaxNumClusters)
           break;
@@ -244,7 +258,7 @@ namespace pxg {
         p.rg[o] = sqrtf(xg * xg + yg * yg);
         p.iphi[o] = atan2s7(yg, xg);
       }
-      __syncthreads();
+      waveSync();
     }
   }
 
@@ -347,7 +361,7 @@ namespace pxg {
   }
 
   void launchHits(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
-    hitsKernel<<<dim3(kMaxNumModules, nEvents), kBlock, 0, s>>>(c, p);
+    hitsKernel<<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(c, p);
   }
 
   void launchPhiBinner(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
