@@ -184,6 +184,12 @@ def main():
     ap.add_argument("--mix-events", type=int, default=200)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB three times (the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--isolated-passes", type=int, default=2,
+                    help="passes over the first stream's batches with the other streams idle, for the "
+                         "per-kernel durations without overlap")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: each box here has one MI355X and events shard across GPUs "
@@ -192,6 +198,16 @@ def main():
 
     from pxtrack import gen, gpu
     import torch
+
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
+        del a, b
 
     t0 = time.time()
     det = gen.Detector()
@@ -263,6 +279,21 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     evps = n * args.steps / elapsed
 
+    # ---- the first stream's batches alone (no overlap with the other streams), for the
+    # kernels' own durations
+    iso = {}
+    if args.isolated_passes > 0:
+        st = streams[0]
+        st.profile(True)
+        st.profile_read()
+        for _ in range(args.isolated_passes):
+            for first in range(0, bounds[1], B):
+                st.run(first, min(B, bounds[1] - first), args.upto)
+        st.sync()
+        iso = st.profile_read()
+        st.profile(False)
+        iso_events = bounds[1] * args.isolated_passes
+
     # per-event counters for the algorithmic bytes
     where = [(st, e) for k, st in enumerate(streams) for e in range(bounds[k + 1] - bounds[k])]
     cnts = np.array([st.event_counts(e) for st, e in where], dtype=np.float64)
@@ -284,6 +315,22 @@ def main():
     alg = algorithmic_bytes(kern, mean) * events_per_launch
     achieved = alg / avg_s / 1e9
     kernel_ms = {k: round(v[1] / v[0], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    kernel_ms_iso = {k: round(v[1] / v[0], 4) for k, v in sorted(iso.items(), key=lambda kv: -kv[1][1])}
+    traffic, iso_roof = None, None
+    tfile = os.path.join(REPO, "profiles", "traffic_chain_pu50_batched128.json")
+    per_event = json.load(open(tfile))["kernels"] if os.path.exists(tfile) else {}
+    if kern in per_event:
+        traffic = round(per_event[kern]["hbm_bytes_per_event"] * events_per_launch)
+    if kern in iso:
+        ic, ims = iso[kern]
+        iso_per_launch = iso_events / ic
+        ialg = algorithmic_bytes(kern, mean) * iso_per_launch
+        iso_roof = {"avg_launch_ms": round(ims / ic, 4), "launches": ic,
+                    "events_per_launch": round(iso_per_launch, 1), "algorithmic_bytes_per_launch": round(ialg),
+                    "achieved": round(ialg / (ims / ic / 1e3) / 1e9, 2),
+                    "frac": round(ialg / (ims / ic / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
+        if kern in per_event:
+            iso_roof["traffic"] = round(per_event[kern]["hbm_bytes_per_event"] * iso_per_launch)
     share = {k: round(v[1] / sum(x[1] for x in prof.values()), 4) for k, v in prof.items()}
 
     # ---- plain per-event path: one launch sequence + host sync per event
@@ -354,11 +401,14 @@ def main():
         "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
         "serving": serving,
         "kernel_ms_per_launch": kernel_ms,
+        "kernel_ms_per_launch_isolated": kernel_ms_iso,
         "kernel_time_share": share,
         "launches": launches_per_kernel,
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes_per_launch": round(alg), "avg_launch_ms": round(avg_s * 1e3, 4)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": round(alg), "avg_launch_ms": round(avg_s * 1e3, 4),
+                     "launches": calls, "events_per_launch": round(events_per_launch, 1),
+                     "isolated": iso_roof},
         "cpu_baseline": cpu,
         "device": gpu.device_name(),
     }

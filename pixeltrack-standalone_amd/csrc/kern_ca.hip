@@ -7,8 +7,9 @@
 // scan, and write each element at its serial position (doublets, tuples), or
 // append with atomics and then sort each short list (outer-hit lists, track
 // lists); kills and flags are written by exactly one thread.  Grid-stride
-// kernels take blockIdx.y = event and stride over that event's items with
-// gridDim.x workgroups; per-event scans run one workgroup per event.
+// kernels run on a (workgroups per event, events) grid, taken in the
+// XCD-aware logical order (logicalBlock), and stride over that event's items
+// with gridDim.x workgroups; per-event scans run one workgroup per event.
 #include <hip/hip_runtime.h>
 
 #include "ca_device.h"
@@ -23,6 +24,7 @@ namespace pxg {
       const uint32_t* layerStart;
       const uint32_t* phiOff;
       const uint16_t* phiBins;
+      const float4* pos;
     };
 
     __device__ __forceinline__ HitView hitsOf(const Pool& p, uint32_t e) {
@@ -36,10 +38,11 @@ namespace pxg {
               p.detInd + h,
               p.hitsLayerStart + size_t(e) * kLayerStartStride,
               p.phiOff + size_t(e) * kPhiOffStride,
-              p.phiBins + h};
+              p.phiBins + h,
+              p.pos + h};
     }
 
-    __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
+    __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
     __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
     __device__ __forceinline__ uint32_t laneId() { return threadIdx.x & 63; }
@@ -73,7 +76,7 @@ namespace pxg {
 
     // wave index within the event's workgroups, uniform over the wave
     __device__ __forceinline__ uint32_t waveStart() {
-      return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+      return __builtin_amdgcn_readfirstlane(logicalBlock().x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     }
     __device__ __forceinline__ uint32_t waveStride() { return gridDim.x * (blockDim.x >> 6); }
 
@@ -147,7 +150,8 @@ namespace pxg {
       constexpr float i2p = 3.14159265358979323846 / 32768;
       if (H.detInd[oi] > 2000)
         return false;
-      const float zo = H.zg[oi], ro = H.rg[oi];
+      const float4 po = H.pos[oi];
+      const float zo = po.z, ro = po.w;
       const float dr = ro - q.mer;
       if (dr > q.maxr || dr < 0 || fabsf(q.mez * ro - q.mer * zo) > z0cut * dr)
         return false;
@@ -175,7 +179,7 @@ namespace pxg {
     // G lanes per slot (G divides 64): 64 / G slots per wave
     template <int G>
     __device__ __forceinline__ uint32_t groupSlotStart() {
-      return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / G) + laneId() / G;
+      return (logicalBlock().x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / G) + laneId() / G;
     }
     template <int G>
     __device__ __forceinline__ uint32_t groupSlotStride() {
@@ -189,7 +193,7 @@ namespace pxg {
 
     template <int G>
     __global__ void __launch_bounds__(kCaBlock) doubletCountKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
       const HitView H = hitsOf(p, e);
       if (threadIdx.x == 0)
@@ -259,7 +263,7 @@ namespace pxg {
 
     template <int G>
     __global__ void __launch_bounds__(kCaBlock) doubletFillKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
       const HitView H = hitsOf(p, e);
       if (threadIdx.x == 0)
@@ -327,7 +331,7 @@ namespace pxg {
     }
 
     __global__ void __launch_bounds__(kCaBlock) ohFillKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       uint32_t* cur = ca.ohCount + size_t(e) * kHitStride;
@@ -361,7 +365,7 @@ namespace pxg {
     // `me` (the serial program's append order); `oc` must be among the first
     // 128 cells of its outer hit's list (GPUCACell::OuterHitOfCell).
     __global__ void __launch_bounds__(kCaBlock) connectKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
       const HitView H = hitsOf(p, e);
       if (threadIdx.x == 0)
@@ -385,8 +389,9 @@ namespace pxg {
         if (!listedInOuterHit(ca.ohCells + cb, lb, ln, oc))
           continue;
         const int lo = kPairOuter[ca.cellPair[cb + oc]];
-        const float r1 = H.rg[hi], z1 = H.zg[hi], x1 = H.xg[hi], y1 = H.yg[hi];
-        const float ri = H.rg[hm], zi = H.zg[hm], x2 = H.xg[hm], y2 = H.yg[hm];
+        const float4 p1 = H.pos[hi], p2 = H.pos[hm];
+        const float r1 = p1.w, z1 = p1.z, x1 = p1.x, y1 = p1.y;
+        const float ri = p2.w, zi = p2.z, x2 = p2.x, y2 = p2.y;
         const float thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
         const float dcut = H.detInd[hi] < 96 ? kCaDcaCutInner : kCaDcaCutOuter;
         int32_t slot = -1;
@@ -399,10 +404,10 @@ namespace pxg {
           const uint32_t beg = dOff[j], end = min(dOff[j + 1], nCells);
           for (uint32_t me = beg; me < end; ++me) {
             const uint32_t ho = ca.cellOuter[cb + me];
-            const float ro = H.rg[ho], zo = H.zg[ho];
-            if (!alignedRZ(r1, z1, ri, zi, ro, zo, kCaPtMin, thetaCut))
+            const float4 p3 = H.pos[ho];
+            if (!alignedRZ(r1, z1, ri, zi, p3.w, p3.z, kCaPtMin, thetaCut))
               continue;
-            if (!dcaCut(x1, y1, x2, y2, H.xg[ho], H.yg[ho], dcut, kCaHardCurvCut))
+            if (!dcaCut(x1, y1, x2, y2, p3.x, p3.y, dcut, kCaHardCurvCut))
               continue;
             atomicOr(&used[me >> 5], 1u << (me & 31));
             if (n == 0)
@@ -443,7 +448,7 @@ namespace pxg {
     // rows do not read the kills, so each row runs on its own (one thread per
     // cell, in list order), finding its first killer as the smallest such cell.
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const HitView H = hitsOf(p, e);
       if (H.layerStart[10] < 2)
         return;
@@ -475,10 +480,12 @@ namespace pxg {
               limit = c;
           }
         }
-        const float xo = H.xg[h], yo = H.yg[h], zo = H.zg[h];
+        const float4 ph = H.pos[h];
+        const float xo = ph.x, yo = ph.y, zo = ph.z;
         const uint32_t hi = ca.cellInner[cb + ci];
         const uint16_t di = H.detInd[hi];
-        const float xi = H.xg[hi] - xo, yi = H.yg[hi] - yo, zi = H.zg[hi] - zo;
+        const float4 pi = H.pos[hi];
+        const float xi = pi.x - xo, yi = pi.y - yo, zi = pi.z - zo;
         const float ni = xi * xi + yi * yi + zi * zi;
         // one pass: the aligned later cells, and the first closer one among them
         uint32_t killer = limit;
@@ -488,7 +495,8 @@ namespace pxg {
           if (cj <= ci || cj >= limit || !isUsed(cj))
             continue;
           const uint32_t hj = ca.cellInner[cb + cj];
-          const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
+          const float4 pj = H.pos[hj];
+          const float xj = pj.x - xo, yj = pj.y - yo, zj = pj.z - zo;
           const float nj = xj * xj + yj * yj + zj * zj;
           const float cos12 = xi * xj + yi * yj + zi * zj;
           if (!(di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj))
@@ -511,7 +519,8 @@ namespace pxg {
             if (cj <= ci || cj >= killer || !isUsed(cj))
               continue;
             const uint32_t hj = ca.cellInner[cb + cj];
-            const float xj = H.xg[hj] - xo, yj = H.yg[hj] - yo, zj = H.zg[hj] - zo;
+            const float4 pj = H.pos[hj];
+            const float xj = pj.x - xo, yj = pj.y - yo, zj = pj.z - zo;
             const float nj = xj * xj + yj * yj + zj * zj;
             const float cos12 = xi * xj + yi * yj + zi * zj;
             if (di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj)
@@ -597,7 +606,7 @@ namespace pxg {
     // are compacted in ascending order first: a bit per cell, a scan of the
     // words' popcounts, and each word's cells written at its offset.
     __global__ void __launch_bounds__(kCaBlock) startMaskKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       uint64_t* mask = ca.startMask + size_t(e) * (ca.caps.maxDoublets / 64 + 64);
@@ -640,7 +649,7 @@ namespace pxg {
     }
 
     __global__ void __launch_bounds__(kCaBlock) ntupletCountKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
       const uint32_t* list = ca.startCells + cb;
@@ -687,7 +696,7 @@ namespace pxg {
     }
 
     __global__ void __launch_bounds__(kCaBlock) ntupletFillKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
       const uint32_t* list = ca.startCells + cb;
@@ -704,12 +713,12 @@ namespace pxg {
 
     // HistoContainer.h:193-204 bulkFinalizeFill, and one track list slot per cell in a tuple
     __global__ void __launch_bounds__(kCaBlock) trackListAllocKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const CACounts cnt = ca.cnt[e];
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
       const uint32_t M = ca.caps.maxTuples;
-      if (blockIdx.x == 0) {
+      if (logicalBlock().x == 0) {
         uint32_t* off = ca.tupleOff + ca.offBase(e);
         if (cnt.apcM >= M) {
           if (threadIdx.x == 0)
@@ -739,7 +748,7 @@ namespace pxg {
     }
 
     __global__ void __launch_bounds__(kCaBlock) trackListFillKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
@@ -761,7 +770,7 @@ namespace pxg {
     // runs a16 kernel_earlyDuplicateRemover (CAHitNtupletGeneratorKernelsImpl.h:126-149)
     // on it: order independent, it only marks shorter tuples as duplicates.
     __global__ void __launch_bounds__(kCaBlock) trackListSortKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nCells = ca.cnt[e].nCells;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t cb = ca.cellBase(e);

@@ -16,12 +16,12 @@
 namespace pxg {
 
   namespace {
-    __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
+    __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
     __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
     // a20 kernel_classifyTracks
     __global__ void __launch_bounds__(kCaBlock) classifyKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
       const uint32_t M = ca.caps.maxTuples;
@@ -120,7 +120,7 @@ namespace pxg {
 
     // a22 hit -> tuple association of the loose tuples
     __global__ void __launch_bounds__(kCaBlock) hitToTupleCountKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
@@ -160,7 +160,7 @@ namespace pxg {
     }
 
     __global__ void __launch_bounds__(kCaBlock) hitToTupleFillKernel(CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
@@ -182,7 +182,7 @@ namespace pxg {
     // sorts each hit's tuples (descending, the serial fill order) and runs the
     // triplet cleaner (CAHitNtupletGeneratorKernelsImpl.h:479-534) on them
     __global__ void __launch_bounds__(kCaBlock) tripletCleanerKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.y;
+      const uint32_t e = logicalBlock().y;
       const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
       const size_t tb = ca.tupleBase(e);
       const uint32_t M = ca.caps.maxTuples;

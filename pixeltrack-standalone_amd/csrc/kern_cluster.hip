@@ -298,8 +298,9 @@ namespace pxg {
   __global__ void __launch_bounds__(kBlock) clusterSmallKernel(Pool p) {
     __shared__ int32_t rowLds[kWaves][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t k = blockIdx.x * kWaves + wave;
-    const uint32_t e = blockIdx.y;
+    const LogicalBlock lb = logicalBlock();
+    const uint32_t k = lb.x * kWaves + wave;
+    const uint32_t e = lb.y;
     const EventDesc& ev = p.events[e];
     const uint32_t nModules = p.counts[e].nModules;
     if (k >= nModules)

@@ -90,8 +90,9 @@ namespace pxg {
 
   __global__ void __launch_bounds__(kBlock) hitsKernel(DevConditions c, Pool p) {
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t k = blockIdx.x * kWaves + wv;
-    const uint32_t e = blockIdx.y;
+    const LogicalBlock lb = logicalBlock();
+    const uint32_t k = lb.x * kWaves + wv;
+    const uint32_t e = lb.y;
     const EventDesc& ev = p.events[e];
     const BeamSpotPOD bs = ev.bs;
     const uint32_t nModules = p.counts[e].nModules;
@@ -255,7 +256,9 @@ namespace pxg {
         p.xg[o] = xg;
         p.yg[o] = yg;
         p.zg[o] = zg;
-        p.rg[o] = sqrtf(xg * xg + yg * yg);
+        const float rg = sqrtf(xg * xg + yg * yg);
+        p.rg[o] = rg;
+        p.pos[o] = make_float4(xg, yg, zg, rg);
         p.iphi[o] = atan2s7(yg, xg);
       }
       waveSync();

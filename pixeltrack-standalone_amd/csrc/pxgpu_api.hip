@@ -93,6 +93,7 @@ namespace pxg::detail {
     f(p.yg, H);
     f(p.zg, H);
     f(p.rg, H);
+    f(p.pos, H);
     f(p.iphi, H);
     f(p.charge, H);
     f(p.xsize, H);

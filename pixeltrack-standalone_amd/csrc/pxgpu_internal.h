@@ -87,6 +87,7 @@ namespace pxg {
     uint32_t *moduleStart, *clusInModule, *moduleId, *clusModuleStart;  // stride kMaxModuleRanges
     // hits, per event (stride kMaxNumClusters)
     float *xl, *yl, *xerr, *yerr, *xg, *yg, *zg, *rg;
+    float4* pos;  // (xg, yg, zg, rg) of each hit in one 16-byte load, for the CA kernels' gathers
     int16_t* iphi;
     int32_t* charge;
     int16_t *xsize, *ysize;
@@ -106,6 +107,30 @@ namespace pxg {
   constexpr uint32_t kHitStride = kMaxNumClusters + 64;      // per-event stride of per-hit offset arrays
   constexpr uint32_t kCaBlock = 256;                         // workgroup of the grid-stride CA kernels
   constexpr uint32_t kEventBlock = 1024;                     // workgroup of the per-event kernels
+
+  // ---- XCD-aware workgroup order -------------------------------------------------------
+  // The dispatcher hands workgroups to the chip's 8 XCDs round-robin by linear
+  // id, and each XCD has its own L2.  Kernels on a (workgroups per event,
+  // events) grid read an event's hits and cells from every one of its
+  // workgroups; taken in dispatch order those workgroups would sit on all eight
+  // XCDs and the event's data would be fetched into all eight L2s.  This
+  // permutation of [0, gridDim.x * gridDim.y) gives consecutive logical ids --
+  // the workgroups of one event -- one XCD: XCD k runs the k-th contiguous
+  // share of the logical ids (the first n % 8 XCDs one workgroup more).  Every
+  // logical workgroup still runs exactly once.
+  constexpr uint32_t kXcds = 8;
+  __host__ __device__ constexpr uint32_t xcdLogicalId(uint32_t pid, uint32_t n) {
+    const uint32_t xcd = pid % kXcds, local = pid / kXcds;
+    const uint32_t per = (n + kXcds - 1) / kXcds, tall = n % kXcds;
+    return (tall == 0 || xcd < tall) ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+  }
+  struct LogicalBlock {
+    uint32_t x, y;
+  };
+  __device__ __forceinline__ LogicalBlock logicalBlock() {
+    const uint32_t l = xcdLogicalId(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    return {l % gridDim.x, l / gridDim.x};
+  }
 
   // Per-event counters of the ntuplet and vertex stages.
   struct CACounts {
