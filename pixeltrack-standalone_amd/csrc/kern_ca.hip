@@ -402,32 +402,41 @@ namespace pxg {
             break;
           const uint32_t j = (pr ? T.cum[pr - 1] : 0) + (hm - T.layer[lo]);
           const uint32_t beg = dOff[j], end = min(dOff[j + 1], nCells);
-          for (uint32_t me = beg; me < end; ++me) {
-            const uint32_t ho = ca.cellOuter[cb + me];
-            const float4 p3 = H.pos[ho];
-            if (!alignedRZ(r1, z1, ri, zi, p3.w, p3.z, kCaPtMin, thetaCut))
-              continue;
-            if (!dcaCut(x1, y1, x2, y2, p3.x, p3.y, dcut, kCaHardCurvCut))
-              continue;
-            atomicOr(&used[me >> 5], 1u << (me & 31));
-            if (n == 0)
-              atomicOr(&used[oc >> 5], 1u << (oc & 31));
-            if (n < kCaMaxNeighbors) {
-              if (slot == -1) {
-                const uint32_t s = waveAlloc(&ca.cnt[e].nbLists, true);
-                if (s + 1 < ca.caps.maxCellLists) {
-                  slot = int32_t(s);
-                } else {
-                  slot = -2;
-                  status |= kCaNeighborStoreOverflow;
-                }
-              }
-              if (slot >= 0)
-                pool[size_t(slot) * kCaMaxNeighbors + n] = me;
-            } else {
-              status |= kCaNeighborOverflow;
+          // chunks of up to 64 candidates: the cuts first, in a loop without
+          // stores (so the candidates' loads are issued together), then the
+          // accepted ones in ascending order
+          for (uint32_t base = beg; base < end; base += 64) {
+            const uint32_t m = min(64u, end - base);
+            uint64_t acc = 0;
+#pragma unroll 4
+            for (uint32_t k = 0; k < m; ++k) {
+              const float4 p3 = H.pos[ca.cellOuter[cb + base + k]];
+              if (alignedRZ(r1, z1, ri, zi, p3.w, p3.z, kCaPtMin, thetaCut) &&
+                  dcaCut(x1, y1, x2, y2, p3.x, p3.y, dcut, kCaHardCurvCut))
+                acc |= 1ull << k;
             }
-            ++n;
+            for (; acc; acc &= acc - 1) {
+              const uint32_t me = base + __ffsll((unsigned long long)acc) - 1;
+              atomicOr(&used[me >> 5], 1u << (me & 31));
+              if (n == 0)
+                atomicOr(&used[oc >> 5], 1u << (oc & 31));
+              if (n < kCaMaxNeighbors) {
+                if (slot == -1) {
+                  const uint32_t s = waveAlloc(&ca.cnt[e].nbLists, true);
+                  if (s + 1 < ca.caps.maxCellLists) {
+                    slot = int32_t(s);
+                  } else {
+                    slot = -2;
+                    status |= kCaNeighborStoreOverflow;
+                  }
+                }
+                if (slot >= 0)
+                  pool[size_t(slot) * kCaMaxNeighbors + n] = me;
+              } else {
+                status |= kCaNeighborOverflow;
+              }
+              ++n;
+            }
           }
         }
         if (slot >= 0) {
@@ -490,6 +499,7 @@ namespace pxg {
         // one pass: the aligned later cells, and the first closer one among them
         uint32_t killer = limit;
         uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
+#pragma unroll 4
         for (uint32_t k = 0; k < n; ++k) {
           const uint32_t cj = cells[b + k];
           if (cj <= ci || cj >= limit || !isUsed(cj))
