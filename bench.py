@@ -372,9 +372,22 @@ def main():
         a = time.perf_counter()
         done = orc.run_many(sample, upto=args.upto, threads=ncores, repeat=rep)
         ct = time.perf_counter() - a
+        # one thread, one event at a time: the serial program's per-event latency
+        a = time.perf_counter()
+        one_done = orc.run_many(sample[:20], upto=args.upto, threads=1, repeat=1)
+        one_ms = 1e3 * (time.perf_counter() - a) / max(1, one_done)
+        model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": round(done / ct, 1), "unit": "events/s", "cores": ncores, "kind": "port",
                "sample": f"{len(sample)} of the PU50 events x {rep} passes, {STAGES[args.upto]}, one event per thread "
-                         f"at a time ({ncores} threads), {ct:.1f} s"}
+                         f"at a time ({ncores} threads), {ct:.1f} s",
+               "single_thread_ms_per_event": round(one_ms, 3), "cpu_model": model}
 
     batch_lat_ms = ms_per_step / ((n + B - 1) // B)
     out = {

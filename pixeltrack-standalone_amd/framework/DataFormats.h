@@ -96,3 +96,62 @@ struct ZVertexHeterogeneous {
   pxg_vertices view;
   uint32_t nvFinal() const { return view.nvFinal; }
 };
+
+// ---- expected counts read with the raw data (DataFormats/DigiClusterCount.h, TrackCount.h,
+// VertexCount.h; bin/Source.cc:58-68 reads them from digicluster.bin, tracks.bin, vertices.bin)
+class DigiClusterCount {
+public:
+  DigiClusterCount(uint32_t nm, uint32_t nd, uint32_t nc) : modules_(nm), digis_(nd), clusters_(nc) {}
+  uint32_t nModules() const { return modules_; }
+  uint32_t nDigis() const { return digis_; }
+  uint32_t nClusters() const { return clusters_; }
+
+private:
+  uint32_t modules_, digis_, clusters_;
+};
+
+class TrackCount {
+public:
+  explicit TrackCount(uint32_t n) : tracks_(n) {}
+  uint32_t nTracks() const { return tracks_; }
+
+private:
+  uint32_t tracks_;
+};
+
+class VertexCount {
+public:
+  explicit VertexCount(uint32_t n) : vertices_(n) {}
+  uint32_t nVertices() const { return vertices_; }
+
+private:
+  uint32_t vertices_;
+};
+
+// ---- host copies made by the transfer modules (the CUDA program's *SoAFromCUDA products) ----
+struct SiPixelDigisSoAHost {  // SiPixelDigisSoAFromCUDA.cc: adc, clus, pdigi, rawIdArr
+  uint32_t nModules = 0, nDigis = 0;
+  std::vector<uint16_t> adc;
+  std::vector<int32_t> clus;
+  std::vector<uint32_t> pdigi, rawIdArr;
+};
+
+struct PixelTrackSoAHost {  // TrackSoAT (PixelTrackHeterogeneous.h) for the stored tuples
+  uint32_t nTracks = 0;
+  std::vector<uint32_t> hitOff;  // [nTracks + 1]
+  std::vector<uint16_t> hitIndices;
+  std::vector<uint8_t> quality;
+  std::vector<float> chi2, eta, pt;
+  std::vector<float> state;  // [5][nTracks]: phi, tip, q/pt, cot(theta), zip
+  int nHits(uint32_t i) const { return int(hitOff[i + 1] - hitOff[i]); }
+  float phi(uint32_t i) const { return state[i]; }
+  float tip(uint32_t i) const { return state[nTracks + i]; }
+  float zip(uint32_t i) const { return state[4 * nTracks + i]; }
+};
+
+struct ZVertexSoAHost {  // ZVertexSoA for the final vertices
+  uint32_t nvFinal = 0;
+  std::vector<float> zv, wv, chi2, ptv2;
+  std::vector<int32_t> ndof;
+  std::vector<uint16_t> sortInd;
+};
