@@ -456,89 +456,132 @@ namespace pxg {
     // aligned later cell that is closer, which kills ci and ends the row.  The
     // rows do not read the kills, so each row runs on its own (one thread per
     // cell, in list order), finding its first killer as the smallest such cell.
+    // A cell seen from the list's hit: its inner hit relative to that hit.
+    struct RowCell {
+      uint32_t c;
+      uint32_t det;      // detInd of the inner hit; bit 16: the cell is used
+      float x, y, z, n;  // inner hit minus the list's hit, squared norm
+    };
+
+    __device__ __forceinline__ void waveLdsSync() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const HitView H = hitsOf(p, e);
       if (H.layerStart[10] < 2)
         return;
+      // each lane publishes its own cell; the lists that lie inside the
+      // wave's 64 positions read their cells from here instead of memory
+      __shared__ float4 sDir[kCaBlock];
+      __shared__ uint2 sCell[kCaBlock];
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       const uint32_t* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
-      for (uint32_t pos = gridStart(); pos < nCells; pos += gridStride()) {
-        const uint32_t ci = cells[pos];
-        if (!isUsed(ci))
-          continue;
-        const uint32_t h = ca.cellOuter[cb + ci];
-        const uint32_t b = off[h], n = off[h + 1] - b;
-        if (n < 2)
-          continue;
-        // with more than 128 cells only the 128 lowest indices take part
-        uint32_t limit = 0xffffffffu;
-        if (n > kCaMaxCellsPerHit) {
-          if (!listedInOuterHit(cells, b, n, ci))
-            continue;
-          for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t c = cells[b + k];
-            uint32_t rank = 0;
-            for (uint32_t m = 0; m < n; ++m)
-              rank += cells[b + m] < c;
-            if (rank == kCaMaxCellsPerHit)
-              limit = c;
+      const uint32_t lane = laneId(), wbase = threadIdx.x - lane;
+      for (uint32_t pos0 = gridStart() - lane; pos0 < nCells; pos0 += gridStride()) {  // uniform per wave
+        const uint32_t pos = pos0 + lane;
+        RowCell me{0, 0, 0.f, 0.f, 0.f, 0.f};
+        uint32_t b = 0, n = 0;
+        float4 ph{0.f, 0.f, 0.f, 0.f};
+        if (pos < nCells) {
+          me.c = cells[pos];
+          const uint32_t h = ca.cellOuter[cb + me.c], hi = ca.cellInner[cb + me.c];
+          b = off[h];
+          n = off[h + 1] - b;
+          ph = H.pos[h];
+          const float4 pi = H.pos[hi];
+          me.x = pi.x - ph.x;
+          me.y = pi.y - ph.y;
+          me.z = pi.z - ph.z;
+          me.n = me.x * me.x + me.y * me.y + me.z * me.z;
+          me.det = uint32_t(H.detInd[hi]) | (isUsed(me.c) << 16);
+        }
+        sDir[threadIdx.x] = make_float4(me.x, me.y, me.z, me.n);
+        sCell[threadIdx.x] = make_uint2(me.c, me.det);
+        waveLdsSync();
+        const uint32_t ci = me.c;
+        if (pos < nCells && (me.det >> 16) && n >= 2) {
+          // with more than 128 cells only the 128 lowest indices take part
+          uint32_t limit = 0xffffffffu;
+          bool listed = true;
+          if (n > kCaMaxCellsPerHit) {
+            listed = listedInOuterHit(cells, b, n, ci);
+            for (uint32_t k = 0; listed && k < n; ++k) {
+              const uint32_t c = cells[b + k];
+              uint32_t rank = 0;
+              for (uint32_t m = 0; m < n; ++m)
+                rank += cells[b + m] < c;
+              if (rank == kCaMaxCellsPerHit)
+                limit = c;
+            }
           }
-        }
-        const float4 ph = H.pos[h];
-        const float xo = ph.x, yo = ph.y, zo = ph.z;
-        const uint32_t hi = ca.cellInner[cb + ci];
-        const uint16_t di = H.detInd[hi];
-        const float4 pi = H.pos[hi];
-        const float xi = pi.x - xo, yi = pi.y - yo, zi = pi.z - zo;
-        const float ni = xi * xi + yi * yi + zi * zi;
-        // one pass: the aligned later cells, and the first closer one among them
-        uint32_t killer = limit;
-        uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
-#pragma unroll 4
-        for (uint32_t k = 0; k < n; ++k) {
-          const uint32_t cj = cells[b + k];
-          if (cj <= ci || cj >= limit || !isUsed(cj))
-            continue;
-          const uint32_t hj = ca.cellInner[cb + cj];
-          const float4 pj = H.pos[hj];
-          const float xj = pj.x - xo, yj = pj.y - yo, zj = pj.z - zo;
-          const float nj = xj * xj + yj * yj + zj * zj;
-          const float cos12 = xi * xj + yi * yj + zi * zj;
-          if (!(di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj))
-            continue;
-          if (ni > nj && cj < killer)
-            killer = cj;
-          if (k < 64)
-            alignedMask |= 1ull << k;
-        }
-        // the aligned cells before the killer are killed (several rows may kill
-        // a cell: they store the same byte)
-        for (uint64_t m = alignedMask; m; m &= m - 1) {
-          const uint32_t cj = cells[b + __ffsll((unsigned long long)m) - 1];
-          if (cj < killer)
-            ca.cellFlags[cb + cj] |= 2;
-        }
-        if (n > 64) {  // positions past 64: second pass
-          for (uint32_t k = 64; k < n; ++k) {
-            const uint32_t cj = cells[b + k];
-            if (cj <= ci || cj >= killer || !isUsed(cj))
-              continue;
-            const uint32_t hj = ca.cellInner[cb + cj];
+          auto fromMemory = [&](uint32_t k) {
+            RowCell d;
+            d.c = cells[b + k];
+            const uint32_t hj = ca.cellInner[cb + d.c];
             const float4 pj = H.pos[hj];
-            const float xj = pj.x - xo, yj = pj.y - yo, zj = pj.z - zo;
-            const float nj = xj * xj + yj * yj + zj * zj;
-            const float cos12 = xi * xj + yi * yj + zi * zj;
-            if (di != H.detInd[hj] && cos12 * cos12 >= 0.99999f * ni * nj)
-              ca.cellFlags[cb + cj] |= 2;
+            d.x = pj.x - ph.x;
+            d.y = pj.y - ph.y;
+            d.z = pj.z - ph.z;
+            d.n = d.x * d.x + d.y * d.y + d.z * d.z;
+            d.det = uint32_t(H.detInd[hj]) | (isUsed(d.c) << 16);
+            return d;
+          };
+          const uint32_t lo = wbase + (b - pos0);
+          auto fromLds = [&](uint32_t k) {
+            const float4 v = sDir[lo + k];
+            const uint2 c = sCell[lo + k];
+            return RowCell{c.x, c.y, v.x, v.y, v.z, v.w};
+          };
+          // one pass: the aligned later cells, and the first closer one among them
+          auto row = [&](auto get) {
+            uint32_t killer = limit;
+            uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
+            for (uint32_t k = 0; k < n; ++k) {
+              const RowCell d = get(k);
+              if (d.c <= ci || d.c >= limit || !(d.det >> 16))
+                continue;
+              const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
+              if (!((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n))
+                continue;
+              if (me.n > d.n && d.c < killer)
+                killer = d.c;
+              if (k < 64)
+                alignedMask |= 1ull << k;
+            }
+            // the aligned cells before the killer are killed (several rows may
+            // kill a cell: they store the same byte)
+            for (uint64_t m = alignedMask; m; m &= m - 1) {
+              const uint32_t cj = get(__ffsll((unsigned long long)m) - 1).c;
+              if (cj < killer)
+                ca.cellFlags[cb + cj] |= 2;
+            }
+            for (uint32_t k = 64; k < n; ++k) {  // positions past 64: second pass
+              const RowCell d = get(k);
+              if (d.c <= ci || d.c >= killer || !(d.det >> 16))
+                continue;
+              const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
+              if ((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n)
+                ca.cellFlags[cb + d.c] |= 2;
+            }
+            if (killer != limit)
+              ca.cellFlags[cb + ci] |= 2;
+          };
+          if (listed) {
+            if (n <= 64 && b >= pos0 && b + n <= pos0 + 64)
+              row(fromLds);
+            else
+              row(fromMemory);
           }
         }
-        if (killer != limit)
-          ca.cellFlags[cb + ci] |= 2;
+        waveLdsSync();
       }
     }
 
