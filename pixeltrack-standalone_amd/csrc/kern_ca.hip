@@ -364,9 +364,45 @@ namespace pxg {
     // whose inner hit is oc's outer hit and which pass the cuts, ascending in
     // `me` (the serial program's append order); `oc` must be among the first
     // 128 cells of its outer hit's list (GPUCACell::OuterHitOfCell).
+    //
+    // Thread per cell in outer-hit list order, so the lanes of a wave form a few
+    // groups sharing an outer hit and hence a candidate list.  The wave first
+    // stages the candidates of all its groups in LDS (cell and hit position,
+    // loaded by all lanes at once), then each lane runs the cuts of its cell
+    // over its group's staged candidates.  Groups that do not fit the staging
+    // area read their candidates from memory.
+    __device__ __forceinline__ void waveLdsSync() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    constexpr uint32_t kStage = 128;  // staged candidate cells per wave
+    constexpr uint32_t kSegs = 64;    // staged candidate ranges per wave
+
+    __device__ __forceinline__ uint32_t waveExclusiveSum(uint32_t v) {
+      uint32_t x = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (laneId() >= uint32_t(o))
+          x += y;
+      }
+      return x - v;
+    }
+
+    __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
+      for (int o = 32; o > 0; o >>= 1)
+        v = max(v, uint32_t(__shfl_xor(v, o)));
+      return v;
+    }
+
     __global__ void __launch_bounds__(kCaBlock) connectKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
+      __shared__ uint32_t sMe[kCaBlock / 64][kStage];
+      __shared__ float4 sPos[kCaBlock / 64][kStage];
+      __shared__ uint32_t sSegBeg[kCaBlock / 64][kSegs];
+      __shared__ uint32_t sSegBase[kCaBlock / 64][kSegs];
       const HitView H = hitsOf(p, e);
       if (threadIdx.x == 0)
         loadPairTable(H.layerStart, T);
@@ -377,73 +413,155 @@ namespace pxg {
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
+      const uint32_t lane = laneId(), w = threadIdx.x >> 6;
       uint32_t status = 0, stored = 0;
-      // cells in outer-hit list order: neighbouring lanes mostly share their
-      // outer hit, hence the candidate cells and their hits (shared loads)
-      for (uint32_t pos = gridStart(); pos < nCells; pos += gridStride()) {
-        const uint32_t oc = ca.ohCells[cb + pos];
-        const uint32_t hi = ca.cellInner[cb + oc], hm = ca.cellOuter[cb + oc];
-        const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
-        if (ln > kCaMaxCellsPerHit)
-          status |= kCaOuterHitOverflow;
-        if (!listedInOuterHit(ca.ohCells + cb, lb, ln, oc))
-          continue;
-        const int lo = kPairOuter[ca.cellPair[cb + oc]];
-        const float4 p1 = H.pos[hi], p2 = H.pos[hm];
-        const float r1 = p1.w, z1 = p1.z, x1 = p1.x, y1 = p1.y;
-        const float ri = p2.w, zi = p2.z, x2 = p2.x, y2 = p2.y;
-        const float thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
-        const float dcut = H.detInd[hi] < 96 ? kCaDcaCutInner : kCaDcaCutOuter;
-        int32_t slot = -1;
-        uint32_t n = 0;
+      for (uint32_t pos0 = gridStart() - lane; pos0 < nCells; pos0 += gridStride()) {  // uniform per wave
+        const uint32_t pos = pos0 + lane;
+        const bool valid = pos < nCells;
+        uint32_t oc = 0, hi = 0, hm = 0xffffffffu;
+        bool live = false;
+        float4 p1{0.f, 0.f, 0.f, 0.f}, p2{0.f, 0.f, 0.f, 0.f};
+        float thetaCut = 0.f, dcut = 0.f;
+        // candidate ranges: the cells of each pair from oc's outer layer starting at hm
+        uint32_t beg[6], len[6];
+        uint32_t total = 0, nseg = 0;
+        if (valid) {
+          oc = ca.ohCells[cb + pos];
+          hi = ca.cellInner[cb + oc];
+          hm = ca.cellOuter[cb + oc];
+          const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
+          if (ln > kCaMaxCellsPerHit)
+            status |= kCaOuterHitOverflow;
+          live = listedInOuterHit(ca.ohCells + cb, lb, ln, oc);
+          p1 = H.pos[hi];
+          p2 = H.pos[hm];
+          thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
+          dcut = H.detInd[hi] < 96 ? kCaDcaCutInner : kCaDcaCutOuter;
+        }
+        const int lo = valid ? kPairOuter[ca.cellPair[cb + oc]] : 0;
+        bool more = valid;
+#pragma unroll
         for (int q = 0; q < 6; ++q) {
-          const int pr = kPairsFromLayer[lo][q];
-          if (pr < 0)
-            break;
-          const uint32_t j = (pr ? T.cum[pr - 1] : 0) + (hm - T.layer[lo]);
-          const uint32_t beg = dOff[j], end = min(dOff[j + 1], nCells);
-          // chunks of up to 64 candidates: the cuts first, in a loop without
-          // stores (so the candidates' loads are issued together), then the
-          // accepted ones in ascending order
-          for (uint32_t base = beg; base < end; base += 64) {
-            const uint32_t m = min(64u, end - base);
-            uint64_t acc = 0;
-#pragma unroll 4
-            for (uint32_t k = 0; k < m; ++k) {
-              const float4 p3 = H.pos[ca.cellOuter[cb + base + k]];
-              if (alignedRZ(r1, z1, ri, zi, p3.w, p3.z, kCaPtMin, thetaCut) &&
-                  dcaCut(x1, y1, x2, y2, p3.x, p3.y, dcut, kCaHardCurvCut))
-                acc |= 1ull << k;
-            }
-            for (; acc; acc &= acc - 1) {
-              const uint32_t me = base + __ffsll((unsigned long long)acc) - 1;
-              atomicOr(&used[me >> 5], 1u << (me & 31));
-              if (n == 0)
-                atomicOr(&used[oc >> 5], 1u << (oc & 31));
-              if (n < kCaMaxNeighbors) {
-                if (slot == -1) {
-                  const uint32_t s = waveAlloc(&ca.cnt[e].nbLists, true);
-                  if (s + 1 < ca.caps.maxCellLists) {
-                    slot = int32_t(s);
-                  } else {
-                    slot = -2;
-                    status |= kCaNeighborStoreOverflow;
-                  }
-                }
-                if (slot >= 0)
-                  pool[size_t(slot) * kCaMaxNeighbors + n] = me;
-              } else {
-                status |= kCaNeighborOverflow;
-              }
-              ++n;
+          beg[q] = 0;
+          len[q] = 0;
+          const int pr = more ? kPairsFromLayer[lo][q] : -1;
+          more = pr >= 0;
+          if (more) {
+            const uint32_t j = (pr ? T.cum[pr - 1] : 0) + (hm - T.layer[lo]);
+            const uint32_t b = dOff[j], en = min(dOff[j + 1], nCells);
+            beg[q] = b;
+            len[q] = en > b ? en - b : 0;
+            total += len[q];
+            nseg += len[q] > 0;
+          }
+        }
+        // groups: consecutive lanes with the same outer hit; the first lane leads
+        const uint32_t hmPrev = __shfl_up(hm, 1);
+        const bool leader = valid && (lane == 0 || hmPrev != hm);
+        const uint32_t segOff = waveExclusiveSum(leader ? nseg : 0);
+        const uint32_t candOff = waveExclusiveSum(leader ? total : 0);
+        const bool fits = leader && segOff + nseg <= kSegs && candOff + total <= kStage;
+        if (fits) {
+          uint32_t sg = segOff, base = candOff;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            if (len[q]) {
+              sSegBeg[w][sg] = beg[q];
+              sSegBase[w][sg] = base;
+              ++sg;
+              base += len[q];
             }
           }
         }
-        if (slot >= 0) {
-          ca.nbSlot[cb + oc] = slot;
-          ca.nbCount[cb + oc] = uint8_t(min(n, kCaMaxNeighbors));
-          stored += min(n, kCaMaxNeighbors);
+        // the groups that fit are the first ones of the wave (offsets only grow)
+        const uint32_t nSegs = waveMax(fits ? segOff + nseg : 0);
+        const uint32_t nStaged = waveMax(fits ? candOff + total : 0);
+        const uint64_t leaders = __ballot(leader);
+        const uint64_t upTo = lane == 63 ? ~0ull : (2ull << lane) - 1;
+        const uint64_t mine = leaders & upTo;
+        const uint32_t lead = mine ? 63 - __clzll(mine) : 0;
+        const bool staged = __shfl(fits ? 1u : 0u, lead) != 0;
+        const uint32_t gBase = __shfl(candOff, lead);
+        waveLdsSync();
+        // all lanes load the staged candidates: slot s belongs to the last
+        // range whose base is <= s
+        for (uint32_t s = lane; s < nStaged; s += 64) {
+          uint32_t lo2 = 0, hi2 = nSegs;  // sSegBase[lo2] <= s < sSegBase[hi2] (hi2 = nSegs: end)
+          while (hi2 - lo2 > 1) {
+            const uint32_t mid = (lo2 + hi2) >> 1;
+            if (sSegBase[w][mid] <= s)
+              lo2 = mid;
+            else
+              hi2 = mid;
+          }
+          const uint32_t me = sSegBeg[w][lo2] + (s - sSegBase[w][lo2]);
+          sMe[w][s] = me;
+          sPos[w][s] = H.pos[ca.cellOuter[cb + me]];
         }
+        waveLdsSync();
+        if (valid && live) {
+          int32_t slot = -1;
+          uint32_t n = 0;
+          // the accepted candidates, in ascending order within each range
+          auto append = [&](uint32_t me) {
+            atomicOr(&used[me >> 5], 1u << (me & 31));
+            if (n == 0)
+              atomicOr(&used[oc >> 5], 1u << (oc & 31));
+            if (n < kCaMaxNeighbors) {
+              if (slot == -1) {
+                const uint32_t sl = waveAlloc(&ca.cnt[e].nbLists, true);
+                if (sl + 1 < ca.caps.maxCellLists) {
+                  slot = int32_t(sl);
+                } else {
+                  slot = -2;
+                  status |= kCaNeighborStoreOverflow;
+                }
+              }
+              if (slot >= 0)
+                pool[size_t(slot) * kCaMaxNeighbors + n] = me;
+            } else {
+              status |= kCaNeighborOverflow;
+            }
+            ++n;
+          };
+          auto cuts = [&](const float4& p3) {
+            return alignedRZ(p1.w, p1.z, p2.w, p2.z, p3.w, p3.z, kCaPtMin, thetaCut) &&
+                   dcaCut(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, dcut, kCaHardCurvCut);
+          };
+          if (staged) {
+            for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+              const uint32_t m = min(64u, total - b0);
+              uint64_t acc = 0;
+              for (uint32_t k = 0; k < m; ++k)
+                if (cuts(sPos[w][gBase + b0 + k]))
+                  acc |= 1ull << k;
+              for (; acc; acc &= acc - 1)
+                append(sMe[w][gBase + b0 + __ffsll((unsigned long long)acc) - 1]);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+              // chunks of up to 64 candidates: the cuts first, in a loop without
+              // stores (so the candidates' loads are issued together), then the
+              // accepted ones in ascending order
+              for (uint32_t b0 = 0; b0 < len[q]; b0 += 64) {
+                const uint32_t m = min(64u, len[q] - b0), base = beg[q] + b0;
+                uint64_t acc = 0;
+                for (uint32_t k = 0; k < m; ++k)
+                  if (cuts(H.pos[ca.cellOuter[cb + base + k]]))
+                    acc |= 1ull << k;
+                for (; acc; acc &= acc - 1)
+                  append(base + __ffsll((unsigned long long)acc) - 1);
+              }
+            }
+          }
+          if (slot >= 0) {
+            ca.nbSlot[cb + oc] = slot;
+            ca.nbCount[cb + oc] = uint8_t(min(n, kCaMaxNeighbors));
+            stored += min(n, kCaMaxNeighbors);
+          }
+        }
+        waveLdsSync();
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
@@ -462,12 +580,6 @@ namespace pxg {
       uint32_t det;      // detInd of the inner hit; bit 16: the cell is used
       float x, y, z, n;  // inner hit minus the list's hit, squared norm
     };
-
-    __device__ __forceinline__ void waveLdsSync() {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
 
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
