@@ -308,7 +308,10 @@ def main():
         for key in ("S", "C", "N", "T", "TH", "F", "V"):
             mean[key] = 0.0
     launches_per_kernel = {k: v[0] for k, v in prof.items()}
-    kern = max(prof, key=lambda k: prof[k][1])
+    # the dominant kernel: the most time with no other stream running (durations in the
+    # timed region depend on what the other streams overlap them with)
+    rank = iso if iso else prof
+    kern = max((k for k in rank if k in prof), key=lambda k: rank[k][1])
     calls, total_ms = prof[kern]
     avg_s = total_ms / 1e3 / calls
     events_per_launch = n * args.steps / calls
