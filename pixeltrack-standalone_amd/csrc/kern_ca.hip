@@ -25,6 +25,7 @@ namespace pxg {
       const uint32_t* phiOff;
       const uint16_t* phiBins;
       const float4* pos;
+      const HitAttr* attr;
     };
 
     __device__ __forceinline__ HitView hitsOf(const Pool& p, uint32_t e) {
@@ -39,7 +40,8 @@ namespace pxg {
               p.hitsLayerStart + size_t(e) * kLayerStartStride,
               p.phiOff + size_t(e) * kPhiOffStride,
               p.phiBins + h,
-              p.pos + h};
+              p.pos + h,
+              p.attr + h};
     }
 
     __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
@@ -107,20 +109,22 @@ namespace pxg {
       q.outer = kPairOuter[pairId];
       q.i = j - (pairId ? T.cum[pairId - 1] : 0) + T.layer[q.inner];
       q.live = false;
-      if (H.detInd[q.i] > 2000)
+      const HitAttr a = H.attr[q.i];
+      if (a.detInd > 2000)
         return q;
-      q.mez = H.zg[q.i];
+      const float4 pi = H.pos[q.i];
+      q.mez = pi.z;
       if (q.mez < kPairMinZ[pairId] || q.mez > kPairMaxZ[pairId])
         return q;
       constexpr int minYsizeB1 = 36, minYsizeB2 = 28;
-      q.mes = H.ysize[q.i];  // ideal conditions: every ladder counts as an outer ladder
+      q.mes = a.ysize;  // ideal conditions: every ladder counts as an outer ladder
       if (q.inner == 0 && q.outer > 3 && q.mes > 0 && q.mes < minYsizeB1)
         return q;
       if (q.inner == 1 && q.outer > 3 && q.mes > 0 && q.mes < minYsizeB2)
         return q;
       q.live = true;
-      q.mep = H.iphi[q.i];
-      q.mer = H.rg[q.i];
+      q.mep = a.iphi;
+      q.mer = pi.w;
       q.iphicut = kPairPhiCut[pairId];
       q.maxr = kPairMaxR[pairId];
       const uint32_t hoff = kPhiBins * q.outer;
@@ -148,19 +152,20 @@ namespace pxg {
       constexpr float minRadius = 0.5f * 87.78f;
       constexpr float minRadius2T4 = 4.f * minRadius * minRadius;
       constexpr float i2p = 3.14159265358979323846 / 32768;
-      if (H.detInd[oi] > 2000)
+      const HitAttr a = H.attr[oi];
+      if (a.detInd > 2000)
         return false;
       const float4 po = H.pos[oi];
       const float zo = po.z, ro = po.w;
       const float dr = ro - q.mer;
       if (dr > q.maxr || dr < 0 || fabsf(q.mez * ro - q.mer * zo) > z0cut * dr)
         return false;
-      const int16_t mop = H.iphi[oi];
+      const int16_t mop = a.iphi;
       const int a1 = abs(int(int16_t(mop - q.mep))), a2 = abs(int(int16_t(q.mep - mop)));
       const uint16_t idphi = uint16_t(a1 < a2 ? a1 : a2);
       if (idphi > q.iphicut)
         return false;
-      const int16_t so = H.ysize[oi];
+      const int16_t so = a.ysize;
       const int dy = q.inner == 0 ? maxDYsize12 : maxDYsize;
       const bool zsize = q.outer < 4
                              ? (q.mes > 0 && so > 0 && abs(so - q.mes) > dy)
@@ -174,6 +179,21 @@ namespace pxg {
 
     __device__ __forceinline__ uint32_t candidate(const HitView& H, const SlotScan& q, uint32_t k) {
       return H.phiBins[k < q.n1 ? q.s1 + k : q.s2 + (k - q.n1)];
+    }
+
+    // position of the r-th set bit of w (r < popcount(w))
+    __device__ __forceinline__ uint32_t nthBit(uint64_t w, uint32_t r) {
+      uint32_t pos = 0;
+#pragma unroll
+      for (int width = 32; width >= 1; width >>= 1) {
+        const uint32_t c = __popcll(w & ((1ull << width) - 1));
+        if (r >= c) {
+          r -= c;
+          w >>= width;
+          pos += width;
+        }
+      }
+      return pos;
     }
 
     // G lanes per slot (G divides 64): 64 / G slots per wave
@@ -282,26 +302,35 @@ namespace pxg {
         if (base >= nCells || dOff[j + 1] == base)
           continue;
         const SlotScan q = slotOf(H, T, j);
+        auto write = [&](uint32_t c, uint32_t oi) {
+          ca.cellInner[cb + c] = uint16_t(q.i);
+          ca.cellOuter[cb + c] = uint16_t(oi);
+          ca.cellPair[cb + c] = uint8_t(q.pairId);
+          atomicAdd(&ohc[oi], 1u);
+        };
+        // the first 128 candidates: the count pass recorded their verdicts; the
+        // r-th accepted one is the slot's r-th cell, written by lane r % G
         const uint64_t m0 = dMask[2 * j], m1 = dMask[2 * j + 1];
-        uint32_t run = 0;
-        for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
+        const uint32_t c0 = __popcll(m0), c01 = c0 + __popcll(m1);
+        for (uint32_t r = sub; r < c01; r += G) {
+          const uint32_t k = r < c0 ? nthBit(m0, r) : 64 + nthBit(m1, r - c0);
+          if (base + r < nCells)
+            write(base + r, candidate(H, q, k));
+        }
+        // candidates past the first 128 are tested here, G at a time
+        uint32_t run = c01;
+        for (uint32_t k0 = 128; k0 < q.n; k0 += G) {
           const uint32_t k = k0 + sub;
           uint32_t oi = 0;
           bool ok = false;
           if (k < q.n) {
-            // the count pass recorded the first 128 candidates' verdicts
-            ok = k < 64 ? (m0 >> k) & 1 : k < 128 ? (m1 >> (k - 64)) & 1 : acceptDoublet(H, q, candidate(H, q, k));
-            if (ok)
-              oi = candidate(H, q, k);
+            oi = candidate(H, q, k);
+            ok = acceptDoublet(H, q, oi);
           }
           const uint64_t m = groupBits<G>(__ballot(ok));
           const uint32_t c = base + run + __popcll(m & below);
-          if (ok && c < nCells) {
-            ca.cellInner[cb + c] = uint16_t(q.i);
-            ca.cellOuter[cb + c] = uint16_t(oi);
-            ca.cellPair[cb + c] = uint8_t(q.pairId);
-            atomicAdd(&ohc[oi], 1u);
-          }
+          if (ok && c < nCells)
+            write(c, oi);
           run += __popcll(m);
         }
       }

@@ -259,7 +259,9 @@ namespace pxg {
         const float rg = sqrtf(xg * xg + yg * yg);
         p.rg[o] = rg;
         p.pos[o] = make_float4(xg, yg, zg, rg);
-        p.iphi[o] = atan2s7(yg, xg);
+        const int16_t iphi = atan2s7(yg, xg);
+        p.iphi[o] = iphi;
+        p.attr[o] = HitAttr{iphi, int16_t(sy), uint16_t(me), 0};
       }
       waveSync();
     }

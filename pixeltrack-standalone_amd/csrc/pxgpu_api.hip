@@ -94,6 +94,7 @@ namespace pxg::detail {
     f(p.zg, H);
     f(p.rg, H);
     f(p.pos, H);
+    f(p.attr, H);
     f(p.iphi, H);
     f(p.charge, H);
     f(p.xsize, H);

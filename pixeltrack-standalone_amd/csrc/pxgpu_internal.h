@@ -66,6 +66,11 @@ namespace pxg {
     uint32_t pad[3];
   };
 
+  struct alignas(8) HitAttr {
+    int16_t iphi, ysize;
+    uint16_t detInd, pad;
+  };
+
   // Device pointers of one stream's pool (batch capacity fixed at creation).
   struct Pool {
     uint32_t maxEvents, maxWords;
@@ -88,6 +93,7 @@ namespace pxg {
     // hits, per event (stride kMaxNumClusters)
     float *xl, *yl, *xerr, *yerr, *xg, *yg, *zg, *rg;
     float4* pos;  // (xg, yg, zg, rg) of each hit in one 16-byte load, for the CA kernels' gathers
+    HitAttr* attr;  // iphi, ysize, detInd of each hit in one 8-byte load, for the doublet search
     int16_t* iphi;
     int32_t* charge;
     int16_t *xsize, *ysize;
