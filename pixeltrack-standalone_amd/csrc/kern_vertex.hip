@@ -169,13 +169,26 @@ namespace pxg {
       __syncthreads();
       countingSort(nt, 256, [&](uint32_t i) { return uint32_t(izt[i]); }, true, zbins, L);
       __syncthreads();
+      // the tracks of the neighbouring z bins, in histogram order (four indices
+      // loaded at a time)
       auto forEachNear = [&](uint32_t value, auto&& f) {
         const int bs = max(0, int(value) - 1);
         const int be = min(255, int(value) + 1);
-        for (uint32_t p = voff[bs]; p < voff[be + 1]; ++p)
+        uint32_t p = voff[bs];
+        const uint32_t pe = voff[be + 1];
+        for (; p + 4 <= pe; p += 4) {
+          const uint32_t j0 = zbins[p], j1 = zbins[p + 1], j2 = zbins[p + 2], j3 = zbins[p + 3];
+          f(j0);
+          f(j1);
+          f(j2);
+          f(j3);
+        }
+        for (; p < pe; ++p)
           f(zbins[p]);
       };
-      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+      // tracks taken in z order, so the lanes of a wave mostly scan the same bins
+      for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+        const uint32_t i = zbins[pp];
         if (ezt2[i] > er2mx)
           continue;
         int32_t c = 0;
@@ -192,7 +205,8 @@ namespace pxg {
         nn[i] = c;
       }
       __syncthreads();
-      for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+      for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+        const uint32_t i = zbins[pp];
         float mdist = eps;
         int32_t best = int32_t(i);
         forEachNear(izt[i], [&](uint32_t j) {
@@ -388,12 +402,20 @@ namespace pxg {
         for (uint32_t i = threadIdx.x; i < nt; i += kVB)
           idv[itrk[i]] = int16_t(iv[i]);
         const uint32_t nc = min(nvFinal, kMaxVertices);
+        // the tracks' pt^2 next to the other per-track arrays (tmp is free now)
+        const float* pt2 = ptt2;
+        if (inLds) {
+          for (uint32_t i = threadIdx.x; i < nt; i += kVB)
+            tmp[i] = __float_as_int(ptt2[i]);
+          __syncthreads();
+          pt2 = reinterpret_cast<const float*>(tmp);
+        }
         for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
           float s = 0;
           for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
             const uint32_t i = zbins[p];
             if (iv[i] == int32_t(v))
-              s += ptt2[i];
+              s += pt2[i];
           }
           ptv2[v] = s;
         }
