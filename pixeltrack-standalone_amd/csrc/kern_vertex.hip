@@ -17,6 +17,8 @@ namespace pxg {
     constexpr int kVB = kEventBlock;
     constexpr int kUnassigned = 9999;
     constexpr uint32_t kLdsTracks = 2048;
+    // loadTracks gives each thread at most 64 tuples
+    static_assert(kCapsReference.maxTuples <= 64 * kVB && kCapsHighOccupancy.maxTuples <= 64 * kVB);
 
     struct VtxLds {
       uint32_t counts[kMaxVertices];
@@ -108,31 +110,34 @@ namespace pxg {
       constexpr int minT = 2;
       constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f, ptMin = 0.5f;
 
-      // ---- loadTracks: loose tuples with 4+ hits and pt >= 0.5 GeV, in tuple order
-      if (threadIdx.x == 0)
-        L.nt = 0;
-      __syncthreads();
-      for (uint32_t base = 0; base < nT; base += kVB) {
-        const uint32_t t = base + threadIdx.x;
-        bool take = false;
-        if (t < nT) {
+      // ---- loadTracks: loose tuples with 4+ hits and pt >= 0.5 GeV, in tuple order.
+      // Each thread takes a contiguous run of tuples (at most 64: maxTuples <= 65536),
+      // one scan over the threads orders the taken ones.
+      {
+        const uint32_t R = (nT + kVB - 1) / kVB;
+        const uint32_t t0 = min(threadIdx.x * R, nT), t1 = min(t0 + R, nT);
+        uint64_t takeMask = 0;
+        uint32_t cnt = 0;
+        for (uint32_t t = t0; t < t1; ++t) {
           const uint32_t nh = toff[t + 1] - toff[t];
           idv[t] = nh == 0 ? 0 : -1;  // only the last stored tuple can be empty
-          take = nh >= 4 && q[t] == kQualityLoose && !(ca.pt[tb + t] < ptMin);
+          const bool take = nh >= 4 && q[t] == kQualityLoose && !(ca.pt[tb + t] < ptMin);
+          takeMask |= uint64_t(take) << (t - t0);
+          cnt += take;
         }
         uint32_t tot;
-        const uint32_t ex = wgExclusiveScan<kVB>(take ? 1u : 0u, tot, L.scan);
-        if (take) {
-          const uint32_t k = L.nt + ex;
+        uint32_t k = wgExclusiveScan<kVB>(cnt, tot, L.scan);
+        for (uint64_t m = takeMask; m; m &= m - 1) {
+          const uint32_t t = t0 + __ffsll((unsigned long long)m) - 1;
           const float pt = ca.pt[tb + t];
           itrk[k] = uint16_t(t);
           zt[k] = ca.state[5 * tb + 4 * size_t(M) + t];
           ezt2[k] = ca.cov[15 * tb + 14 * size_t(M) + t];
           ptt2[k] = pt * pt;
+          ++k;
         }
-        __syncthreads();
         if (threadIdx.x == 0)
-          L.nt += tot;
+          L.nt = tot;
         __syncthreads();
       }
       const uint32_t nt = L.nt;
@@ -141,6 +146,11 @@ namespace pxg {
       __shared__ int32_t sIv[kLdsTracks], sTmp[kLdsTracks], sNn[kLdsTracks];
       __shared__ uint32_t sBins[kLdsTracks];
       __shared__ uint8_t sIzt[kLdsTracks];
+      __shared__ float sZs[kLdsTracks], sEs[kLdsTracks];
+      __shared__ int32_t sNnS[kLdsTracks];
+      float* zS = ca.vZs + tb;
+      float* eS = ca.vEs + tb;
+      int32_t* nnS = ca.vNnS + tb;
       const bool inLds = nt <= kLdsTracks;
       if (inLds) {
         for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
@@ -154,6 +164,9 @@ namespace pxg {
         nn = sNn;
         zbins = sBins;
         izt = sIzt;
+        zS = sZs;
+        eS = sEs;
+        nnS = sNnS;
         __syncthreads();
       }
 
@@ -169,60 +182,84 @@ namespace pxg {
       __syncthreads();
       countingSort(nt, 256, [&](uint32_t i) { return uint32_t(izt[i]); }, true, zbins, L);
       __syncthreads();
-      // the tracks of the neighbouring z bins, in histogram order (four indices
-      // loaded at a time)
-      auto forEachNear = [&](uint32_t value, auto&& f) {
-        const int bs = max(0, int(value) - 1);
-        const int be = min(255, int(value) + 1);
-        uint32_t p = voff[bs];
-        const uint32_t pe = voff[be + 1];
-        for (; p + 4 <= pe; p += 4) {
-          const uint32_t j0 = zbins[p], j1 = zbins[p + 1], j2 = zbins[p + 2], j3 = zbins[p + 3];
-          f(j0);
-          f(j1);
-          f(j2);
-          f(j3);
-        }
-        for (; p < pe; ++p)
-          f(zbins[p]);
+      // z-ordered copies of z, its error and (below) the neighbour count: the
+      // density loops read their candidates by z-ordered position, without
+      // going through the track index
+      for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+        const uint32_t i = zbins[pp];
+        zS[pp] = zt[i];
+        eS[pp] = ezt2[i];
+        nnS[pp] = 0;
+      }
+      __syncthreads();
+      // z-ordered positions of the tracks of the neighbouring bins, in histogram order
+      auto nearRange = [&](uint32_t i, uint32_t& pb, uint32_t& pe) {
+        const int bs = max(0, int(izt[i]) - 1);
+        const int be = min(255, int(izt[i]) + 1);
+        pb = voff[bs];
+        pe = voff[be + 1];
       };
       // tracks taken in z order, so the lanes of a wave mostly scan the same bins
       for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
         const uint32_t i = zbins[pp];
-        if (ezt2[i] > er2mx)
+        const float zi = zS[pp], ei = eS[pp];
+        if (ei > er2mx)
           continue;
+        uint32_t pb, pe;
+        nearRange(i, pb, pe);
+        // a neighbour: another track within eps and within the error-scaled cut
+        auto near = [&](uint32_t p, float zj, float ej) -> int32_t {
+          const float dist = fabsf(zi - zj);
+          return p != pp && !(dist > eps) && !(dist * dist > chi2max * (ei + ej));
+        };
         int32_t c = 0;
-        forEachNear(izt[i], [&](uint32_t j) {
-          if (i == j)
-            return;
-          const float dist = fabsf(zt[i] - zt[j]);
-          if (dist > eps)
-            return;
-          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
-            return;
-          ++c;
-        });
+        uint32_t p = pb;
+        for (; p + 4 <= pe; p += 4) {
+          const float z0 = zS[p], z1 = zS[p + 1], z2 = zS[p + 2], z3 = zS[p + 3];
+          const float e0 = eS[p], e1 = eS[p + 1], e2 = eS[p + 2], e3 = eS[p + 3];
+          c += near(p, z0, e0) + near(p + 1, z1, e1) + near(p + 2, z2, e2) + near(p + 3, z3, e3);
+        }
+        for (; p < pe; ++p)
+          c += near(p, zS[p], eS[p]);
         nn[i] = c;
+        nnS[pp] = c;
       }
       __syncthreads();
       for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
         const uint32_t i = zbins[pp];
+        const float zi = zS[pp], ei = eS[pp];
+        const int32_t ni = nnS[pp];
+        uint32_t pb, pe;
+        nearRange(i, pb, pe);
         float mdist = eps;
-        int32_t best = int32_t(i);
-        forEachNear(izt[i], [&](uint32_t j) {
-          if (nn[j] < nn[i])
+        uint32_t bestP = 0xffffffffu;
+        // the closest qualifying track, first one on ties (candidates in order)
+        auto visit = [&](uint32_t p, int32_t nj, float zj, float ej) {
+          if (nj < ni)
             return;
-          if (nn[j] == nn[i] && zt[j] >= zt[i])
+          if (nj == ni && zj >= zi)
             return;
-          const float dist = fabsf(zt[i] - zt[j]);
+          const float dist = fabsf(zi - zj);
           if (dist > mdist)
             return;
-          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+          if (dist * dist > chi2max * (ei + ej))
             return;
           mdist = dist;
-          best = int32_t(j);
-        });
-        iv[i] = best;
+          bestP = p;
+        };
+        uint32_t p = pb;
+        for (; p + 4 <= pe; p += 4) {
+          const int32_t n0 = nnS[p], n1 = nnS[p + 1], n2 = nnS[p + 2], n3 = nnS[p + 3];
+          const float z0 = zS[p], z1 = zS[p + 1], z2 = zS[p + 2], z3 = zS[p + 3];
+          const float e0 = eS[p], e1 = eS[p + 1], e2 = eS[p + 2], e3 = eS[p + 3];
+          visit(p, n0, z0, e0);
+          visit(p + 1, n1, z1, e1);
+          visit(p + 2, n2, z2, e2);
+          visit(p + 3, n3, z3, e3);
+        }
+        for (; p < pe; ++p)
+          visit(p, nnS[p], zS[p], eS[p]);
+        iv[i] = bestP == 0xffffffffu ? int32_t(i) : int32_t(zbins[bestP]);
       }
       __syncthreads();
       // follow the links to the root of each tree
@@ -266,6 +303,20 @@ namespace pxg {
       __syncthreads();
 
       // ---- fitVertices (twice) and splitVertices
+      // the tracks of vertex v in ascending order (four indices loaded at a time)
+      auto forEachOf = [&](uint32_t v, auto&& f) {
+        uint32_t p = voff[v];
+        const uint32_t pe = voff[v + 1];
+        for (; p + 4 <= pe; p += 4) {
+          const uint32_t i0 = zbins[p], i1 = zbins[p + 1], i2 = zbins[p + 2], i3 = zbins[p + 3];
+          f(i0);
+          f(i1);
+          f(i2);
+          f(i3);
+        }
+        for (; p < pe; ++p)
+          f(zbins[p]);
+      };
       auto fit = [&](float chi2Max) {
         const uint32_t nvf = L.nvIntermediate;
         const uint32_t nc = min(nvf, kMaxVertices);
@@ -276,12 +327,11 @@ namespace pxg {
         __syncthreads();
         for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
           float z = 0, w = 0;
-          for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
-            const uint32_t i = zbins[p];
+          forEachOf(v, [&](uint32_t i) {
             const float wi = 1.f / ezt2[i];
             z += zt[i] * wi;
             w += wi;
-          }
+          });
           zv[v] = z / w;
           wv[v] = w;
           nn[v] = -1;
@@ -301,13 +351,12 @@ namespace pxg {
         for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
           float c = 0;
           int32_t k = -1;
-          for (uint32_t p = voff[v]; p < voff[v + 1]; ++p) {
-            const uint32_t i = zbins[p];
+          forEachOf(v, [&](uint32_t i) {
             if (iv[i] != int32_t(v))
-              continue;
+              return;
             c += __int_as_float(tmp[i]);
             ++k;
-          }
+          });
           chi2[v] = c;
           nn[v] = k;
           if (k > 0)
@@ -341,27 +390,25 @@ namespace pxg {
               more = false;
               znew[0] = znew[1] = 0;
               wnew[0] = wnew[1] = 0;
-              for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
-                const uint32_t i = zbins[p];
+              forEachOf(kv, [&](uint32_t i) {
                 if (iv[i] != int32_t(kv))
-                  continue;
+                  return;
                 const float ww = 1.f / ezt2[i];
                 znew[izt[i]] += __int_as_float(tmp[i]) * ww;
                 wnew[izt[i]] += ww;
-              }
+              });
               znew[0] /= wnew[0];
               znew[1] /= wnew[1];
-              for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
-                const uint32_t i = zbins[p];
+              forEachOf(kv, [&](uint32_t i) {
                 if (iv[i] != int32_t(kv))
-                  continue;
+                  return;
                 const float zz = __int_as_float(tmp[i]);
                 const float d0 = fabsf(zz - znew[0]);
                 const float d1 = fabsf(zz - znew[1]);
                 const uint8_t newer = d0 < d1 ? 0 : 1;
                 more |= newer != izt[i];
                 izt[i] = newer;
-              }
+              });
               --maxiter;
               if (maxiter <= 0)
                 more = false;

@@ -156,6 +156,9 @@ namespace pxg::detail {
     f(ca.vPtt2, T);
     f(ca.vIv, T);
     f(ca.vTmp, T);
+    f(ca.vZs, T);
+    f(ca.vEs, T);
+    f(ca.vNnS, T);
     f(ca.vIzt, T);
     f(ca.vTrk, T);
     f(ca.vOff, E * (kMaxVertices + 64));
@@ -257,6 +260,9 @@ namespace pxg::detail {
     v.vPtt2 += T;
     v.vIv += T;
     v.vTmp += T;
+    v.vZs += T;
+    v.vEs += T;
+    v.vNnS += T;
     v.vIzt += T;
     v.vTrk += T;
     v.vOff += E * (kMaxVertices + 64);

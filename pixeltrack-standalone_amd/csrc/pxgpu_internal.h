@@ -202,6 +202,8 @@ namespace pxg {
     uint16_t* vItrk;         // [maxTuples] tuple index of each loaded track
     float *vZt, *vEzt2, *vPtt2;      // [maxTuples]
     int32_t *vIv, *vTmp;     // [maxTuples]
+    float *vZs, *vEs;        // [maxTuples]: z and its error of the tracks in z-bin order
+    int32_t* vNnS;           // [maxTuples]: neighbour counts in z-bin order
     uint8_t* vIzt;           // [maxTuples]
     uint16_t* vBins;         // [maxTuples]: tracks by z bin
     uint32_t* vTrk;          // [maxTuples]: tracks by vertex, ascending within a vertex
