@@ -26,6 +26,7 @@ namespace pxg {
       uint32_t running[kMaxVertices];
       uint32_t scan[kVB / 64];
       uint32_t nvIntermediate, nvFinal, found, nt;
+      uint8_t waveCnt[kVB / 64][kMaxVertices];  // items of a bin in each wave of a batch (<= 64)
     };
 
     // Stable counting sort of items 0..n-1 by key(i) in [0, K) (items with a
@@ -58,7 +59,12 @@ namespace pxg {
         L.start[K] = carry;
       __syncthreads();
       // rank of an item among the earlier items of its bin: within its wave by
-      // shuffles, across waves by visiting the waves in order
+      // shuffles, from the earlier waves of the batch by their per-bin counts,
+      // from the earlier batches by the running count
+      uint32_t* wc = reinterpret_cast<uint32_t*>(&L.waveCnt[0][0]);
+      for (uint32_t w = threadIdx.x; w < (kVB / 64) * kMaxVertices / 4; w += kVB)
+        wc[w] = 0;
+      __syncthreads();
       const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
       for (uint32_t base = 0; base < n; base += kVB) {
         const uint32_t i = base + threadIdx.x;
@@ -69,16 +75,24 @@ namespace pxg {
           same += ks == k;
           below += ks == k && src < lane;
         }
-        for (uint32_t w = 0; w < kVB / 64; ++w) {
-          if (wid == w && i < n && k < K) {
-            const uint32_t rank = L.running[k] + below;
-            const uint32_t pos = descending ? L.start[k] + L.counts[k] - 1 - rank : L.start[k] + rank;
-            list[pos] = i;
-            if (below == same - 1)
-              L.running[k] += same;
-          }
-          __syncthreads();
+        const bool mine = i < n && k < K;
+        const bool last = mine && below == same - 1;  // the wave's last item of the bin
+        if (last)
+          L.waveCnt[wid][k] = uint8_t(same);
+        __syncthreads();
+        if (mine) {
+          uint32_t rank = L.running[k] + below;
+          for (uint32_t w = 0; w < wid; ++w)
+            rank += L.waveCnt[w][k];
+          const uint32_t pos = descending ? L.start[k] + L.counts[k] - 1 - rank : L.start[k] + rank;
+          list[pos] = i;
         }
+        __syncthreads();
+        if (last) {
+          atomicAdd(&L.running[k], same);
+          L.waveCnt[wid][k] = 0;
+        }
+        __syncthreads();
       }
     }
 
@@ -102,10 +116,8 @@ namespace pxg {
       uint8_t* izt = ca.vIzt + tb;
       uint32_t* zbins = ca.vTrk + tb;  // tracks by z bin, then tracks by vertex
       const uint32_t* voff = L.start;  // bin starts of the last counting sort
-      float* zv = ca.zv + size_t(e) * kMaxVertices;
-      float* wv = ca.wv + size_t(e) * kMaxVertices;
-      float* chi2 = ca.vchi2 + size_t(e) * kMaxVertices;
-      float* ptv2 = ca.ptv2 + size_t(e) * kMaxVertices;
+      // the per-vertex results live in LDS and are stored once at the end
+      __shared__ float zv[kMaxVertices], wv[kMaxVertices], chi2[kMaxVertices], ptv2[kMaxVertices];
       uint16_t* sortInd = ca.sortInd + size_t(e) * kMaxVertices;
       constexpr int minT = 2;
       constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f, ptMin = 0.5f;
@@ -373,16 +385,26 @@ namespace pxg {
         for (uint32_t base = 0; base < nvf; base += kVB) {
           const uint32_t kv = base + threadIdx.x;
           bool split = false;
+          // the vertex's assigned tracks in ascending order, compacted into its own
+          // segment [b, b + m) of arrays that are free here (the z-ordered copies
+          // and tmp): offset from the vertex, weight, side, track index
+          const uint32_t b = kv < nvf && kv < kMaxVertices ? voff[kv] : 0;
+          float* const czz = zS;
+          float* const cww = eS;
+          int32_t* const cside = nnS;
+          int32_t* const cidx = tmp;
+          uint32_t m = 0;
           if (kv < nvf && kv < kMaxVertices && nn[kv] >= 4 && !(chi2[kv] < 9.f * float(nn[kv])) && nn[kv] < 512) {
-            // tracks of the vertex in ascending order: zbins[voff[kv], voff[kv+1]) still assigned
-            for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
-              const uint32_t i = zbins[p];
+            forEachOf(kv, [&](uint32_t i) {
               if (iv[i] != int32_t(kv))
-                continue;
+                return;
               const float zz = zt[i] - zv[kv];
-              tmp[i] = __float_as_int(zz);
-              izt[i] = zz < 0 ? 0 : 1;
-            }
+              czz[b + m] = zz;
+              cww[b + m] = 1.f / ezt2[i];
+              cside[b + m] = zz < 0 ? 0 : 1;
+              cidx[b + m] = int32_t(i);
+              ++m;
+            });
             float znew[2], wnew[2];
             int maxiter = 20;
             bool more = true;
@@ -390,25 +412,22 @@ namespace pxg {
               more = false;
               znew[0] = znew[1] = 0;
               wnew[0] = wnew[1] = 0;
-              forEachOf(kv, [&](uint32_t i) {
-                if (iv[i] != int32_t(kv))
-                  return;
-                const float ww = 1.f / ezt2[i];
-                znew[izt[i]] += __int_as_float(tmp[i]) * ww;
-                wnew[izt[i]] += ww;
-              });
+              for (uint32_t u = 0; u < m; ++u) {
+                const int32_t sd = cside[b + u];
+                const float ww = cww[b + u];
+                znew[sd] += czz[b + u] * ww;
+                wnew[sd] += ww;
+              }
               znew[0] /= wnew[0];
               znew[1] /= wnew[1];
-              forEachOf(kv, [&](uint32_t i) {
-                if (iv[i] != int32_t(kv))
-                  return;
-                const float zz = __int_as_float(tmp[i]);
+              for (uint32_t u = 0; u < m; ++u) {
+                const float zz = czz[b + u];
                 const float d0 = fabsf(zz - znew[0]);
                 const float d1 = fabsf(zz - znew[1]);
-                const uint8_t newer = d0 < d1 ? 0 : 1;
-                more |= newer != izt[i];
-                izt[i] = newer;
-              });
+                const int32_t newer = d0 < d1 ? 0 : 1;
+                more |= newer != cside[b + u];
+                cside[b + u] = newer;
+              }
               --maxiter;
               if (maxiter <= 0)
                 more = false;
@@ -424,11 +443,9 @@ namespace pxg {
           if (split) {
             // the new id is >= nvf, so no other vertex of this pass sees the moved tracks
             const int32_t igv = int32_t(L.nvIntermediate + ex);
-            for (uint32_t p = voff[kv]; p < voff[kv + 1]; ++p) {
-              const uint32_t i = zbins[p];
-              if (iv[i] == int32_t(kv) && izt[i] == 1)
-                iv[i] = igv;
-            }
+            for (uint32_t u = 0; u < m; ++u)
+              if (cside[b + u] == 1)
+                iv[cidx[b + u]] = igv;
           }
           __syncthreads();
           if (threadIdx.x == 0)
@@ -473,6 +490,16 @@ namespace pxg {
           for (uint32_t u = 0; u < nc; ++u)
             r += (ptv2[u] < pv) || (ptv2[u] == pv && u < v);
           sortInd[r] = uint16_t(v);
+        }
+        float* gZv = ca.zv + size_t(e) * kMaxVertices;
+        float* gWv = ca.wv + size_t(e) * kMaxVertices;
+        float* gChi2 = ca.vchi2 + size_t(e) * kMaxVertices;
+        float* gPtv2 = ca.ptv2 + size_t(e) * kMaxVertices;
+        for (uint32_t v = threadIdx.x; v < nc; v += kVB) {
+          gZv[v] = zv[v];
+          gWv[v] = wv[v];
+          gChi2[v] = chi2[v];
+          gPtv2[v] = ptv2[v];
         }
       }
     }
