@@ -126,7 +126,8 @@ typedef struct {
   uint8_t* cellPair;
   uint8_t* cellFlags;    /* bit0 listed in its outer hit's list, bit1 killed by the fishbone */
   uint32_t* cellUsed;    /* one bit per cell */
-  uint32_t *outerHitOff, *outerHitCells;   /* cells by outer hit, ascending */
+  uint32_t* outerHitOff;
+  uint32_t* outerHitCells;  /* cells by outer hit in fill order, as u32 pairs (cell, inner hit | outer hit << 16) */
   int32_t* neighborSlot; uint8_t* neighborCount; uint32_t* neighborPool;  /* 36 per slot */
   int32_t* trackSlot; uint32_t* trackCount; uint16_t* trackPool;          /* 48 per slot */
   uint32_t* multOff; uint16_t* multBins;   /* tuple multiplicity */

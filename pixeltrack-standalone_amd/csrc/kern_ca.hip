@@ -366,9 +366,9 @@ namespace pxg {
       uint32_t* cur = ca.ohCount + size_t(e) * kHitStride;
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
-        const uint32_t h = ca.cellOuter[cb + c];
+        const uint32_t h = ca.cellOuter[cb + c], hi = ca.cellInner[cb + c];
         const uint32_t pos = atomicAdd(&cur[h], 1u);
-        ca.ohCells[cb + off[h] + pos] = c;
+        ca.ohCells[cb + off[h] + pos] = make_uint2(c, hi | h << 16);
         ca.cellFlags[cb + c] = 0;
         ca.nbSlot[cb + c] = -1;
         ca.nbCount[cb + c] = 0;
@@ -379,12 +379,12 @@ namespace pxg {
 
     // rank of cell c in the outer-hit list of h (ascending cell order); only
     // needed when the list is longer than the 128 cells the serial container keeps
-    __device__ __forceinline__ bool listedInOuterHit(const uint32_t* cells, uint32_t b, uint32_t n, uint32_t c) {
+    __device__ __forceinline__ bool listedInOuterHit(const uint2* cells, uint32_t b, uint32_t n, uint32_t c) {
       if (n <= kCaMaxCellsPerHit)
         return true;
       uint32_t rank = 0;
       for (uint32_t k = 0; k < n; ++k)
-        rank += cells[b + k] < c;
+        rank += cells[b + k].x < c;
       return rank < kCaMaxCellsPerHit;
     }
 
@@ -455,9 +455,10 @@ namespace pxg {
         uint32_t beg[6], len[6];
         uint32_t total = 0, nseg = 0;
         if (valid) {
-          oc = ca.ohCells[cb + pos];
-          hi = ca.cellInner[cb + oc];
-          hm = ca.cellOuter[cb + oc];
+          const uint2 rec = ca.ohCells[cb + pos];
+          oc = rec.x;
+          hi = rec.y & 0xffff;
+          hm = rec.y >> 16;
           const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
           if (ln > kCaMaxCellsPerHit)
             status |= kCaOuterHitOverflow;
@@ -467,7 +468,10 @@ namespace pxg {
           thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
           dcut = H.detInd[hi] < 96 ? kCaDcaCutInner : kCaDcaCutOuter;
         }
-        const int lo = valid ? kPairOuter[ca.cellPair[cb + oc]] : 0;
+        // the outer layer of the cell's pair: the layer of its outer hit
+        int lo = 0;
+        for (int l = 1; l < 10; ++l)
+          lo += valid && hm >= T.layer[l];
         bool more = valid;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -622,7 +626,7 @@ namespace pxg {
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
-      const uint32_t* cells = ca.ohCells + cb;
+      const uint2* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
       auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
       const uint32_t lane = laneId(), wbase = threadIdx.x - lane;
@@ -632,8 +636,9 @@ namespace pxg {
         uint32_t b = 0, n = 0;
         float4 ph{0.f, 0.f, 0.f, 0.f};
         if (pos < nCells) {
-          me.c = cells[pos];
-          const uint32_t h = ca.cellOuter[cb + me.c], hi = ca.cellInner[cb + me.c];
+          const uint2 rec = cells[pos];
+          me.c = rec.x;
+          const uint32_t h = rec.y >> 16, hi = rec.y & 0xffff;
           b = off[h];
           n = off[h + 1] - b;
           ph = H.pos[h];
@@ -655,18 +660,19 @@ namespace pxg {
           if (n > kCaMaxCellsPerHit) {
             listed = listedInOuterHit(cells, b, n, ci);
             for (uint32_t k = 0; listed && k < n; ++k) {
-              const uint32_t c = cells[b + k];
+              const uint32_t c = cells[b + k].x;
               uint32_t rank = 0;
               for (uint32_t m = 0; m < n; ++m)
-                rank += cells[b + m] < c;
+                rank += cells[b + m].x < c;
               if (rank == kCaMaxCellsPerHit)
                 limit = c;
             }
           }
           auto fromMemory = [&](uint32_t k) {
             RowCell d;
-            d.c = cells[b + k];
-            const uint32_t hj = ca.cellInner[cb + d.c];
+            const uint2 rec = cells[b + k];
+            d.c = rec.x;
+            const uint32_t hj = rec.y & 0xffff;
             const float4 pj = H.pos[hj];
             d.x = pj.x - ph.x;
             d.y = pj.y - ph.y;

@@ -646,7 +646,7 @@ int pxg_event_ca_internals(pxg_stream* s, uint32_t e, pxg_ca_internals* o) {
   o->cellFlags = ca.cellFlags;
   o->cellUsed = ca.cellUsed;
   o->outerHitOff = ca.ohOff;
-  o->outerHitCells = ca.ohCells;
+  o->outerHitCells = reinterpret_cast<uint32_t*>(ca.ohCells);
   o->neighborSlot = ca.nbSlot;
   o->neighborCount = ca.nbCount;
   o->neighborPool = ca.nbPool;

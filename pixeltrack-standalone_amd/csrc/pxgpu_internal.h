@@ -170,7 +170,7 @@ namespace pxg {
     uint32_t* cellUsed;      // bit per cell [maxDoublets / 32 + 64]
     uint32_t* ohCount;       // [kHitStride] cells per outer hit, then fill cursors
     uint32_t* ohOff;         // [kHitStride]
-    uint32_t* ohCells;       // [maxDoublets] cells by outer hit (fill order)
+    uint2* ohCells;          // [maxDoublets] cells by outer hit (fill order): x cell, y inner hit | outer hit << 16
     uint32_t* startCells;    // [maxDoublets] start cells of the ntuplet walk, ascending
     int32_t* nbSlot;         // [maxDoublets] neighbour list slot (-1 none, -2 store full)
     uint8_t* nbCount;        // [maxDoublets] neighbours stored (<= 36)

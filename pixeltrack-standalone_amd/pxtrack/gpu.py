@@ -291,7 +291,7 @@ class Stream:
         bits = np.unpackbits(used.view(np.uint8), bitorder="little")[:nc] if nc else np.zeros(0, np.uint8)
         out["cellUsed"] = bits.astype(np.uint16)
         ohoff = self._copy(ci.outerHitOff, np.uint32, n_hits + 1)
-        ohc = self._copy(ci.outerHitCells, np.uint32, int(ohoff[-1]) if n_hits else 0)
+        ohc = self._copy(ci.outerHitCells, np.uint32, 2 * int(ohoff[-1]) if n_hits else 0)[0::2]
         # the device keeps each hit's cells in fill order; the serial container holds the
         # first 128 in ascending order
         lists = [np.sort(ohc[ohoff[h]:ohoff[h + 1]])[:128] for h in range(n_hits)]

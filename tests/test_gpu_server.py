@@ -60,7 +60,8 @@ def test_server_budget_limits_batch_words(detector, gpu_ctx, gpu_stream):
     from pxtrack import gpu
     raws = [event(detector, "pu50", 60 + i)[0] for i in range(6)]
     inputs = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in raws]
-    sv = gpu.Server(gpu_ctx, max_batch=4, n_streams=1, hbm_budget=int(0.22e9))
+    # room for 4 events but the words of only 2 full-size ones
+    sv = gpu.Server(gpu_ctx, max_batch=4, n_streams=1, hbm_budget=gpu.pool_bytes(4, 2 * 300000))
     cap = sv.info()["max_batch_words"]
     assert cap < 4 * 300000
     ids = {sv.submit(*x): i for i, x in enumerate(inputs)}
