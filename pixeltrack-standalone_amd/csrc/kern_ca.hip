@@ -928,16 +928,22 @@ namespace pxg {
             off[i] = cnt.apcN;
         }
       }
+      // slots are handed out compactly: slot s records its cell and the cell's
+      // tuple count (the walk's scratch is free now), so the later passes visit
+      // only the cells that are in tuples
+      uint32_t* trCells = ca.dfsT + cb;
+      uint32_t* trTotal = ca.dfsH + cb;
       uint32_t status = 0;
       for (uint32_t c = gridStart(); c < cnt.nCells; c += gridStride()) {
         const uint32_t n = ca.trCount[cb + c];
-        ca.dfsT[cb + c] = n;  // tracks of the cell (all of them)
         ca.trCount[cb + c] = 0;
         const uint32_t s = waveAlloc(&ca.cnt[e].trLists, n > 0);
         if (n == 0)
           continue;
         if (s + 1 < ca.caps.maxCellLists) {
           ca.trSlot[cb + c] = int32_t(s);
+          trCells[s] = c;
+          trTotal[s] = n;
         } else {
           ca.trSlot[cb + c] = -2;
           status |= kCaTrackStoreOverflow;
@@ -959,8 +965,10 @@ namespace pxg {
           if (c == 0xffffffffu)
             break;
           const int32_t slot = ca.trSlot[cb + c];
+          if (slot < 0)
+            continue;  // no list: the store is full
           const uint32_t pos = atomicAdd(&ca.trCount[cb + c], 1u);
-          if (slot >= 0 && pos < kCaMaxCellTracks)
+          if (pos < kCaMaxCellTracks)
             pool[size_t(slot) * kCaMaxCellTracks + pos] = uint16_t(t);
         }
       }
@@ -971,20 +979,17 @@ namespace pxg {
     // on it: order independent, it only marks shorter tuples as duplicates.
     __global__ void __launch_bounds__(kCaBlock) trackListSortKernel(CAPool ca) {
       const uint32_t e = logicalBlock().y;
-      const uint32_t nCells = ca.cnt[e].nCells;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
       uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
+      const uint32_t* trCells = ca.dfsT + cb;
+      const uint32_t* trTotal = ca.dfsH + cb;
+      const uint32_t nLists = min(ca.cnt[e].trLists, ca.caps.maxCellLists - 1);
       uint32_t status = 0;
-      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
-        const uint32_t total = ca.dfsT[cb + c];
-        const int32_t slot = ca.trSlot[cb + c];
-        if (total == 0 || slot < 0) {
-          ca.trCount[cb + c] = 0;
-          continue;
-        }
+      for (uint32_t slot = gridStart(); slot < nLists; slot += gridStride()) {
+        const uint32_t c = trCells[slot], total = trTotal[slot];
         uint16_t* list = pool + size_t(slot) * kCaMaxCellTracks;
         uint32_t n = total;
         if (total > kCaMaxCellTracks) {
