@@ -32,7 +32,7 @@ namespace pxg {
     constexpr int kWaves = kBlock / 64;
     constexpr uint32_t kNoPixel = 0xffffffffu;
     constexpr int kCols = kColsPerModule;  // 416
-    constexpr int kBigWorkgroups = 512;  // persistent workgroups for the large module runs
+    constexpr int kBigWorkgroups = 768;  // persistent workgroups for the large module runs (3 per CU by LDS)
 
     // exclusive workgroup scan of one 32-bit value per thread; returns the total
     __device__ __forceinline__ uint32_t scanExclusive(uint32_t v, uint32_t& excl, uint32_t* wsum) {
@@ -333,13 +333,14 @@ namespace pxg {
       adc = p.adc[g];
       valid = p.moduleInd[g] != kInvId && x < kRowsPerModule && y < kColsPerModule;
     }
-    // 8-neighbourhood as a lane mask (duplicated pixels are neighbours too)
+    // 8-neighbourhood as a lane mask (duplicated pixels are neighbours too); a
+    // valid pixel is x | y << 16 with x < 160, y < 416, an invalid one ~0
     uint64_t nb = 0;
-    const int vflag = valid ? 1 : 0;
+    const uint32_t xy = valid ? x | y << 16 : 0xffffffffu;
     for (uint32_t j = 0; j < n; ++j) {
-      const int xj = __shfl(int(x), int(j)), yj = __shfl(int(y), int(j)), vj = __shfl(vflag, int(j));
-      const int dx = xj - int(x), dy = yj - int(y);
-      if (valid && vj && j != lane && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
+      const uint32_t xyj = __shfl(xy, int(j));
+      const int dx = int(xyj & 0xffff) - int(x), dy = int(xyj >> 16) - int(y);
+      if (valid && xyj != 0xffffffffu && j != lane && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
         nb |= 1ull << j;
     }
     // min-label propagation with pointer jumping; every label is a lane of the same component
