@@ -365,15 +365,34 @@ namespace pxg {
       const size_t cb = ca.cellBase(e);
       uint32_t* cur = ca.ohCount + size_t(e) * kHitStride;
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
-      for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
-        const uint32_t h = ca.cellOuter[cb + c], hi = ca.cellInner[cb + c];
-        const uint32_t pos = atomicAdd(&cur[h], 1u);
-        ca.ohCells[cb + off[h] + pos] = make_uint2(c, hi | h << 16);
-        ca.cellFlags[cb + c] = 0;
-        ca.nbSlot[cb + c] = -1;
-        ca.nbCount[cb + c] = 0;
-        ca.trSlot[cb + c] = -1;
-        ca.trCount[cb + c] = 0;
+      // four cells per iteration: their atomics are issued together
+      const uint32_t stride = gridStride();
+      for (uint32_t c0 = gridStart(); c0 < nCells; c0 += 4 * stride) {
+        uint32_t h[4], hi[4], pos[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t c = c0 + u * stride;
+          if (c < nCells) {
+            h[u] = ca.cellOuter[cb + c];
+            hi[u] = ca.cellInner[cb + c];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (c0 + u * stride < nCells)
+            pos[u] = atomicAdd(&cur[h[u]], 1u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t c = c0 + u * stride;
+          if (c < nCells) {
+            ca.ohCells[cb + off[h[u]] + pos[u]] = make_uint2(c, hi[u] | h[u] << 16);
+            ca.cellFlags[cb + c] = 0;
+            ca.nbSlot[cb + c] = -1;
+            ca.nbCount[cb + c] = 0;
+            ca.trSlot[cb + c] = -1;
+            ca.trCount[cb + c] = 0;
+          }
+        }
       }
     }
 
