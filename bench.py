@@ -70,6 +70,9 @@ def algorithmic_bytes(kernel, c):
         # detInd (22 B) and its 36 B local frame, 92 B of results out
         "fit": F * (4 + 2 * 4 + 4 * (22 + 36) + 92),
         "classify": T * (4 + 1 + 4 * 5 + 12 + 1),
+        # dupPreds: per cell with two or more tuples (MU) its list (2 B per tuple, ~3), the
+        # tuples' cells (16 B each) and those cells' counts (4 B, ~3 per tuple), 33 B out
+        "dupPreds": c.get("MU", 0.0) * (4 + 4 + 3 * (2 + 16 + 12) + 33),
         "fastDuplicate": 8 * C + 2 * TH + 4 * T + T,
         "hitToTupleCount": T * 5 + 2 * TH + 4 * H,
         "hitToTupleScan": 12 * H,
@@ -301,11 +304,11 @@ def main():
     if args.upto >= 4:
         cac = [st.ca_counts(e) for st, e in where]
         for key, name in (("S", "nSlots"), ("C", "nCells"), ("N", "nNeighbors"), ("T", "nTuples"), ("TH", "apcN"),
-                          ("F", "nMult"), ("V", "vtxTracks")):
+                          ("F", "nMult"), ("V", "vtxTracks"), ("MU", "nMulti")):
             mean[key] = float(np.mean([c[name] for c in cac]))
         mean["vertices"] = float(np.mean([c["nVertices"] for c in cac]))
     else:
-        for key in ("S", "C", "N", "T", "TH", "F", "V"):
+        for key in ("S", "C", "N", "T", "TH", "F", "V", "MU"):
             mean[key] = 0.0
     launches_per_kernel = {k: v[0] for k, v in prof.items()}
     # the dominant kernel: the most time with no other stream running (durations in the

@@ -51,59 +51,196 @@ namespace pxg {
       }
     }
 
-    // a21 kernel_fastDuplicateRemover, one workgroup per event
-    __global__ void __launch_bounds__(kEventBlock) fastDuplicateKernel(CAPool ca) {
-      const uint32_t e = blockIdx.x;
+    // a21 kernel_fastDuplicateRemover.  The serial loop visits the cells with two
+    // or more tuples in ascending order, keeps the loose tuple with the smallest
+    // |tip| of each and marks the other (not bad) tuples duplicates.  A cell's
+    // outcome depends only on the lower cells sharing one of its tuples (its
+    // predecessors), so cells run in rounds: a cell is processed once all its
+    // predecessors have been, and the cells of one round share no tuple.
+
+    // the predecessors of cell c: lower cells with two or more tuples in its tuples
+    template <typename F>
+    __device__ __forceinline__ void forPreds(const CAPool& ca, size_t cb, size_t tb, const uint16_t* pool, uint32_t c,
+                                             F&& f) {
+      const uint16_t* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
+      const uint32_t n = ca.trCount[cb + c];
+      for (uint32_t a = 0; a < n; ++a) {
+        const uint32_t* cells = ca.tupleCells + 4 * tb + 4 * size_t(list[a]);
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const uint32_t o = cells[k2];
+          if (o == 0xffffffffu)
+            break;
+          if (o < c && ca.trCount[cb + o] >= 2 && f(o))
+            return;
+        }
+      }
+    }
+
+    // keep the loose tuple of cell c with the smallest |tip|, the others are duplicates
+    __device__ __forceinline__ void dupProcess(const CAPool& ca, size_t cb, const uint16_t* pool, uint8_t* q,
+                                               const float* tip, uint32_t c) {
+      const uint16_t* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
+      const uint32_t n = ca.trCount[cb + c];
+      float mc = 10000.f;
+      uint16_t im = 60000;
+      for (uint32_t a = 0; a < n; ++a) {
+        const uint16_t it = list[a];
+        const float score = fabsf(tip[it]);
+        if (q[it] == kQualityLoose && score < mc) {
+          mc = score;
+          im = it;
+        }
+      }
+      for (uint32_t a = 0; a < n; ++a) {
+        const uint16_t it = list[a];
+        if (q[it] != kQualityBad && it != im)
+          q[it] = kQualityDup;
+      }
+    }
+
+    // the first kDupPredCap predecessors of every cell with two or more tuples
+    // (all slots written, unused ones ~0), on the whole grid
+    __global__ void __launch_bounds__(kCaBlock) dupPredsKernel(CAPool ca) {
+      const uint32_t e = logicalBlock().y;
       const uint32_t nMulti = ca.cnt[e].nMulti;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
+      const size_t lb = size_t(e) * ca.caps.maxCellLists;
+      const uint32_t* multi = ca.multiCells + cb;
+      const uint16_t* pool = ca.trPool + lb * kCaMaxCellTracks;
+      uint32_t* preds = ca.dupPreds + lb * kDupPredCap;
+      for (uint32_t k = gridStart(); k < nMulti; k += gridStride()) {
+        uint32_t m = 0;
+        uint32_t keep[kDupPredCap];
+#pragma unroll
+        for (uint32_t j = 0; j < kDupPredCap; ++j)
+          keep[j] = 0xffffffffu;
+        bool over = false;
+        forPreds(ca, cb, tb, pool, multi[k], [&](uint32_t o) {
+          bool seen = false;
+#pragma unroll
+          for (uint32_t j = 0; j < kDupPredCap; ++j)
+            seen |= keep[j] == o;
+          if (seen)
+            return false;
+          if (m == kDupPredCap) {
+            over = true;  // more distinct predecessors than kept: walked when needed
+            return true;
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kDupPredCap; ++j)
+            if (j == m)
+              keep[j] = o;
+          ++m;
+          return false;
+        });
+#pragma unroll
+        for (uint32_t j = 0; j < kDupPredCap; ++j)
+          preds[size_t(k) * kDupPredCap + j] = keep[j];
+        ca.dupCount[lb + k] = over ? uint8_t(255) : uint8_t(m);
+      }
+    }
+
+    constexpr uint32_t kDupBitmapCells = kCapsReference.maxDoublets;  // done bits of the cells in LDS
+
+    __global__ void __launch_bounds__(kEventBlock) fastDuplicateKernel(CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      const uint32_t nMulti = ca.cnt[e].nMulti;
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      const size_t lb = size_t(e) * ca.caps.maxCellLists;
       const uint32_t M = ca.caps.maxTuples;
       const uint32_t* multi = ca.multiCells + cb;
-      const uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      const uint16_t* pool = ca.trPool + lb * kCaMaxCellTracks;
+      const uint32_t* preds = ca.dupPreds + lb * kDupPredCap;
+      const uint8_t* count = ca.dupCount + lb;
       uint8_t* q = ca.quality + tb;
       const float* tip = ca.state + 5 * tb + 1 * size_t(M);
-      // bit2 of the cell flags: processed.  The lowest pending cell is always
-      // ready, so every round makes progress; nMulti + 1 rounds bound the loop.
+
+      // ---- done bits of all cells in LDS; each thread keeps its (at most kDupPer)
+      // cells with their predecessors from dupPredsKernel in registers
+      constexpr uint32_t kDupPer = 6;
+      __shared__ uint32_t sDone[kDupBitmapCells / 32];
+      bool lds = nCells <= kDupBitmapCells && nMulti <= kDupPer * kEventBlock;
+      uint32_t myC[kDupPer], myN[kDupPer], myP[kDupPer][kDupPredCap];
+      if (lds) {
+#pragma unroll
+        for (uint32_t i = 0; i < kDupPer; ++i) {
+          const uint32_t k = threadIdx.x + i * kEventBlock;
+          myN[i] = 0;
+          myC[i] = 0;
+          if (k < nMulti) {
+            myC[i] = multi[k];
+            myN[i] = count[k];
+#pragma unroll
+            for (uint32_t j = 0; j < kDupPredCap; ++j)
+              myP[i][j] = preds[size_t(k) * kDupPredCap + j];
+          }
+        }
+        for (uint32_t w = threadIdx.x; w < (nCells + 31) / 32; w += kEventBlock)
+          sDone[w] = 0;
+        __syncthreads();
+      }
+      if (lds) {
+        auto done = [&](uint32_t c) { return (sDone[c >> 5] >> (c & 31)) & 1; };
+        uint32_t mine = 0;  // bit i: cell i of this thread still pending
+#pragma unroll
+        for (uint32_t i = 0; i < kDupPer; ++i)
+          mine |= uint32_t(threadIdx.x + i * kEventBlock < nMulti) << i;
+        // the lowest pending cell is always ready: nMulti + 1 rounds bound the loop
+        for (uint32_t round = 0; round <= nMulti; ++round) {
+          uint32_t now = 0;  // processed in this round
+#pragma unroll
+          for (uint32_t i = 0; i < kDupPer; ++i) {
+            if (!((mine >> i) & 1))
+              continue;
+            bool ready = true;
+            if (myN[i] <= kDupPredCap) {
+#pragma unroll
+              for (uint32_t j = 0; j < kDupPredCap; ++j)
+                ready = ready && (j >= myN[i] || done(myP[i][j]));
+            } else {  // too many predecessors to keep: walk them
+              forPreds(ca, cb, tb, pool, myC[i], [&](uint32_t o) {
+                ready = done(o);
+                return !ready;
+              });
+            }
+            if (!ready)
+              continue;
+            dupProcess(ca, cb, pool, q, tip, myC[i]);
+            now |= 1u << i;
+          }
+          __syncthreads();
+#pragma unroll
+          for (uint32_t i = 0; i < kDupPer; ++i)
+            if ((now >> i) & 1)
+              atomicOr(&sDone[myC[i] >> 5], 1u << (myC[i] & 31));
+          mine &= ~now;
+          if (!__syncthreads_or(mine != 0))
+            break;
+        }
+        return;
+      }
+
+      // ---- otherwise the same rounds over per-cell flags in memory (bit2
+      // processed, bit3 processed in this round) and the full predecessor walk
       for (uint32_t round = 0; round <= nMulti; ++round) {
         int pending = 0;
         for (uint32_t k = threadIdx.x; k < nMulti; k += kEventBlock) {
           const uint32_t c = multi[k];
           if (ca.cellFlags[cb + c] & 4)
             continue;
-          const uint16_t* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
-          const uint32_t n = ca.trCount[cb + c];
           bool ready = true;
-          for (uint32_t a = 0; a < n && ready; ++a) {
-            const uint32_t* cells = ca.tupleCells + 4 * tb + 4 * size_t(list[a]);
-            for (int k2 = 0; k2 < 4; ++k2) {
-              const uint32_t o = cells[k2];
-              if (o == 0xffffffffu)
-                break;
-              if (o < c && ca.trCount[cb + o] >= 2 && !(ca.cellFlags[cb + o] & 4)) {
-                ready = false;
-                break;
-              }
-            }
-          }
+          forPreds(ca, cb, tb, pool, c, [&](uint32_t o) {
+            ready = (ca.cellFlags[cb + o] & 4) != 0;
+            return !ready;
+          });
           if (!ready) {
             pending = 1;
             continue;
           }
-          float mc = 10000.f;
-          uint16_t im = 60000;
-          for (uint32_t a = 0; a < n; ++a) {
-            const uint16_t it = list[a];
-            const float score = fabsf(tip[it]);
-            if (q[it] == kQualityLoose && score < mc) {
-              mc = score;
-              im = it;
-            }
-          }
-          for (uint32_t a = 0; a < n; ++a) {
-            const uint16_t it = list[a];
-            if (q[it] != kQualityBad && it != im)
-              q[it] = kQualityDup;
-          }
+          dupProcess(ca, cb, pool, q, tip, c);
           ca.cellFlags[cb + c] |= 8;  // done in this round
         }
         __syncthreads();
@@ -237,6 +374,9 @@ namespace pxg {
   void launchClassify(const CALaunch& l) {
     l.t->b("classify");
     classifyKernel<<<dim3(l.blocks, l.nEvents), kCaBlock, 0, l.s>>>(*l.ca);
+    l.t->e();
+    l.t->b("dupPreds");
+    dupPredsKernel<<<dim3(l.blocks, l.nEvents), kCaBlock, 0, l.s>>>(*l.ca);
     l.t->e();
     l.t->b("fastDuplicate");
     fastDuplicateKernel<<<l.nEvents, kEventBlock, 0, l.s>>>(*l.ca);

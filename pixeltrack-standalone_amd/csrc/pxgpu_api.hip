@@ -128,6 +128,8 @@ namespace pxg::detail {
     f(ca.trCount, D);
     f(ca.trPool, L * kCaMaxCellTracks);
     f(ca.multiCells, D);
+    f(ca.dupPreds, L * kDupPredCap);
+    f(ca.dupCount, L);
     f(ca.tupleOff, E * (ca.caps.maxTuples + 64));
     f(ca.tupleBins, 5 * T);
     f(ca.tupleCells, 4 * T);
@@ -232,6 +234,8 @@ namespace pxg::detail {
     v.trCount += D;
     v.trPool += L * kCaMaxCellTracks;
     v.multiCells += D;
+    v.dupPreds += L * kDupPredCap;
+    v.dupCount += L;
     v.tupleOff += c.offBase(first);
     v.tupleBins += 5 * T;
     v.tupleCells += 4 * T;

@@ -113,6 +113,7 @@ namespace pxg {
   constexpr uint32_t kHitStride = kMaxNumClusters + 64;      // per-event stride of per-hit offset arrays
   constexpr uint32_t kCaBlock = 256;                         // workgroup of the grid-stride CA kernels
   constexpr uint32_t kEventBlock = 1024;                     // workgroup of the per-event kernels
+  constexpr uint32_t kDupPredCap = 6;                        // predecessors kept per cell for the duplicate remover
 
   // ---- XCD-aware workgroup order -------------------------------------------------------
   // The dispatcher hands workgroups to the chip's 8 XCDs round-robin by linear
@@ -181,6 +182,8 @@ namespace pxg {
     uint32_t* trCount;       // [maxDoublets] tracks stored (<= 48)
     uint16_t* trPool;        // [maxCellLists * 48]
     uint32_t* multiCells;    // [maxDoublets] cells with two or more tracks (any order)
+    uint32_t* dupPreds;      // [maxCellLists * 8] per multiCells entry: its lower cells sharing a tuple (first 8)
+    uint8_t* dupCount;       // [maxCellLists] their number, 255 if more than 8
     uint32_t* tupleOff;      // [maxTuples + 64]
     uint16_t* tupleBins;     // [5 * maxTuples]
     uint32_t* tupleCells;    // [4 * maxTuples] cells of a tuple (0xffffffff past the end)
