@@ -273,6 +273,14 @@ int pxg_server_info(pxg_server* sv, uint32_t* max_batch_words, uint64_t* batches
  * (layer pair, inner hit) slot of the doublet search together (power of two, 1..64), and
  * the number of workgroups of a grid-stride launch, shared by the events of the launch. */
 int pxg_set_tuning(pxg_stream* s, uint32_t doublet_lanes, uint32_t grid_target);
+
+/* Vertex clustering algorithm of the stream's vertex finder: the switches of the reference's
+ * gpuVertexFinder::Producer (plugin-PixelVertexFinding/gpuVertexFinder.h:46-60, dispatched in
+ * gpuVertexFinderImpl.h:96-102); PixelVertexProducerCUDA.cc:26-31 configures density, the default. */
+#define PXG_VERTEX_DENSITY 0
+#define PXG_VERTEX_DBSCAN 1
+#define PXG_VERTEX_ITERATIVE 2
+int pxg_set_vertex_algorithm(pxg_stream* s, int algo);
 int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
 /* Times the chain on resident events [first, first+count) for each launch shape of a fixed
  * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */

@@ -44,6 +44,10 @@ def _load(path, prefix):
             lib.orc_invert_spd.argtypes = [vp, vp, C.c_int]
             lib.orc_vertices.restype = C.c_int
             lib.orc_vertices.argtypes = [C.c_uint32] + [vp] * 10
+            lib.orc_vertices_algo.restype = C.c_int
+            lib.orc_vertices_algo.argtypes = [C.c_int, C.c_uint32] + [vp] * 10
+            lib.orc_conditions_set_vertex_algo.restype = C.c_int
+            lib.orc_conditions_set_vertex_algo.argtypes = [vp, C.c_int]
             lib.orc_circle.argtypes = [C.c_float] * 6 + [C.POINTER(C.c_float)] * 2
             lib.orc_run_many.restype = C.c_uint32
             lib.orc_run_many.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_uint32]
@@ -129,10 +133,15 @@ class Oracle(_Chain):
     prefix = "orc"
     path = ORACLE_LIB
 
-    def __init__(self, conditions, profile=0):
+    # vertex clustering algorithms (gpuVertexFinder.h:46-60)
+    VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2
+
+    def __init__(self, conditions, profile=0, vertex_algo=0):
         super().__init__(conditions)
         if self.lib.orc_conditions_set_profile(self._h, profile) != 0:
             raise ValueError(f"unknown capacity profile {profile}")
+        if self.lib.orc_conditions_set_vertex_algo(self._h, vertex_algo) != 0:
+            raise ValueError(f"unknown vertex algorithm {vertex_algo}")
 
     def fit_helix(self, hits, ge, B):
         """Broken-line fit of one track: hits (3, n) global xyz in cm (double), ge (6, n) global
@@ -155,8 +164,9 @@ class Oracle(_Chain):
         self.lib.orc_invert_spd(m.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), m.shape[0])
         return out
 
-    def vertices(self, zt, ezt2, ptt2):
-        """Vertex finder on given tracks; returns dict of per-track vertex ids and per-vertex arrays."""
+    def vertices(self, zt, ezt2, ptt2, algo=0):
+        """Vertex finder on given tracks (algo: 0 density, 1 DBSCAN, 2 iterative); returns dict of
+        per-track vertex ids and per-vertex arrays."""
         zt, ezt2, ptt2 = (np.ascontiguousarray(a, dtype=np.float32) for a in (zt, ezt2, ptt2))
         nt = zt.size
         iv = np.zeros(nt, np.int16)
@@ -164,8 +174,10 @@ class Oracle(_Chain):
         ndof = np.zeros(1024, np.int32)
         srt = np.zeros(1024, np.uint16)
         a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
-        nv = self.lib.orc_vertices(nt, a(zt), a(ezt2), a(ptt2), a(iv), a(out["zv"]), a(out["wv"]), a(out["chi2"]),
-                                   a(out["ptv2"]), a(ndof), a(srt))
+        nv = self.lib.orc_vertices_algo(algo, nt, a(zt), a(ezt2), a(ptt2), a(iv), a(out["zv"]), a(out["wv"]),
+                                        a(out["chi2"]), a(out["ptv2"]), a(ndof), a(srt))
+        if nv < 0:
+            raise ValueError(f"unknown vertex algorithm {algo}")
         res = {k: v[:nv] for k, v in out.items()}
         res.update(iv=iv, ndof=ndof[:nv], sortInd=srt[:nv], nv=nv)
         return res

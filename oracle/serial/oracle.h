@@ -43,6 +43,7 @@ namespace orc {
     const uint32_t* fedIds;
     uint32_t nFeds;
     CACapacities caps = kCapsReference;
+    int vertexAlgo = 0;  // 0 density, 1 DBSCAN, 2 iterative (gpuVertexFinder.h:46-60)
   };
 
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
@@ -130,6 +131,7 @@ namespace orc {
     AverageGeometry avgCorrected;
     // a11-a22: CA ntuplets and fits; a23-a25: vertices
     CACapacities caps = kCapsReference;
+    int vertexAlgo = 0;
     CAState ca;
     VertexState vtx;
   };
@@ -168,9 +170,9 @@ namespace orc {
   // a23-a25 (orc_stageD.cc)
   void findVertices(Event& ev);
   void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
-                     const std::vector<float>& ezt2, const std::vector<float>& ptt2);
+                     const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo = 0);
   int verticesDirect(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv,
-                     float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd);
+                     float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd, int algo = 0);
 
   // runs the chain up to and including the given stage: 1 = digis, 2 = clusters,
   // 3 = rec-hits, 4 = ntuplets + fits, 5 = vertices

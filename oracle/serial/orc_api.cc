@@ -137,6 +137,14 @@ orc_conditions* orc_conditions_create(const void* cabling, const uint8_t* modToU
 void orc_conditions_destroy(orc_conditions* h) { delete h; }
 
 // capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
+// vertex clustering algorithm of the chain: 0 density (the reference's default), 1 DBSCAN, 2 iterative
+int orc_conditions_set_vertex_algo(orc_conditions* h, int algo) {
+  if (algo < 0 || algo > 2)
+    return -1;
+  h->c.vertexAlgo = algo;
+  return 0;
+}
+
 int orc_conditions_set_profile(orc_conditions* h, int profile) {
   if (profile == 0)
     h->c.caps = kCapsReference;
@@ -264,6 +272,14 @@ void orc_invert_spd(const double* a, double* inv, int n) { invertSPDForTest(a, i
 int orc_vertices(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv, float* wv,
                  float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd) {
   return verticesDirect(nt, zt, ezt2, ptt2, iv, zv, wv, chi2, ptv2, ndof, sortInd);
+}
+
+// the same with the clustering algorithm chosen: 0 density, 1 DBSCAN, 2 iterative
+int orc_vertices_algo(int algo, uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv,
+                      float* zv, float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd) {
+  if (algo < 0 || algo > 2)
+    return -1;
+  return verticesDirect(nt, zt, ezt2, ptt2, iv, zv, wv, chi2, ptv2, ndof, sortInd, algo);
 }
 
 // circle through three points as the CA cuts see it: curvature and dca0 (distance of closest

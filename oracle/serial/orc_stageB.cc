@@ -290,6 +290,7 @@ namespace orc {
 
   bool process(const Conditions& c, const uint8_t* raw, size_t len, int upto, Event& ev) {
     ev.caps = c.caps;
+    ev.vertexAlgo = c.vertexAlgo;
     if (!unpackFeds(c, raw, len, ev))
       return false;
     rawToDigi(c, ev);

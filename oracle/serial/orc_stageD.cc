@@ -41,12 +41,12 @@ namespace orc {
       ezt2.push_back(C.cov[idx][14]);
       ptt2.push_back(pt * pt);
     }
-    clusterAndFit(V, itrk, zt, ezt2, ptt2);
+    clusterAndFit(V, itrk, zt, ezt2, ptt2, ev.vertexAlgo);
   }
 
   // density clustering, fit, split, refit and sort over loaded tracks
   void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
-                     const std::vector<float>& ezt2, const std::vector<float>& ptt2) {
+                     const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo) {
     V.zv.assign(kMaxVertices, 0);
     V.wv.assign(kMaxVertices, 0);
     V.chi2.assign(kMaxVertices, 0);
@@ -83,41 +83,159 @@ namespace orc {
       for (uint32_t p = off[bs]; p < off[be + 1]; ++p)
         f(uint32_t(bins[p]));
     };
-    for (uint32_t i = 0; i < nt; ++i) {
-      if (ezt2[i] > er2mx)
-        continue;
-      forEachNear(izt[i], [&](uint32_t j) {
-        if (i == j)
-          return;
-        float dist = std::abs(zt[i] - zt[j]);
-        if (dist > eps)
-          return;
-        if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
-          return;
-        nn[i]++;
-      });
+    if (algo == 1) {
+      // clusterTracksDBSCAN (gpuClusterTracksDBSCAN.h:17-215): neighbours within
+      // eps only; a core track links to its lowest-z core neighbour (first in
+      // histogram order on a tie); the chains are followed to their roots; the
+      // other tracks take the root of the nearest core (last on a tie) within
+      // eps and the error-scaled cut
+      for (uint32_t i = 0; i < nt; ++i) {
+        if (ezt2[i] > er2mx)
+          continue;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (i == j)
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > eps)
+            return;
+          nn[i]++;
+        });
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        if (nn[i] < minT)
+          continue;
+        float mz = zt[i];
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (zt[j] >= mz)
+            return;
+          if (nn[j] < minT)
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > eps)
+            return;
+          mz = zt[j];
+          iv[i] = int32_t(j);
+        });
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        int32_t m = iv[i];
+        while (m != iv[m])
+          m = iv[m];
+        iv[i] = m;
+      }
+    } else if (algo == 2) {
+      // clusterTracksIterative (gpuClusterTracksIterative.h:17-193): neighbours
+      // within eps and the error-scaled cut; the core tracks merge labels with
+      // their core neighbours later in histogram order (within the next bin)
+      // and the labels are followed to their roots, in alternate passes, until
+      // a merging pass changes nothing
+      for (uint32_t i = 0; i < nt; ++i) {
+        if (ezt2[i] > er2mx)
+          continue;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (i == j)
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > eps)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          nn[i]++;
+        });
+      }
+      int nloops = 0;
+      bool more = true;
+      while (more) {
+        if (1 == nloops % 2) {
+          for (uint32_t i = 0; i < nt; ++i) {
+            int32_t m = iv[i];
+            while (m != iv[m])
+              m = iv[m];
+            iv[i] = m;
+          }
+        } else {
+          more = false;
+          for (uint32_t k = 0; k < nt; ++k) {
+            const uint32_t i = bins[k];
+            const int be = std::min(int(izt[i]) + 1, 255);
+            if (nn[i] < minT)
+              continue;
+            for (uint32_t p = k + 1; p < off[be + 1]; ++p) {
+              const uint32_t j = bins[p];
+              if (nn[j] < minT)
+                continue;
+              float dist = std::abs(zt[i] - zt[j]);
+              if (dist > eps)
+                continue;
+              if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+                continue;
+              const int32_t old = iv[j];
+              iv[j] = std::min(old, iv[i]);
+              if (old != iv[i])
+                more = true;
+              iv[i] = std::min(iv[i], old);
+            }
+          }
+        }
+        ++nloops;
+      }
+    } else {
+      // clusterTracksByDensity (gpuClusterTracksByDensity.h:20-207)
+      for (uint32_t i = 0; i < nt; ++i) {
+        if (ezt2[i] > er2mx)
+          continue;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (i == j)
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > eps)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          nn[i]++;
+        });
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        float mdist = eps;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (nn[j] < nn[i])
+            return;
+          if (nn[j] == nn[i] && zt[j] >= zt[i])
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > mdist)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          mdist = dist;
+          iv[i] = int32_t(j);
+        });
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        int32_t m = iv[i];
+        while (m != iv[m])
+          m = iv[m];
+        iv[i] = m;
+      }
     }
-    for (uint32_t i = 0; i < nt; ++i) {
-      float mdist = eps;
-      forEachNear(izt[i], [&](uint32_t j) {
-        if (nn[j] < nn[i])
-          return;
-        if (nn[j] == nn[i] && zt[j] >= zt[i])
-          return;
-        float dist = std::abs(zt[i] - zt[j]);
-        if (dist > mdist)
-          return;
-        if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
-          return;
-        mdist = dist;
-        iv[i] = int32_t(j);
-      });
-    }
-    for (uint32_t i = 0; i < nt; ++i) {
-      int32_t m = iv[i];
-      while (m != iv[m])
-        m = iv[m];
-      iv[i] = m;
+    if (algo == 1 || algo == 2) {
+      // edges (both algorithms): non-core tracks join the nearest core track
+      for (uint32_t i = 0; i < nt; ++i) {
+        if (nn[i] >= minT)
+          continue;
+        float mdist = eps;
+        forEachNear(izt[i], [&](uint32_t j) {
+          if (nn[j] < minT)
+            return;
+          float dist = std::abs(zt[i] - zt[j]);
+          if (dist > mdist)
+            return;
+          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+            return;
+          mdist = dist;
+          iv[i] = iv[j];
+        });
+      }
     }
     uint32_t found = 0;
     for (uint32_t i = 0; i < nt; ++i) {
@@ -268,7 +386,7 @@ namespace orc {
   }
 
   int verticesDirect(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv,
-                     float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd) {
+                     float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd, int algo) {
     VertexState V;
     V.idv.assign(nt, -1);
     V.ndof.assign(std::max<uint32_t>(nt, kMaxVertices), 0);
@@ -276,7 +394,7 @@ namespace orc {
     for (uint32_t i = 0; i < nt; ++i)
       itrk[i] = uint16_t(i);
     clusterAndFit(V, itrk, std::vector<float>(zt, zt + nt), std::vector<float>(ezt2, ezt2 + nt),
-                  std::vector<float>(ptt2, ptt2 + nt));
+                  std::vector<float>(ptt2, ptt2 + nt), algo);
     for (uint32_t i = 0; i < nt; ++i)
       iv[i] = V.idv[i];
     for (uint32_t k = 0; k < V.nvFinal && k < kMaxVertices; ++k) {

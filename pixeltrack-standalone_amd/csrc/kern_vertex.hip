@@ -96,7 +96,7 @@ namespace pxg {
       }
     }
 
-    __global__ void __launch_bounds__(kVB) vertexKernel(CAPool ca) {
+    __global__ void __launch_bounds__(kVB) vertexKernel(CAPool ca, int algo) {
       const uint32_t e = blockIdx.x;
       __shared__ VtxLds L;
       const size_t tb = ca.tupleBase(e);
@@ -219,10 +219,11 @@ namespace pxg {
           continue;
         uint32_t pb, pe;
         nearRange(i, pb, pe);
-        // a neighbour: another track within eps and within the error-scaled cut
+        // a neighbour: another track within eps and (not for DBSCAN) within the
+        // error-scaled cut
         auto near = [&](uint32_t p, float zj, float ej) -> int32_t {
           const float dist = fabsf(zi - zj);
-          return p != pp && !(dist > eps) && !(dist * dist > chi2max * (ei + ej));
+          return p != pp && !(dist > eps) && !(algo != kVertexDBSCAN && dist * dist > chi2max * (ei + ej));
         };
         int32_t c = 0;
         uint32_t p = pb;
@@ -237,41 +238,112 @@ namespace pxg {
         nnS[pp] = c;
       }
       __syncthreads();
-      for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
-        const uint32_t i = zbins[pp];
-        const float zi = zS[pp], ei = eS[pp];
-        const int32_t ni = nnS[pp];
-        uint32_t pb, pe;
-        nearRange(i, pb, pe);
-        float mdist = eps;
-        uint32_t bestP = 0xffffffffu;
-        // the closest qualifying track, first one on ties (candidates in order)
-        auto visit = [&](uint32_t p, int32_t nj, float zj, float ej) {
-          if (nj < ni)
-            return;
-          if (nj == ni && zj >= zi)
-            return;
-          const float dist = fabsf(zi - zj);
-          if (dist > mdist)
-            return;
-          if (dist * dist > chi2max * (ei + ej))
-            return;
-          mdist = dist;
-          bestP = p;
-        };
-        uint32_t p = pb;
-        for (; p + 4 <= pe; p += 4) {
-          const int32_t n0 = nnS[p], n1 = nnS[p + 1], n2 = nnS[p + 2], n3 = nnS[p + 3];
-          const float z0 = zS[p], z1 = zS[p + 1], z2 = zS[p + 2], z3 = zS[p + 3];
-          const float e0 = eS[p], e1 = eS[p + 1], e2 = eS[p + 2], e3 = eS[p + 3];
-          visit(p, n0, z0, e0);
-          visit(p + 1, n1, z1, e1);
-          visit(p + 2, n2, z2, e2);
-          visit(p + 3, n3, z3, e3);
+      if (algo == kVertexDensity) {
+        // clusterTracksByDensity: each track links to the closest track with more
+        // neighbours (or as many at smaller z)
+        for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+          const uint32_t i = zbins[pp];
+          const float zi = zS[pp], ei = eS[pp];
+          const int32_t ni = nnS[pp];
+          uint32_t pb, pe;
+          nearRange(i, pb, pe);
+          float mdist = eps;
+          uint32_t bestP = 0xffffffffu;
+          // the closest qualifying track, first one on ties (candidates in order)
+          auto visit = [&](uint32_t p, int32_t nj, float zj, float ej) {
+            if (nj < ni)
+              return;
+            if (nj == ni && zj >= zi)
+              return;
+            const float dist = fabsf(zi - zj);
+            if (dist > mdist)
+              return;
+            if (dist * dist > chi2max * (ei + ej))
+              return;
+            mdist = dist;
+            bestP = p;
+          };
+          uint32_t p = pb;
+          for (; p + 4 <= pe; p += 4) {
+            const int32_t n0 = nnS[p], n1 = nnS[p + 1], n2 = nnS[p + 2], n3 = nnS[p + 3];
+            const float z0 = zS[p], z1 = zS[p + 1], z2 = zS[p + 2], z3 = zS[p + 3];
+            const float e0 = eS[p], e1 = eS[p + 1], e2 = eS[p + 2], e3 = eS[p + 3];
+            visit(p, n0, z0, e0);
+            visit(p + 1, n1, z1, e1);
+            visit(p + 2, n2, z2, e2);
+            visit(p + 3, n3, z3, e3);
+          }
+          for (; p < pe; ++p)
+            visit(p, nnS[p], zS[p], eS[p]);
+          iv[i] = bestP == 0xffffffffu ? int32_t(i) : int32_t(zbins[bestP]);
         }
-        for (; p < pe; ++p)
-          visit(p, nnS[p], zS[p], eS[p]);
-        iv[i] = bestP == 0xffffffffu ? int32_t(i) : int32_t(zbins[bestP]);
+      } else if (algo == kVertexDBSCAN) {
+        // clusterTracksDBSCAN (gpuClusterTracksDBSCAN.h:100-119): a core track
+        // links to its lowest-z core neighbour within eps, the first one in
+        // histogram order on a tie
+        for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+          if (nnS[pp] < minT)
+            continue;
+          const uint32_t i = zbins[pp];
+          const float zi = zS[pp];
+          uint32_t pb, pe;
+          nearRange(i, pb, pe);
+          float mz = zi;
+          uint32_t linkP = 0xffffffffu;
+          for (uint32_t p = pb; p < pe; ++p) {
+            const float zj = zS[p];
+            if (zj >= mz)
+              continue;
+            if (nnS[p] < minT)
+              continue;
+            if (fabsf(zi - zj) > eps)
+              continue;
+            mz = zj;
+            linkP = p;
+          }
+          if (linkP != 0xffffffffu)
+            iv[i] = int32_t(zbins[linkP]);
+        }
+      } else {
+        // clusterTracksIterative (gpuClusterTracksIterative.h:101-146): the core
+        // tracks connected by neighbour pairs share the smallest track index
+        // among them.  Here: each core track takes the smallest label of its
+        // core neighbours, then labels are followed to their roots, until
+        // nothing changes (labels only decrease and never exceed the index).
+        for (uint32_t r = 0; r <= nt; ++r) {
+          bool changed = false;
+          for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+            if (nnS[pp] < minT)
+              continue;
+            const uint32_t i = zbins[pp];
+            const float zi = zS[pp], ei = eS[pp];
+            uint32_t pb, pe;
+            nearRange(i, pb, pe);
+            int32_t m = iv[i];
+            for (uint32_t p = pb; p < pe; ++p) {
+              if (p == pp || nnS[p] < minT)
+                continue;
+              const float ej = eS[p];
+              const float dist = fabsf(zi - zS[p]);
+              if (dist > eps || dist * dist > chi2max * (ei + ej))
+                continue;
+              m = min(m, iv[zbins[p]]);
+            }
+            if (m < iv[i]) {
+              atomicMin(&iv[i], m);
+              changed = true;
+            }
+          }
+          __syncthreads();
+          for (uint32_t i = threadIdx.x; i < nt; i += kVB) {
+            int32_t m = iv[i];
+            for (uint32_t s2 = 0; m != iv[m] && s2 < nt; ++s2)
+              m = iv[m];
+            atomicMin(&iv[i], m);
+          }
+          if (!__syncthreads_or(changed))
+            break;
+        }
       }
       __syncthreads();
       // follow the links to the root of each tree
@@ -283,6 +355,35 @@ namespace pxg {
         tmp[i] = m;
       }
       __syncthreads();
+      if (algo != kVertexDensity) {
+        // edges (gpuClusterTracksDBSCAN.h:156-175): a track that is not core
+        // joins the root of the nearest core track within eps and the
+        // error-scaled cut, the last one in histogram order on a tie
+        for (uint32_t pp = threadIdx.x; pp < nt; pp += kVB) {
+          if (nnS[pp] >= minT)
+            continue;
+          const uint32_t i = zbins[pp];
+          const float zi = zS[pp], ei = eS[pp];
+          uint32_t pb, pe;
+          nearRange(i, pb, pe);
+          float mdist = eps;
+          uint32_t nearP = 0xffffffffu;
+          for (uint32_t p = pb; p < pe; ++p) {
+            if (nnS[p] < minT)
+              continue;
+            const float dist = fabsf(zi - zS[p]);
+            if (dist > mdist)
+              continue;
+            if (dist * dist > chi2max * (ei + eS[p]))
+              continue;
+            mdist = dist;
+            nearP = p;
+          }
+          if (nearP != 0xffffffffu)
+            tmp[i] = tmp[zbins[nearP]];
+        }
+        __syncthreads();
+      }
       // roots with enough neighbours become vertices, numbered in track order
       if (threadIdx.x == 0)
         L.found = 0;
@@ -507,7 +608,7 @@ namespace pxg {
 
   void launchVertices(const CALaunch& l) {
     l.t->b("vertices");
-    vertexKernel<<<l.nEvents, kVB, 0, l.s>>>(*l.ca);
+    vertexKernel<<<l.nEvents, kVB, 0, l.s>>>(*l.ca, l.vertexAlgo);
     l.t->e();
   }
 

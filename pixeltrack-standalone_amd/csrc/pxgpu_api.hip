@@ -337,7 +337,7 @@ namespace pxg::detail {
     if (s->profiling)
       timer = {timerBegin, timerEnd, s};
     const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
-                     s->stream, &timer, s->doubletLanes};
+                     s->stream, &timer, s->doubletLanes, s->vertexAlgo};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -766,6 +766,17 @@ int pxg_set_tuning(pxg_stream* s, uint32_t doubletLanes, uint32_t gridTarget) {
     return fail(PXG_ERR_ARG, "pxg_set_tuning: lanes must be a power of two in [1, 64], grid target >= 4");
   s->doubletLanes = doubletLanes;
   s->gridTarget = gridTarget;
+  return 0;
+}
+
+int pxg_set_vertex_algorithm(pxg_stream* s, int algo) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_set_vertex_algorithm: null stream");
+  if (algo != PXG_VERTEX_DENSITY && algo != PXG_VERTEX_DBSCAN && algo != PXG_VERTEX_ITERATIVE)
+    return fail(PXG_ERR_ARG, "pxg_set_vertex_algorithm: unknown algorithm %d", algo);
+  static_assert(PXG_VERTEX_DENSITY == kVertexDensity && PXG_VERTEX_DBSCAN == kVertexDBSCAN &&
+                PXG_VERTEX_ITERATIVE == kVertexIterative);
+  s->vertexAlgo = algo;
   return 0;
 }
 

@@ -113,6 +113,8 @@ namespace pxg {
   constexpr uint32_t kHitStride = kMaxNumClusters + 64;      // per-event stride of per-hit offset arrays
   constexpr uint32_t kCaBlock = 256;                         // workgroup of the grid-stride CA kernels
   constexpr uint32_t kEventBlock = 1024;                     // workgroup of the per-event kernels
+  // vertex clustering algorithms (gpuVertexFinder.h:46-60; density is the reference's default)
+  enum : int { kVertexDensity = 0, kVertexDBSCAN = 1, kVertexIterative = 2 };
   constexpr uint32_t kDupPredCap = 6;                        // predecessors kept per cell for the duplicate remover
 
   // ---- XCD-aware workgroup order -------------------------------------------------------
@@ -243,6 +245,7 @@ namespace pxg {
     hipStream_t s;
     const Timer* t;
     uint32_t doubletLanes;  // lanes per (layer pair, inner hit) slot of the doublet search: 1..64
+    int vertexAlgo;         // vertex clustering: kVertexDensity, kVertexDBSCAN or kVertexIterative
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);

@@ -77,6 +77,7 @@ class ServerResult(C.Structure):
 CA_COUNTS = ["nCells", "nDoublets", "apcM", "apcN", "nTuples", "caStatus", "nbLists", "trLists", "nMulti", "nMult",
              "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors", "nStart"]
 PROFILE_REFERENCE, PROFILE_HIGH_OCCUPANCY = 0, 1
+VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2  # pxg_set_vertex_algorithm
 
 
 DIGI_FIELDS = {"xx": np.uint16, "yy": np.uint16, "adc": np.uint16, "moduleInd": np.uint16, "clus": np.int32,
@@ -110,6 +111,7 @@ def _gpu():
         lib.pxg_server_release.argtypes = [vp, C.c_uint64]
         lib.pxg_server_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         lib.pxg_set_tuning.argtypes = [vp, u32, u32]
+        lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
         lib.pxg_get_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         lib.pxg_autotune.argtypes = [vp, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(C.c_float)]
@@ -422,6 +424,10 @@ class Stream:
 
     def set_tuning(self, doublet_lanes, grid_target):
         _check(self.lib.pxg_set_tuning(self._h, doublet_lanes, grid_target), "pxg_set_tuning")
+
+    def set_vertex_algorithm(self, algo):
+        """VERTEX_DENSITY (default), VERTEX_DBSCAN or VERTEX_ITERATIVE."""
+        _check(self.lib.pxg_set_vertex_algorithm(self._h, algo), "pxg_set_vertex_algorithm")
 
     def tuning(self):
         a, b = u32(), u32()

@@ -427,3 +427,99 @@ def test_chain_finds_generated_vertices(small_events):
         tz = np.array(truth.vertex_z[:truth.n_vertices]) - 0.2  # beam spot z
         d = np.abs(o["zv"][:nv, None] - tz[None, :]).min(axis=1)
         assert np.median(d) < 0.05, name
+
+
+def _brute_vertex_clusters(zt, ezt2, algo):
+    """Vertex ids the DBSCAN (algo 1) and iterative (algo 2) clusterings give, from their
+    definitions over all track pairs: core tracks (at least 2 neighbours within eps, with
+    the error-scaled cut for algo 2, counted for tracks with error below errmax) connected
+    by neighbour pairs form a vertex rooted at its lowest-z core (algo 1) or its lowest
+    track index (algo 2); other tracks join the nearest core within eps and the cut;
+    vertices are numbered in the order of their roots' indices; None for noise."""
+    f = np.float32
+    z, e = zt.astype(np.float32), ezt2.astype(np.float32)
+    eps, chi2max, minT = f(0.07), f(9.0), 2
+    er2mx = f(0.01) * f(0.01)
+    nt = len(z)
+
+    def near(i, j, chi2):
+        d = f(abs(z[i] - z[j]))
+        return not (d > eps) and not (chi2 and d * d > chi2max * (e[i] + e[j]))
+
+    nn = [0 if e[i] > er2mx else sum(near(i, j, algo == 2) for j in range(nt) if j != i) for i in range(nt)]
+    core = [n >= minT for n in nn]
+    comp = list(range(nt))
+
+    def find(a):
+        while comp[a] != a:
+            a = comp[a]
+        return a
+    for i in range(nt):
+        for j in range(i + 1, nt):
+            if core[i] and core[j] and near(i, j, algo == 2):
+                comp[find(i)] = find(j)
+    members = {}
+    for i in range(nt):
+        if core[i]:
+            members.setdefault(find(i), []).append(i)
+    root = {}
+    for m in members.values():
+        r = min(m, key=lambda k: z[k]) if algo == 1 else min(m)
+        for k in m:
+            root[k] = r
+    for i in range(nt):
+        if not core[i]:
+            cand = [j for j in range(nt) if core[j] and near(i, j, True)]
+            if cand:
+                root[i] = root[min(cand, key=lambda j: abs(z[i] - z[j]))]
+    ids = {r: n for n, r in enumerate(sorted(set(root.values())))}
+    return [ids[root[i]] if i in root else None for i in range(nt)]
+
+
+@pytest.mark.parametrize("algo", [1, 2])
+def test_vertex_clustering_variants_against_brute_force(oracle, algo):
+    """DBSCAN and iterative clustering (gpuClusterTracksDBSCAN.h, gpuClusterTracksIterative.h)
+    on well separated tight vertices, wide-error tracks near them and isolated noise, where
+    the fits and the split change nothing: the ids equal the definitions' above."""
+    rng = np.random.default_rng(20 + algo)
+    tracks = []
+    for zv in (-6.0, -3.1, 0.2, 2.9, 5.5, 8.4):
+        for _ in range(int(rng.integers(5, 12))):
+            tracks.append((zv + rng.normal(0, 0.004), 0.004 ** 2 * rng.uniform(0.8, 1.2)))
+    for zv in (-6.0, -3.1, 0.2):
+        tracks.append((zv + 0.03 + rng.uniform(-0.005, 0.005), 0.02 ** 2))  # not core: joins the vertex
+    tracks += [(-10.0, 0.004 ** 2), (11.5, 0.004 ** 2)]  # noise
+    order = rng.permutation(len(tracks))
+    zt = np.array([tracks[k][0] for k in order], np.float32)
+    ezt2 = np.array([tracks[k][1] for k in order], np.float32)
+    got = oracle.vertices(zt, ezt2, np.ones_like(zt), algo=algo)
+    exp = _brute_vertex_clusters(zt, ezt2, algo)
+    assert got["nv"] == 6
+    for i, x in enumerate(exp):
+        if x is None:
+            assert got["iv"][i] > 9990, i  # unassigned
+        else:
+            assert got["iv"][i] == x, i
+    # the three clusterings agree on these vertices up to their numbering
+    def groups(iv):
+        return sorted(sorted(np.nonzero(iv == v)[0].tolist()) for v in range(6))
+    dens = oracle.vertices(zt, ezt2, np.ones_like(zt), algo=0)
+    assert dens["nv"] == 6 and groups(dens["iv"]) == groups(got["iv"])
+
+
+def test_dbscan_and_iterative_number_vertices_by_different_roots(oracle):
+    """A vertex whose lowest-index track is not its lowest-z one: DBSCAN roots it at the
+    lowest z, the iterative clustering at the lowest index, so the numbering differs."""
+    f = np.float32
+    # vertex A at z=1 listed first with its lowest-z track last; vertex B at z=-1 in between
+    zt = np.array([1.004, 1.0, 1.002, -1.0, -1.002, -1.004, 0.996], f)
+    ezt2 = np.full(7, 0.004 ** 2, f)
+    d = oracle.vertices(zt, ezt2, np.ones_like(zt), algo=1)
+    it = oracle.vertices(zt, ezt2, np.ones_like(zt), algo=2)
+    assert d["nv"] == it["nv"] == 2
+    # DBSCAN: roots are track 6 (A, z=0.996) and track 5 (B, z=-1.004): B first
+    assert list(d["iv"]) == [1, 1, 1, 0, 0, 0, 1]
+    # iterative: roots are track 0 (A) and track 3 (B): A first
+    assert list(it["iv"]) == [0, 0, 0, 1, 1, 1, 0]
+    with pytest.raises(ValueError):
+        oracle.vertices(zt, ezt2, np.ones_like(zt), algo=3)
