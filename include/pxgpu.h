@@ -281,6 +281,11 @@ int pxg_set_tuning(pxg_stream* s, uint32_t doublet_lanes, uint32_t grid_target);
 #define PXG_VERTEX_DBSCAN 1
 #define PXG_VERTEX_ITERATIVE 2
 int pxg_set_vertex_algorithm(pxg_stream* s, int algo);
+/* Fishbone passes of the stream's ntuplet stage: the reference's earlyFishbone / lateFishbone
+ * switches (CAHitNtupletGeneratorKernels.cc:109-112 before the ntuplet search on the connected
+ * cells, :133-136 after it on the cells in tuples, with kernel_fishboneCleaner after the
+ * classification at :161-164; CAHitNtupletGeneratorOnGPU.cc:55-56 sets early on, late off). */
+int pxg_set_fishbone(pxg_stream* s, int early, int late);
 int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
 /* Times the chain on resident events [first, first+count) for each launch shape of a fixed
  * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */

@@ -48,6 +48,8 @@ def _load(path, prefix):
             lib.orc_vertices_algo.argtypes = [C.c_int, C.c_uint32] + [vp] * 10
             lib.orc_conditions_set_vertex_algo.restype = C.c_int
             lib.orc_conditions_set_vertex_algo.argtypes = [vp, C.c_int]
+            lib.orc_conditions_set_fishbone.restype = C.c_int
+            lib.orc_conditions_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
             lib.orc_circle.argtypes = [C.c_float] * 6 + [C.POINTER(C.c_float)] * 2
             lib.orc_run_many.restype = C.c_uint32
             lib.orc_run_many.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_uint32]
@@ -136,8 +138,9 @@ class Oracle(_Chain):
     # vertex clustering algorithms (gpuVertexFinder.h:46-60)
     VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2
 
-    def __init__(self, conditions, profile=0, vertex_algo=0):
+    def __init__(self, conditions, profile=0, vertex_algo=0, early_fishbone=True, late_fishbone=False):
         super().__init__(conditions)
+        self.lib.orc_conditions_set_fishbone(self._h, int(early_fishbone), int(late_fishbone))
         if self.lib.orc_conditions_set_profile(self._h, profile) != 0:
             raise ValueError(f"unknown capacity profile {profile}")
         if self.lib.orc_conditions_set_vertex_algo(self._h, vertex_algo) != 0:

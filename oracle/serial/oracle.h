@@ -44,6 +44,7 @@ namespace orc {
     uint32_t nFeds;
     CACapacities caps = kCapsReference;
     int vertexAlgo = 0;  // 0 density, 1 DBSCAN, 2 iterative (gpuVertexFinder.h:46-60)
+    bool earlyFishbone = true, lateFishbone = false;  // CAHitNtupletGeneratorOnGPU.cc:51-52
   };
 
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
@@ -132,6 +133,7 @@ namespace orc {
     // a11-a22: CA ntuplets and fits; a23-a25: vertices
     CACapacities caps = kCapsReference;
     int vertexAlgo = 0;
+    bool earlyFishbone = true, lateFishbone = false;
     CAState ca;
     VertexState vtx;
   };
@@ -154,7 +156,8 @@ namespace orc {
   // a11-a17, a20-a22 (orc_stageC.cc)
   void buildDoublets(Event& ev);
   void connectCells(Event& ev);
-  void fishbone(Event& ev);
+  void fishbone(Event& ev, bool checkTrack = false);
+  void fishboneCleaner(Event& ev);
   void findNtuplets(Event& ev);
   void earlyDuplicatesAndMultiplicity(Event& ev);
   void fillHitDetIndices(Event& ev);

@@ -137,6 +137,13 @@ orc_conditions* orc_conditions_create(const void* cabling, const uint8_t* modToU
 void orc_conditions_destroy(orc_conditions* h) { delete h; }
 
 // capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
+// fishbone passes of the chain (the reference runs the early one only)
+int orc_conditions_set_fishbone(orc_conditions* h, int early, int late) {
+  h->c.earlyFishbone = early != 0;
+  h->c.lateFishbone = late != 0;
+  return 0;
+}
+
 // vertex clustering algorithm of the chain: 0 density (the reference's default), 1 DBSCAN, 2 iterative
 int orc_conditions_set_vertex_algo(orc_conditions* h, int algo) {
   if (algo < 0 || algo > 2)

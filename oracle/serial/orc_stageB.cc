@@ -291,6 +291,8 @@ namespace orc {
   bool process(const Conditions& c, const uint8_t* raw, size_t len, int upto, Event& ev) {
     ev.caps = c.caps;
     ev.vertexAlgo = c.vertexAlgo;
+    ev.earlyFishbone = c.earlyFishbone;
+    ev.lateFishbone = c.lateFishbone;
     if (!unpackFeds(c, raw, len, ev))
       return false;
     rawToDigi(c, ev);
@@ -311,14 +313,19 @@ namespace orc {
     // CAHitNtupletGeneratorOnGPU::makeTuples (CAHitNtupletGeneratorOnGPU.cc:96-126)
     buildDoublets(ev);
     connectCells(ev);
-    if (ev.nHits > 1)
+    if (ev.nHits > 1 && ev.earlyFishbone)
       fishbone(ev);
     findNtuplets(ev);
     earlyDuplicatesAndMultiplicity(ev);
+    // CAHitNtupletGeneratorKernels.cc:133-136: the fishbone again over the cells in tuples
+    if (ev.nHits > 1 && ev.lateFishbone)
+      fishbone(ev, true);
     fillHitDetIndices(ev);
     fitTracks(c, ev);
     if (ev.nHits > 0) {
       classifyTracks(ev);
+      if (ev.lateFishbone)  // CAHitNtupletGeneratorKernels.cc:161-164
+        fishboneCleaner(ev);
       fastDuplicateRemover(ev);
       hitToTupleAndTripletCleaner(ev);
     }

@@ -252,8 +252,8 @@ namespace orc {
     }
   }
 
-  // a14 fishbone (early, no track check): gpuFishbone.h:21-90
-  void fishbone(Event& ev) {
+  // a14 fishbone: gpuFishbone.h:21-90 (checkTrack: the late pass, over the cells in tuples)
+  void fishbone(Event& ev, bool checkTrack) {
     auto& C = ev.ca;
     constexpr int kMax = kMaxCellsPerHit;
     float x[kMax], y[kMax], z[kMax], n[kMax];
@@ -269,6 +269,8 @@ namespace orc {
       for (int ic = 0; ic < vc.size; ++ic) {
         const Cell& ci = C.cells[vc.data[ic]];
         if (ci.used == 0)
+          continue;
+        if (checkTrack && (ci.tracks == 0 || C.trackStore[ci.tracks].size == 0))
           continue;
         cc[sg] = vc.data[ic];
         d[sg] = uint16_t(float(ev.detInd[ci.innerHit]));
@@ -538,6 +540,20 @@ namespace orc {
         if (C.quality[it] != kQualityBad && it != im)
           C.quality[it] = kQualityDup;
       }
+    }
+  }
+
+  // kernel_fishboneCleaner (CAHitNtupletGeneratorKernelsImpl.h:113-125): the tuples of
+  // the cells the fishbone killed are bad
+  void fishboneCleaner(Event& ev) {
+    auto& C = ev.ca;
+    for (uint32_t idx = 0; idx < C.nCells; ++idx) {
+      const Cell& cell = C.cells[idx];
+      if (cell.doubletId >= 0 || cell.tracks == 0)
+        continue;
+      const auto& tr = C.trackStore[cell.tracks];
+      for (int k = 0; k < tr.size; ++k)
+        C.quality[tr.data[k]] = kQualityBad;
     }
   }
 

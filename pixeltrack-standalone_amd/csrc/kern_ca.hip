@@ -626,6 +626,7 @@ namespace pxg {
     // aligned later cell that is closer, which kills ci and ends the row.  The
     // rows do not read the kills, so each row runs on its own (one thread per
     // cell, in list order), finding its first killer as the smallest such cell.
+    // The late pass (checkTrack) takes part only with the cells that are in tuples.
     // A cell seen from the list's hit: its inner hit relative to that hit.
     struct RowCell {
       uint32_t c;
@@ -633,6 +634,7 @@ namespace pxg {
       float x, y, z, n;  // inner hit minus the list's hit, squared norm
     };
 
+    template <bool kCheckTrack>
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const HitView H = hitsOf(p, e);
@@ -647,7 +649,10 @@ namespace pxg {
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       const uint2* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
-      auto isUsed = [&](uint32_t c) { return (used[c >> 5] >> (c & 31)) & 1; };
+      const uint32_t* trCount = ca.trCount + cb;
+      auto isUsed = [&](uint32_t c) {
+        return ((used[c >> 5] >> (c & 31)) & 1) && (!kCheckTrack || trCount[c] > 0);
+      };
       const uint32_t lane = laneId(), wbase = threadIdx.x - lane;
       for (uint32_t pos0 = gridStart() - lane; pos0 < nCells; pos0 += gridStride()) {  // uniform per wave
         const uint32_t pos = pos0 + lane;
@@ -1182,8 +1187,11 @@ namespace pxg {
     PXG_K("connect", connectKernel, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
   }
 
-  void launchFishbone(const CALaunch& l) {
-    PXG_K("fishbone", fishboneKernel, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+  void launchFishbone(const CALaunch& l, bool checkTrack) {
+    if (checkTrack)
+      PXG_K("lateFishbone", fishboneKernel<true>, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+    else
+      PXG_K("fishbone", fishboneKernel<false>, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
   }
 
   void launchNtuplets(const CALaunch& l) {

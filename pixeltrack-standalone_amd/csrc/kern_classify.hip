@@ -51,6 +51,27 @@ namespace pxg {
       }
     }
 
+    // kernel_fishboneCleaner (CAHitNtupletGeneratorKernelsImpl.h:113-125): the tuples of
+    // a cell the late fishbone killed are bad.  Visits the cells that are in tuples
+    // through their track list slots.
+    __global__ void __launch_bounds__(kCaBlock) fishboneCleanerKernel(CAPool ca) {
+      const uint32_t e = logicalBlock().y;
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      const uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      const uint32_t* trCells = ca.dfsT + cb;
+      const uint32_t nLists = min(ca.cnt[e].trLists, ca.caps.maxCellLists - 1);
+      for (uint32_t slot = gridStart(); slot < nLists; slot += gridStride()) {
+        const uint32_t c = trCells[slot];
+        if (!(ca.cellFlags[cb + c] & 2))
+          continue;
+        const uint16_t* list = pool + size_t(slot) * kCaMaxCellTracks;
+        const uint32_t n = ca.trCount[cb + c];
+        for (uint32_t k = 0; k < n; ++k)
+          ca.quality[tb + list[k]] = kQualityBad;
+      }
+    }
+
     // a21 kernel_fastDuplicateRemover.  The serial loop visits the cells with two
     // or more tuples in ascending order, keeps the loose tuple with the smallest
     // |tip| of each and marks the other (not bad) tuples duplicates.  A cell's
@@ -375,6 +396,11 @@ namespace pxg {
     l.t->b("classify");
     classifyKernel<<<dim3(l.blocks, l.nEvents), kCaBlock, 0, l.s>>>(*l.ca);
     l.t->e();
+    if (l.lateFishbone) {
+      l.t->b("fishboneCleaner");
+      fishboneCleanerKernel<<<dim3(l.blocks, l.nEvents), kCaBlock, 0, l.s>>>(*l.ca);
+      l.t->e();
+    }
     l.t->b("dupPreds");
     dupPredsKernel<<<dim3(l.blocks, l.nEvents), kCaBlock, 0, l.s>>>(*l.ca);
     l.t->e();

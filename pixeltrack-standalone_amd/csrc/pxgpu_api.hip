@@ -337,14 +337,17 @@ namespace pxg::detail {
     if (s->profiling)
       timer = {timerBegin, timerEnd, s};
     const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
-                     s->stream, &timer, s->doubletLanes, s->vertexAlgo};
+                     s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
       launchConnect(l);
-      launchFishbone(l);
+      if (l.earlyFishbone)
+        launchFishbone(l, false);
       launchNtuplets(l);
       launchMultiplicity(l);
+      if (l.lateFishbone)
+        launchFishbone(l, true);
       launchFit(l);
       launchClassify(l);
       launchHitToTuple(l);
@@ -777,6 +780,14 @@ int pxg_set_vertex_algorithm(pxg_stream* s, int algo) {
   static_assert(PXG_VERTEX_DENSITY == kVertexDensity && PXG_VERTEX_DBSCAN == kVertexDBSCAN &&
                 PXG_VERTEX_ITERATIVE == kVertexIterative);
   s->vertexAlgo = algo;
+  return 0;
+}
+
+int pxg_set_fishbone(pxg_stream* s, int early, int late) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_set_fishbone: null stream");
+  s->earlyFishbone = early != 0;
+  s->lateFishbone = late != 0;
   return 0;
 }
 

@@ -246,11 +246,13 @@ namespace pxg {
     const Timer* t;
     uint32_t doubletLanes;  // lanes per (layer pair, inner hit) slot of the doublet search: 1..64
     int vertexAlgo;         // vertex clustering: kVertexDensity, kVertexDBSCAN or kVertexIterative
+    bool earlyFishbone;     // fishbone on the connected cells before the ntuplet search
+    bool lateFishbone;      // fishbone on the cells in tuples after it, and the cleaner after classify
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);
   void launchConnect(const CALaunch& l);
-  void launchFishbone(const CALaunch& l);
+  void launchFishbone(const CALaunch& l, bool checkTrack);
   void launchNtuplets(const CALaunch& l);
   void launchMultiplicity(const CALaunch& l);
   void launchFit(const CALaunch& l);

@@ -37,6 +37,8 @@ struct pxg_stream {
   uint32_t doubletLanes = 8;   // tuning: lanes per doublet slot
   uint32_t gridTarget = 2048;  // tuning: workgroups per grid-stride launch (divided among the events)
   int vertexAlgo = 0;          // vertex clustering algorithm (PXG_VERTEX_*)
+  bool earlyFishbone = true;   // the reference's defaults: early fishbone on, late off
+  bool lateFishbone = false;
   std::vector<ProfileMark> marks;
   std::map<std::string, std::pair<uint32_t, double>> profile;
 };

@@ -112,6 +112,7 @@ def _gpu():
         lib.pxg_server_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         lib.pxg_set_tuning.argtypes = [vp, u32, u32]
         lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
+        lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
         lib.pxg_get_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         lib.pxg_autotune.argtypes = [vp, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(C.c_float)]
@@ -428,6 +429,11 @@ class Stream:
     def set_vertex_algorithm(self, algo):
         """VERTEX_DENSITY (default), VERTEX_DBSCAN or VERTEX_ITERATIVE."""
         _check(self.lib.pxg_set_vertex_algorithm(self._h, algo), "pxg_set_vertex_algorithm")
+
+    def set_fishbone(self, early=True, late=False):
+        """The fishbone passes: early (before the ntuplet search, default on) and late
+        (on the cells in tuples, with the cleaner after classification, default off)."""
+        _check(self.lib.pxg_set_fishbone(self._h, int(early), int(late)), "pxg_set_fishbone")
 
     def tuning(self):
         a, b = u32(), u32()

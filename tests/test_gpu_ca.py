@@ -164,6 +164,23 @@ def test_alternative_vertex_clustering(detector, conditions, gpu_ctx, algo):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw, internals=False), f"algo {algo} pu50[{idx}]", internals=False)
 
 
+@pytest.mark.parametrize("early,late", [(False, True), (True, True), (False, False)])
+def test_fishbone_switches(detector, conditions, gpu_ctx, early, late):
+    """The reference's earlyFishbone / lateFishbone switches
+    (CAHitNtupletGeneratorKernels.cc:109-112, :133-136, :161-164) on the whole chain,
+    kills and per-cell track lists included, against the oracle."""
+    import pyoracle
+    orc = pyoracle.Oracle(conditions, early_fishbone=early, late_fishbone=late)
+    s = gpu_ctx.stream(max_events=1, max_words=300000)
+    s.set_fishbone(early, late)
+    for idx in (0, 3):
+        raw, _ = event(detector, "pu50", idx)
+        o = orc.process(raw, upto=5)
+        kills = int((o["cellDoubletId"] < 0).sum())
+        print(f"fishbone early={early} late={late} pu50[{idx}] kills={kills} tuples={o['nTuples']}", flush=True)
+        compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"fishbone {early}/{late} pu50[{idx}]")
+
+
 def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
     """Every launch shape the autotuner may pick gives the same products."""
     from pxtrack import gpu

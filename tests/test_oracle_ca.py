@@ -523,3 +523,85 @@ def test_dbscan_and_iterative_number_vertices_by_different_roots(oracle):
     assert list(it["iv"]) == [0, 0, 0, 1, 1, 1, 0]
     with pytest.raises(ValueError):
         oracle.vertices(zt, ezt2, np.ones_like(zt), algo=3)
+
+
+def _fishbone_pairs(o, participating):
+    """Aligned cell pairs of each outer-hit list as the fishbone criterion defines them
+    (gpuFishbone.h:75-77): different inner detector, cos^2 of the angle between the two
+    inner-hit directions seen from the common outer hit >= 0.99999.  Yields (list
+    position order) pairs (a, b, n_a, n_b) with a before b, both participating."""
+    f = np.float32
+    off, cells = o["outerHitOff"], o["outerHitCells"]
+    inner, outer = o["cellInner"], o["cellOuter"]
+    xg, yg, zg, det = o["xg"], o["yg"], o["zg"], o["detInd"]
+    for h in range(off.size - 1):
+        lst = [int(c) for c in cells[off[h]:off[h + 1]] if participating[c]]
+        if len(lst) < 2:
+            continue
+        ho = outer[lst[0]]
+        v = {c: (f(xg[inner[c]] - xg[ho]), f(yg[inner[c]] - yg[ho]), f(zg[inner[c]] - zg[ho])) for c in lst}
+        nrm = {c: f(f(v[c][0] * v[c][0] + v[c][1] * v[c][1]) + v[c][2] * v[c][2]) for c in lst}
+        for i, a in enumerate(lst):
+            for b in lst[i + 1:]:
+                cos = f(f(v[a][0] * v[b][0] + v[a][1] * v[b][1]) + v[a][2] * v[b][2])
+                if det[inner[a]] != det[inner[b]] and cos * cos >= f(0.99999) * nrm[a] * nrm[b]:
+                    yield a, b, nrm[a], nrm[b]
+
+
+def _check_fishbone_kills(o, participating, killed, ctx):
+    has_partner = set()
+    for a, b, na, nb in _fishbone_pairs(o, participating):
+        if nb < na:
+            assert killed[a], f"{ctx}: cell {a} has a closer aligned later cell {b} but survives"
+        if nb < na:
+            has_partner.add(a)
+        if na <= nb:
+            has_partner.add(b)
+    for c in np.nonzero(killed & participating)[0]:
+        assert int(c) in has_partner, f"{ctx}: cell {c} killed without an aligned cell at least as close"
+
+
+def test_fishbone_kills(small_events):
+    """Early fishbone (gpuFishbone.h:21-90) on the used cells: every killed cell has an
+    aligned cell of its list at least as close to the outer hit, and every cell with a
+    closer aligned cell later in its list is killed."""
+    for name, o, _ in small_events:
+        killed = o["cellDoubletId"] < 0
+        used = o["cellUsed"].astype(bool)
+        assert not (killed & ~used).any(), name
+        _check_fishbone_kills(o, used, killed, name)
+
+
+def test_late_fishbone_and_cleaner(detector, conditions):
+    """Late fishbone (CAHitNtupletGeneratorKernels.cc:133-136, on the cells in tuples) and
+    kernel_fishboneCleaner, with the early pass off: the tuples do not change, the kills
+    satisfy the fishbone criterion among the cells with tuples, every tuple of a killed
+    cell is bad.  With the early pass on, the late
+    pass finds nothing: every aligned pair of surviving used cells was already examined."""
+    import pyoracle
+    none = pyoracle.Oracle(conditions, early_fishbone=False)
+    late = pyoracle.Oracle(conditions, early_fishbone=False, late_fishbone=True)
+    early = pyoracle.Oracle(conditions)
+    both = pyoracle.Oracle(conditions, late_fishbone=True)
+    for idx in (0, 1):
+        raw, _ = event(detector, "pu50", idx, pu_lo=12, pu_hi=12)
+        a, b = none.process(raw, upto=5), late.process(raw, upto=5)
+        nt = b["nTuples"]
+        assert a["nTuples"] == nt and np.array_equal(a["tupleBins"], b["tupleBins"])
+        ka, kb = a["cellDoubletId"] < 0, b["cellDoubletId"] < 0
+        assert not ka.any() and kb.any(), (ka.sum(), kb.sum())
+        toff, tr = b["cellTrackOff"], b["cellTracks"]
+        with_tracks = np.diff(toff) > 0
+        assert not (kb & ~with_tracks).any()
+        used = b["cellUsed"].astype(bool)
+        _check_fishbone_kills(b, used & with_tracks, kb, f"late pu50[{idx}]")
+        bad = set()
+        for c in np.nonzero(kb)[0]:
+            bad |= set(tr[toff[c]:toff[c + 1]].tolist())
+        assert bad and (b["quality"][sorted(bad)] == 0).all()
+        # the duplicate removers then run without the bad tuples, so others may change too
+        changed = np.nonzero(a["quality"][:nt] != b["quality"][:nt])[0]
+        assert len(set(changed.tolist()) - bad) < len(bad)
+        e, eb = early.process(raw, upto=5), both.process(raw, upto=5)
+        for k in ("cellDoubletId", "quality", "tupleBins"):
+            assert np.array_equal(e[k], eb[k]), k
