@@ -286,6 +286,10 @@ int pxg_set_vertex_algorithm(pxg_stream* s, int algo);
  * cells, :133-136 after it on the cells in tuples, with kernel_fishboneCleaner after the
  * classification at :161-164; CAHitNtupletGeneratorOnGPU.cc:55-56 sets early on, late off). */
 int pxg_set_fishbone(pxg_stream* s, int early, int late);
+/* Helix fit of the stream's ntuplet stage: the reference's useRiemannFit switch
+ * (CAHitNtupletGeneratorOnGPU.cc:131-135 and :168-172, RiemannFitOnGPU.cc:3-113; default off,
+ * CAHitNtupletGeneratorOnGPU.cc:52): nonzero fits with the Riemann fit, zero with the broken line. */
+int pxg_set_riemann_fit(pxg_stream* s, int riemann);
 int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
 /* Times the chain on resident events [first, first+count) for each launch shape of a fixed
  * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */

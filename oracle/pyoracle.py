@@ -42,6 +42,15 @@ def _load(path, prefix):
             lib.orc_fit_helix.restype = C.c_int
             lib.orc_fit_helix.argtypes = [C.c_int, vp, vp, C.c_double, vp, vp, vp, vp, vp]
             lib.orc_invert_spd.argtypes = [vp, vp, C.c_int]
+            lib.orc_fit_helix_riemann.restype = C.c_int
+            lib.orc_fit_helix_riemann.argtypes = [C.c_int, vp, vp, C.c_double, vp, vp, vp, vp, vp]
+            lib.orc_circle_fit_riemann.restype = C.c_int
+            lib.orc_circle_fit_riemann.argtypes = [C.c_int, vp, vp, C.c_double, vp, vp, vp, vp, vp]
+            lib.orc_eigen3.argtypes = [vp, C.c_int, vp, vp]
+            lib.orc_rmath.restype = C.c_double
+            lib.orc_rmath.argtypes = [C.c_int, C.c_double, C.c_double]
+            lib.orc_conditions_set_riemann.restype = C.c_int
+            lib.orc_conditions_set_riemann.argtypes = [vp, C.c_int]
             lib.orc_vertices.restype = C.c_int
             lib.orc_vertices.argtypes = [C.c_uint32] + [vp] * 10
             lib.orc_vertices_algo.restype = C.c_int
@@ -138,9 +147,11 @@ class Oracle(_Chain):
     # vertex clustering algorithms (gpuVertexFinder.h:46-60)
     VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2
 
-    def __init__(self, conditions, profile=0, vertex_algo=0, early_fishbone=True, late_fishbone=False):
+    def __init__(self, conditions, profile=0, vertex_algo=0, early_fishbone=True, late_fishbone=False,
+                 riemann_fit=False):
         super().__init__(conditions)
         self.lib.orc_conditions_set_fishbone(self._h, int(early_fishbone), int(late_fishbone))
+        self.lib.orc_conditions_set_riemann(self._h, int(riemann_fit))
         if self.lib.orc_conditions_set_profile(self._h, profile) != 0:
             raise ValueError(f"unknown capacity profile {profile}")
         if self.lib.orc_conditions_set_vertex_algo(self._h, vertex_algo) != 0:
@@ -160,6 +171,49 @@ class Oracle(_Chain):
         if rc != 0:
             raise ValueError(f"fit_helix: unsupported number of hits {n}")
         return st, cv, float(sc[0]), float(sc[1]), float(sc[2])
+
+    def fit_helix_riemann(self, hits, ge, B):
+        """Riemann fit of one track, same arguments and results as fit_helix."""
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ge = np.ascontiguousarray(ge, dtype=np.float32)
+        n = hits.shape[1]
+        st = np.zeros(5, np.float32)
+        cv = np.zeros(15, np.float32)
+        sc = np.zeros(3, np.float32)
+        a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        rc = self.lib.orc_fit_helix_riemann(n, a(hits), a(ge), B, a(st), a(cv), a(sc[0:]), a(sc[1:]), a(sc[2:]))
+        if rc != 0:
+            raise ValueError(f"fit_helix_riemann: unsupported number of hits {n}")
+        return st, cv, float(sc[0]), float(sc[1]), float(sc[2])
+
+    def circle_fit_riemann(self, hits, ge, B):
+        """The Riemann circle fit alone: fast fit (X0, Y0, R, tan theta), circle (X0, Y0, R),
+        its 3x3 covariance, charge and chi2."""
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ge = np.ascontiguousarray(ge, dtype=np.float32)
+        n = hits.shape[1]
+        fast, par, cov = np.zeros(4), np.zeros(3), np.zeros((3, 3))
+        q = np.zeros(1, np.int32)
+        chi2 = np.zeros(1, np.float32)
+        a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        if self.lib.orc_circle_fit_riemann(n, a(hits), a(ge), B, a(fast), a(par), a(cov), a(q), a(chi2)) != 0:
+            raise ValueError(f"circle_fit_riemann: unsupported number of hits {n}")
+        return fast, par, cov, int(q[0]), float(chi2[0])
+
+    def eigen3(self, m, use_float=False):
+        """Closed-form eigen decomposition of a symmetric 3x3 (lower triangle read):
+        ascending eigenvalues and the eigenvectors as columns."""
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        ev, vec = np.zeros(3), np.zeros((3, 3))
+        a = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.lib.orc_eigen3(a(m), int(use_float), a(ev), a(vec))
+        return ev, vec
+
+    def rmath(self, fn, x, y=0.0):
+        """The Riemann fit's own elementary functions: fn 'atan2' (of y, x), 'atan', 'sin',
+        'cos' or 'log'."""
+        code = {"atan2": 0, "atan": 1, "sin": 2, "cos": 3, "log": 4}[fn]
+        return self.lib.orc_rmath(code, float(x), float(y))
 
     def invert_spd(self, m):
         m = np.ascontiguousarray(m, dtype=np.float64)

@@ -44,7 +44,8 @@ namespace orc {
     uint32_t nFeds;
     CACapacities caps = kCapsReference;
     int vertexAlgo = 0;  // 0 density, 1 DBSCAN, 2 iterative (gpuVertexFinder.h:46-60)
-    bool earlyFishbone = true, lateFishbone = false;  // CAHitNtupletGeneratorOnGPU.cc:51-52
+    bool earlyFishbone = true, lateFishbone = false;  // CAHitNtupletGeneratorOnGPU.cc:55-56
+    bool riemannFit = false;                          // CAHitNtupletGeneratorOnGPU.cc:52 useRiemannFit
   };
 
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
@@ -170,6 +171,14 @@ namespace orc {
   int fitHelix(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt, float* eta,
                float* chi2);
   void invertSPDForTest(const double* a, double* inv, int n);
+  // a18/a19 alternative: the Riemann fit (orc_riemann.cc)
+  void fitTracksRiemann(const Conditions& c, Event& ev);
+  int fitHelixRiemann(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt,
+                      float* eta, float* chi2);
+  int circleFitRiemann(int n, const double* hits, const float* ge, double B, double* fast, double* par, double* cov,
+                       int* q, float* chi2);
+  void eigen3ForTest(const double* a, int useFloat, double* evals, double* evecs);
+  double rmathForTest(int fn, double x, double y);
   // a23-a25 (orc_stageD.cc)
   void findVertices(Event& ev);
   void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,

@@ -136,7 +136,6 @@ orc_conditions* orc_conditions_create(const void* cabling, const uint8_t* modToU
 
 void orc_conditions_destroy(orc_conditions* h) { delete h; }
 
-// capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
 // fishbone passes of the chain (the reference runs the early one only)
 int orc_conditions_set_fishbone(orc_conditions* h, int early, int late) {
   h->c.earlyFishbone = early != 0;
@@ -144,6 +143,13 @@ int orc_conditions_set_fishbone(orc_conditions* h, int early, int late) {
   return 0;
 }
 
+// helix fit of the chain: 0 broken line (the reference's default), 1 Riemann
+int orc_conditions_set_riemann(orc_conditions* h, int riemann) {
+  h->c.riemannFit = riemann != 0;
+  return 0;
+}
+
+// capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
 // vertex clustering algorithm of the chain: 0 density (the reference's default), 1 DBSCAN, 2 iterative
 int orc_conditions_set_vertex_algo(orc_conditions* h, int algo) {
   if (algo < 0 || algo > 2)
@@ -274,6 +280,22 @@ int orc_fit_helix(int n, const double* hits, const float* ge, double B, float* s
 }
 
 void orc_invert_spd(const double* a, double* inv, int n) { invertSPDForTest(a, inv, n); }
+
+int orc_fit_helix_riemann(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt,
+                          float* eta, float* chi2) {
+  return fitHelixRiemann(n, hits, ge, B, state, cov, pt, eta, chi2);
+}
+
+int orc_circle_fit_riemann(int n, const double* hits, const float* ge, double B, double* fast, double* par,
+                           double* cov, int* q, float* chi2) {
+  return circleFitRiemann(n, hits, ge, B, fast, par, cov, q, chi2);
+}
+
+double orc_rmath(int fn, double x, double y) { return rmathForTest(fn, x, y); }
+
+void orc_eigen3(const double* a, int use_float, double* evals, double* evecs) {
+  eigen3ForTest(a, use_float, evals, evecs);
+}
 
 // vertex finder on given track z, z error^2 and pt^2 (all tracks used)
 int orc_vertices(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv, float* wv,

@@ -321,7 +321,10 @@ namespace orc {
     if (ev.nHits > 1 && ev.lateFishbone)
       fishbone(ev, true);
     fillHitDetIndices(ev);
-    fitTracks(c, ev);
+    if (c.riemannFit)  // CAHitNtupletGeneratorOnGPU.cc:168-172
+      fitTracksRiemann(c, ev);
+    else
+      fitTracks(c, ev);
     if (ev.nHits > 0) {
       classifyTracks(ev);
       if (ev.lateFishbone)  // CAHitNtupletGeneratorKernels.cc:161-164

@@ -337,7 +337,8 @@ namespace pxg::detail {
     if (s->profiling)
       timer = {timerBegin, timerEnd, s};
     const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
-                     s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone};
+                     s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone,
+                     s->riemannFit};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -348,7 +349,10 @@ namespace pxg::detail {
       launchMultiplicity(l);
       if (l.lateFishbone)
         launchFishbone(l, true);
-      launchFit(l);
+      if (l.riemannFit)
+        launchRiemannFit(l);
+      else
+        launchFit(l);
       launchClassify(l);
       launchHitToTuple(l);
     }
@@ -788,6 +792,13 @@ int pxg_set_fishbone(pxg_stream* s, int early, int late) {
     return fail(PXG_ERR_ARG, "pxg_set_fishbone: null stream");
   s->earlyFishbone = early != 0;
   s->lateFishbone = late != 0;
+  return 0;
+}
+
+int pxg_set_riemann_fit(pxg_stream* s, int riemann) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_set_riemann_fit: null stream");
+  s->riemannFit = riemann != 0;
   return 0;
 }
 

@@ -248,6 +248,7 @@ namespace pxg {
     int vertexAlgo;         // vertex clustering: kVertexDensity, kVertexDBSCAN or kVertexIterative
     bool earlyFishbone;     // fishbone on the connected cells before the ntuplet search
     bool lateFishbone;      // fishbone on the cells in tuples after it, and the cleaner after classify
+    bool riemannFit;        // Riemann fit in place of the broken-line fit
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);
@@ -256,6 +257,7 @@ namespace pxg {
   void launchNtuplets(const CALaunch& l);
   void launchMultiplicity(const CALaunch& l);
   void launchFit(const CALaunch& l);
+  void launchRiemannFit(const CALaunch& l);
   void launchClassify(const CALaunch& l);
   void launchHitToTuple(const CALaunch& l);
   void launchVertices(const CALaunch& l);

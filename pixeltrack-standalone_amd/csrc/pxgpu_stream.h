@@ -39,6 +39,7 @@ struct pxg_stream {
   int vertexAlgo = 0;          // vertex clustering algorithm (PXG_VERTEX_*)
   bool earlyFishbone = true;   // the reference's defaults: early fishbone on, late off
   bool lateFishbone = false;
+  bool riemannFit = false;     // the reference's default: broken-line fit
   std::vector<ProfileMark> marks;
   std::map<std::string, std::pair<uint32_t, double>> profile;
 };

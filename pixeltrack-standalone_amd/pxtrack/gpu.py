@@ -113,6 +113,7 @@ def _gpu():
         lib.pxg_set_tuning.argtypes = [vp, u32, u32]
         lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
         lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
+        lib.pxg_set_riemann_fit.argtypes = [vp, C.c_int]
         lib.pxg_get_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         lib.pxg_autotune.argtypes = [vp, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(C.c_float)]
@@ -434,6 +435,10 @@ class Stream:
         """The fishbone passes: early (before the ntuplet search, default on) and late
         (on the cells in tuples, with the cleaner after classification, default off)."""
         _check(self.lib.pxg_set_fishbone(self._h, int(early), int(late)), "pxg_set_fishbone")
+
+    def set_riemann_fit(self, on=True):
+        """Riemann fit in place of the broken-line fit (default off)."""
+        _check(self.lib.pxg_set_riemann_fit(self._h, int(on)), "pxg_set_riemann_fit")
 
     def tuning(self):
         a, b = u32(), u32()

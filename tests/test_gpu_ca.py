@@ -164,6 +164,21 @@ def test_alternative_vertex_clustering(detector, conditions, gpu_ctx, algo):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw, internals=False), f"algo {algo} pu50[{idx}]", internals=False)
 
 
+def test_riemann_fit(detector, conditions, gpu_ctx):
+    """The reference's useRiemannFit switch (CAHitNtupletGeneratorOnGPU.cc:168-172,
+    RiemannFitOnGPU.cc:3-113) on the whole chain against the oracle: fitted
+    parameters and covariances within the fits' tolerance, everything else exact."""
+    import pyoracle
+    orc = pyoracle.Oracle(conditions, riemann_fit=True)
+    s = gpu_ctx.stream(max_events=1, max_words=300000)
+    s.set_riemann_fit(True)
+    for idx in (0, 3):
+        raw, _ = event(detector, "pu50", idx)
+        o = orc.process(raw, upto=5)
+        print(f"riemann pu50[{idx}] tuples={o['nTuples']} vertices={o['nVertices']}", flush=True)
+        compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"riemann pu50[{idx}]")
+
+
 @pytest.mark.parametrize("early,late", [(False, True), (True, True), (False, False)])
 def test_fishbone_switches(detector, conditions, gpu_ctx, early, late):
     """The reference's earlyFishbone / lateFishbone switches
