@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--mix-events", type=int, default=200)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
+    ap.add_argument("--riemann-fit", action="store_true",
+                    help="fit with the Riemann fit instead of the broken line (batched path)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -234,6 +236,8 @@ def main():
     for k in range(S):
         part = inputs[bounds[k]:bounds[k + 1]]
         st = ctx.stream(max_events=len(part), max_words=int(sum(len(w) for w, _ in part)))
+        if args.riemann_fit:
+            st.set_riemann_fit(True)
         st.load(part)
         streams.append(st)
     stream = streams[0]
@@ -410,7 +414,8 @@ def main():
         "dtype": "int32/f32/f64",
         "data": "synthetic",
         "config": {"workload": f"{n} PU50 events ({STAGES[args.upto]}), batched path, {B} events per launch, "
-                               f"{S} streams",
+                               f"{S} streams" + (", Riemann fit" if args.riemann_fit else ""),
+                   "fit": "riemann" if args.riemann_fit else "broken line",
                    "events": n, "batch": B, "streams": S, "mean_words": round(float(W.mean()), 1),
                    "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
                    "stages": STAGES[args.upto]},

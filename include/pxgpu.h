@@ -290,6 +290,30 @@ int pxg_set_fishbone(pxg_stream* s, int early, int late);
  * (CAHitNtupletGeneratorOnGPU.cc:131-135 and :168-172, RiemannFitOnGPU.cc:3-113; default off,
  * CAHitNtupletGeneratorOnGPU.cc:52): nonzero fits with the Riemann fit, zero with the broken line. */
 int pxg_set_riemann_fit(pxg_stream* s, int riemann);
+
+/* The ntuplet generator's statistics: the reference's doStats switch (default off,
+ * CAHitNtupletGeneratorOnGPU.cc:58) and its Counters (CAHitNtupletGeneratorKernels.h:14-26),
+ * accumulated over every event the stream processes while on (kernel_checkOverflows,
+ * kernel_doStatsForTracks, kernel_doStatsForHitInTracks in CAHitNtupletGeneratorKernelsImpl.h).
+ * pxg_print_counters prints them as kernel_printCounters does (:577-608). */
+enum {
+  PXG_COUNTER_EVENTS = 0,
+  PXG_COUNTER_HITS,
+  PXG_COUNTER_CELLS,
+  PXG_COUNTER_TUPLES,
+  PXG_COUNTER_FIT_TRACKS,
+  PXG_COUNTER_GOOD_TRACKS,
+  PXG_COUNTER_USED_HITS,
+  PXG_COUNTER_DUP_HITS,
+  PXG_COUNTER_KILLED_CELLS,
+  PXG_COUNTER_EMPTY_CELLS,
+  PXG_COUNTER_ZERO_TRACK_CELLS,
+  PXG_N_COUNTERS
+};
+int pxg_set_stats(pxg_stream* s, int on);
+int pxg_get_counters(pxg_stream* s, uint64_t* out); /* PXG_N_COUNTERS values */
+int pxg_reset_counters(pxg_stream* s);
+int pxg_print_counters(pxg_stream* s);
 int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
 /* Times the chain on resident events [first, first+count) for each launch shape of a fixed
  * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */

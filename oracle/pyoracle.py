@@ -134,7 +134,7 @@ FIELDS = ["words", "fedIds", "xx", "yy", "adc", "moduleInd", "clus", "pdigi", "r
 CA_FIELDS = ["cellInner", "cellOuter", "cellDoubletId", "cellLayerPair", "cellUsed", "cellNeighborOff",
              "cellNeighbors", "cellTrackOff", "cellTracks", "outerHitOff", "outerHitCells", "tupleOff", "tupleBins",
              "quality", "multOff", "multBins", "detIndices", "hitToTupleOff", "hitToTupleBins", "state",
-             "covariance", "pt", "eta", "chi2", "idv", "zv", "wv", "chi2v", "ptv2", "ndof", "sortInd"]
+             "covariance", "pt", "eta", "chi2", "idv", "zv", "wv", "chi2v", "ptv2", "ndof", "sortInd", "stats"]
 FIELDS = FIELDS + CA_FIELDS
 AB_SCALARS = ["nWords", "nModules", "nDigis", "nClusters", "nHits", "fedCounter"]
 SCALARS = AB_SCALARS + ["nCells", "nTuples", "caStatus", "nVertices", "vtxTracks", "nNeighborLists", "nTrackLists"]

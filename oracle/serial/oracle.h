@@ -95,6 +95,10 @@ namespace orc {
     std::vector<std::array<float, 5>> state;
     std::vector<std::array<float, 15>> cov;
     std::vector<float> pt, eta, chi2;
+    // the event's contribution to the generator's counters (CAHitNtupletGeneratorKernels.h:14-26):
+    // nEvents, nHits, nCells, nTuples, nFitTracks, nGoodTracks, nUsedHits, nDupHits,
+    // nKilledCells, nEmptyCells, nZeroTrackCells
+    std::array<uint32_t, 11> stats{};
   };
 
   struct VertexState {
@@ -165,6 +169,7 @@ namespace orc {
   void classifyTracks(Event& ev);
   void fastDuplicateRemover(Event& ev);
   void hitToTupleAndTripletCleaner(Event& ev);
+  void countStats(Event& ev);
   void circleForTest(float x1, float y1, float x2, float y2, float x3, float y3, float* curvature, float* dca);
   // a18-a19 (orc_fit.cc)
   void fitTracks(const Conditions& c, Event& ev);

@@ -85,6 +85,8 @@ namespace {
                                                                                               C.tupleOff.empty() ? 0 : C.tupleOff.back()));
     exportVec(e, "tupleBins", bins, 2);
     exportVec(e, "quality", C.quality, 1);
+    std::vector<uint32_t> stats(C.stats.begin(), C.stats.end());
+    exportVec(e, "stats", stats, 4);
     exportVec(e, "multOff", C.multOff, 4);
     exportVec(e, "multBins", C.multBins, 2);
     std::vector<uint16_t> det(C.detIndices.begin(), C.detIndices.begin() + bins.size());

@@ -331,6 +331,7 @@ namespace orc {
         fishboneCleaner(ev);
       fastDuplicateRemover(ev);
       hitToTupleAndTripletCleaner(ev);
+      countStats(ev);
     }
     if (upto < 5)
       return true;

@@ -557,4 +557,36 @@ namespace orc {
     }
   }
 
+  // doStats counters of one event: kernel_checkOverflows (CAHitNtupletGeneratorKernelsImpl.h:30-104,
+  // called after the late fishbone, while the hit -> tuple map is still empty, so that every
+  // cell counts as a zero-track cell), kernel_doStatsForTracks (:410-421) and
+  // kernel_doStatsForHitInTracks (:468-478) after the cleaners
+  void countStats(Event& ev) {
+    auto& C = ev.ca;
+    auto& s = C.stats;
+    s.fill(0);
+    s[0] = 1;
+    s[1] = ev.nHits;
+    s[2] = C.nCells;
+    s[3] = C.apcM;
+    s[4] = C.multOff.empty() ? 0 : C.multOff[8];
+    for (uint32_t t = 0; t < C.nTuples; ++t) {
+      if (C.tupleOff[t + 1] == C.tupleOff[t])
+        break;
+      if (C.quality[t] == kQualityLoose)
+        ++s[5];
+    }
+    if (!C.hitToTupleOff.empty())
+      for (size_t h = 0; h + 1 < C.hitToTupleOff.size(); ++h) {
+        const uint32_t n = C.hitToTupleOff[h + 1] - C.hitToTupleOff[h];
+        s[6] += n > 0;
+        s[7] += n > 1;
+      }
+    for (uint32_t c = 0; c < C.nCells; ++c) {
+      s[8] += C.cells[c].doubletId < 0;
+      s[9] += C.cells[c].used == 0;
+    }
+    s[10] = C.nCells;
+  }
+
 }  // namespace orc

@@ -276,6 +276,52 @@ namespace pxg {
       }
     }
 
+    // doStats counters (CAHitNtupletGeneratorKernels.h:14-26): kernel_checkOverflows
+    // per-event and per-cell counts (CAHitNtupletGeneratorKernelsImpl.h:30-104; it runs
+    // while the hit -> tuple map is still empty, so every cell is a zero-track cell),
+    // kernel_doStatsForTracks (:410-421) and kernel_doStatsForHitInTracks (:468-478).
+    // Counters accumulate over the stream's events; each wave adds its partial sums.
+    __global__ void __launch_bounds__(kCaBlock) statsKernel(Pool p, CAPool ca, unsigned long long* counters) {
+      const uint32_t e = logicalBlock().y;
+      const CACounts cnt = ca.cnt[e];
+      const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
+      const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
+      const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      const uint32_t* toff = ca.tupleOff + ca.offBase(e);
+      const uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
+      uint32_t good = 0, usedHits = 0, dupHits = 0, killed = 0, empty = 0;
+      for (uint32_t c = gridStart(); c < cnt.nCells; c += gridStride()) {
+        killed += (ca.cellFlags[cb + c] & 2) != 0;
+        empty += ((used[c >> 5] >> (c & 31)) & 1) == 0;
+      }
+      // the serial loop stops at the first empty tuple: only the last stored one can be
+      for (uint32_t t = gridStart(); t < cnt.nTuples; t += gridStride())
+        good += toff[t + 1] > toff[t] && ca.quality[tb + t] == kQualityLoose;
+      for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
+        const uint32_t n = hoff[h + 1] - hoff[h];
+        usedHits += n > 0;
+        dupHits += n > 1;
+      }
+      const uint32_t v[5] = {good, usedHits, dupHits, killed, empty};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        uint32_t x = v[k];
+        for (int o = 32; o > 0; o >>= 1)
+          x += __shfl_xor(x, o);
+        if ((threadIdx.x & 63) == 0 && x)
+          atomicAdd(&counters[5 + k], (unsigned long long)x);
+      }
+      if (logicalBlock().x == 0 && threadIdx.x == 0) {
+        atomicAdd(&counters[0], 1ull);
+        atomicAdd(&counters[1], (unsigned long long)nHits);
+        atomicAdd(&counters[2], (unsigned long long)cnt.nCells);
+        atomicAdd(&counters[3], (unsigned long long)cnt.apcM);
+        atomicAdd(&counters[4], (unsigned long long)cnt.nMult);
+        atomicAdd(&counters[10], (unsigned long long)cnt.nCells);
+      }
+    }
+
     // a22 hit -> tuple association of the loose tuples
     __global__ void __launch_bounds__(kCaBlock) hitToTupleCountKernel(CAPool ca) {
       const uint32_t e = logicalBlock().y;
@@ -423,6 +469,11 @@ namespace pxg {
     l.t->b("tripletCleaner");
     tripletCleanerKernel<<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca);
     l.t->e();
+    if (l.counters) {
+      l.t->b("stats");
+      statsKernel<<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca, l.counters);
+      l.t->e();
+    }
   }
 
 }  // namespace pxg

@@ -338,7 +338,7 @@ namespace pxg::detail {
       timer = {timerBegin, timerEnd, s};
     const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
                      s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone,
-                     s->riemannFit};
+                     s->riemannFit, s->doStats ? s->counters : nullptr};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -553,6 +553,10 @@ int pxg_stream_create_profile(pxg_context* ctx, uint32_t maxEvents, uint32_t max
     e = hipMemsetAsync(p.counts, 0, E * sizeof(EventCounts), s->stream);
   if (e == hipSuccess)
     e = hipMemsetAsync(ca.cnt, 0, E * sizeof(CACounts), s->stream);
+  if (e == hipSuccess)
+    e = devAlloc(s->counters, PXG_N_COUNTERS, o);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(s->counters, 0, PXG_N_COUNTERS * sizeof(unsigned long long), s->stream);
   if (e == hipSuccess)
     e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
@@ -799,6 +803,51 @@ int pxg_set_riemann_fit(pxg_stream* s, int riemann) {
   if (!s)
     return fail(PXG_ERR_ARG, "pxg_set_riemann_fit: null stream");
   s->riemannFit = riemann != 0;
+  return 0;
+}
+
+int pxg_set_stats(pxg_stream* s, int on) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_set_stats: null stream");
+  s->doStats = on != 0;
+  return 0;
+}
+
+int pxg_get_counters(pxg_stream* s, uint64_t* out) {
+  if (!s || !out)
+    return fail(PXG_ERR_ARG, "pxg_get_counters: null argument");
+  static_assert(sizeof(unsigned long long) == sizeof(uint64_t));
+  PXG_HIP(hipMemcpyAsync(out, s->counters, PXG_N_COUNTERS * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int pxg_reset_counters(pxg_stream* s) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_reset_counters: null stream");
+  PXG_HIP(hipMemsetAsync(s->counters, 0, PXG_N_COUNTERS * sizeof(uint64_t), s->stream));
+  PXG_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int pxg_print_counters(pxg_stream* s) {
+  uint64_t c[PXG_N_COUNTERS];
+  if (int rc = pxg_get_counters(s, c))
+    return rc;
+  // kernel_printCounters (CAHitNtupletGeneratorKernelsImpl.h:577-608), including its order of
+  // the raw nGoodTracks and nFitTracks columns
+  std::printf(
+      "||Counters | nEvents | nHits | nCells | nTuples | nFitTacks  |  nGoodTracks | nUsedHits | nDupHits | "
+      "nKilledCells | nEmptyCells | nZeroTrackCells ||\n");
+  std::printf("Counters Raw %llu %llu %llu %llu %llu %llu %llu %llu %llu %llu %llu\n", (unsigned long long)c[0],
+              (unsigned long long)c[1], (unsigned long long)c[2], (unsigned long long)c[3], (unsigned long long)c[5],
+              (unsigned long long)c[4], (unsigned long long)c[6], (unsigned long long)c[7], (unsigned long long)c[8],
+              (unsigned long long)c[9], (unsigned long long)c[10]);
+  const double ne = double(c[0]), nc = double(c[2]);
+  std::printf("Counters Norm %llu ||  %.1f|  %.1f|  %.1f|  %.1f|  %.1f|  %.1f|  %.1f|  %.1f|  %.3f|  %.3f||\n",
+              (unsigned long long)c[0], c[1] / ne, c[2] / ne, c[3] / ne, c[4] / ne, c[5] / ne, c[6] / ne, c[7] / ne,
+              c[8] / ne, c[9] / nc, c[10] / nc);
+  std::fflush(stdout);
   return 0;
 }
 

@@ -249,6 +249,7 @@ namespace pxg {
     bool earlyFishbone;     // fishbone on the connected cells before the ntuplet search
     bool lateFishbone;      // fishbone on the cells in tuples after it, and the cleaner after classify
     bool riemannFit;        // Riemann fit in place of the broken-line fit
+    unsigned long long* counters;  // doStats counters (11), null when off
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);

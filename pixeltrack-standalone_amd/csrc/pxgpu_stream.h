@@ -40,6 +40,8 @@ struct pxg_stream {
   bool earlyFishbone = true;   // the reference's defaults: early fishbone on, late off
   bool lateFishbone = false;
   bool riemannFit = false;     // the reference's default: broken-line fit
+  bool doStats = false;        // the reference's default: no counters
+  unsigned long long* counters = nullptr;  // device, PXG_N_COUNTERS
   std::vector<ProfileMark> marks;
   std::map<std::string, std::pair<uint32_t, double>> profile;
 };

@@ -114,6 +114,10 @@ def _gpu():
         lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
         lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
         lib.pxg_set_riemann_fit.argtypes = [vp, C.c_int]
+        lib.pxg_set_stats.argtypes = [vp, C.c_int]
+        lib.pxg_get_counters.argtypes = [vp, C.POINTER(C.c_uint64)]
+        lib.pxg_reset_counters.argtypes = [vp]
+        lib.pxg_print_counters.argtypes = [vp]
         lib.pxg_get_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         lib.pxg_autotune.argtypes = [vp, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(C.c_float)]
@@ -435,6 +439,24 @@ class Stream:
         """The fishbone passes: early (before the ntuplet search, default on) and late
         (on the cells in tuples, with the cleaner after classification, default off)."""
         _check(self.lib.pxg_set_fishbone(self._h, int(early), int(late)), "pxg_set_fishbone")
+
+    COUNTERS = ("nEvents", "nHits", "nCells", "nTuples", "nFitTracks", "nGoodTracks", "nUsedHits", "nDupHits",
+                "nKilledCells", "nEmptyCells", "nZeroTrackCells")
+
+    def set_stats(self, on=True):
+        """Accumulate the ntuplet generator's counters (the reference's doStats)."""
+        _check(self.lib.pxg_set_stats(self._h, int(on)), "pxg_set_stats")
+
+    def counters(self):
+        out = (C.c_uint64 * len(self.COUNTERS))()
+        _check(self.lib.pxg_get_counters(self._h, out), "pxg_get_counters")
+        return dict(zip(self.COUNTERS, (int(x) for x in out)))
+
+    def reset_counters(self):
+        _check(self.lib.pxg_reset_counters(self._h), "pxg_reset_counters")
+
+    def print_counters(self):
+        _check(self.lib.pxg_print_counters(self._h), "pxg_print_counters")
 
     def set_riemann_fit(self, on=True):
         """Riemann fit in place of the broken-line fit (default off)."""

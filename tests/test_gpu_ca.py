@@ -196,6 +196,33 @@ def test_fishbone_switches(detector, conditions, gpu_ctx, early, late):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"fishbone {early}/{late} pu50[{idx}]")
 
 
+def test_stats_counters(detector, oracle, gpu_ctx):
+    """doStats (CAHitNtupletGeneratorKernels.h:14-26): the counters the GPU accumulates
+    over a batch equal the oracle's per-event counts summed; off by default, a second
+    pass adds the same again, and a reset clears them."""
+    from pxtrack import gpu
+    evs = [event(detector, "pu20_80", 30 + i)[0] for i in range(4)]
+    unpacked = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in evs]
+    s = gpu_ctx.stream(max_events=4, max_words=sum(w.size for w, _ in unpacked) + 1024)
+    s.load(unpacked)
+    s.run(0, 4, upto=5)
+    s.fetch_counts(0, 4)
+    assert all(v == 0 for v in s.counters().values())
+    s.set_stats(True)
+    s.run(0, 4, upto=5)
+    s.fetch_counts(0, 4)
+    exp = np.sum([oracle.process(r, upto=5)["stats"].astype(np.int64) for r in evs], axis=0)
+    got = s.counters()
+    print("counters", got, flush=True)
+    assert [got[k] for k in s.COUNTERS] == exp.tolist()
+    s.run(0, 4, upto=5)
+    s.fetch_counts(0, 4)
+    assert [s.counters()[k] for k in s.COUNTERS] == (2 * exp).tolist()
+    s.print_counters()
+    s.reset_counters()
+    assert all(v == 0 for v in s.counters().values())
+
+
 def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
     """Every launch shape the autotuner may pick gives the same products."""
     from pxtrack import gpu

@@ -605,3 +605,22 @@ def test_late_fishbone_and_cleaner(detector, conditions):
         e, eb = early.process(raw, upto=5), both.process(raw, upto=5)
         for k in ("cellDoubletId", "quality", "tupleBins"):
             assert np.array_equal(e[k], eb[k]), k
+
+
+def test_stats_counters(small_events):
+    """The doStats counters of one event (CAHitNtupletGeneratorKernelsImpl.h:30-104,
+    :410-421, :468-478) recounted from the event's products: loose tuples, hits in
+    one / several loose tuples, killed and unconnected cells; every cell counts as a
+    zero-track cell because the overflow check runs before the hit -> tuple fill."""
+    for name, o, _ in small_events:
+        s = o["stats"]
+        nt, nc = o["nTuples"], o["nCells"]
+        sizes = np.diff(o["tupleOff"][:nt + 1].astype(np.int64))
+        assert s[0] == 1 and s[1] == o["nHits"] and s[2] == nc, name
+        assert s[4] == o["multOff"][8], name  # entries of the multiplicity container
+        assert s[5] == np.sum((sizes > 0) & (o["quality"][:nt] == 2)), name
+        h2t = np.diff(o["hitToTupleOff"].astype(np.int64))
+        assert s[6] == np.sum(h2t > 0) and s[7] == np.sum(h2t > 1), name
+        assert s[8] == np.sum(o["cellDoubletId"][:nc] < 0), name
+        assert s[9] == np.sum(o["cellUsed"][:nc] == 0), name
+        assert s[10] == nc, name
