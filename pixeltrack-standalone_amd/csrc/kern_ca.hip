@@ -634,16 +634,20 @@ namespace pxg {
       float x, y, z, n;  // inner hit minus the list's hit, squared norm
     };
 
+    // lists that the wave's 64 positions touch, staged whole: at most 128 cells before
+    // the first position, the 64 positions and 128 cells after the last
+    constexpr uint32_t kFbWindow = 320;
+
     template <bool kCheckTrack>
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const HitView H = hitsOf(p, e);
       if (H.layerStart[10] < 2)
         return;
-      // each lane publishes its own cell; the lists that lie inside the
-      // wave's 64 positions read their cells from here instead of memory
-      __shared__ float4 sDir[kCaBlock];
-      __shared__ uint2 sCell[kCaBlock];
+      // per wave: every cell of the lists its positions belong to, as seen from its
+      // list's hit (direction, squared norm; cell, detInd | used << 16)
+      __shared__ float4 sDir[kCaBlock / 64][kFbWindow];
+      __shared__ uint2 sCell[kCaBlock / 64][kFbWindow];
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
@@ -653,103 +657,112 @@ namespace pxg {
       auto isUsed = [&](uint32_t c) {
         return ((used[c >> 5] >> (c & 31)) & 1) && (!kCheckTrack || trCount[c] > 0);
       };
-      const uint32_t lane = laneId(), wbase = threadIdx.x - lane;
+      // a list entry seen from its own list's hit
+      auto entry = [&](uint32_t k) {
+        RowCell d;
+        const uint2 rec = cells[k];
+        d.c = rec.x;
+        const uint32_t hj = rec.y & 0xffff;
+        const float4 ph = H.pos[rec.y >> 16];
+        const float4 pj = H.pos[hj];
+        d.x = pj.x - ph.x;
+        d.y = pj.y - ph.y;
+        d.z = pj.z - ph.z;
+        d.n = d.x * d.x + d.y * d.y + d.z * d.z;
+        d.det = uint32_t(H.detInd[hj]) | (isUsed(d.c) << 16);
+        return d;
+      };
+      const uint32_t lane = laneId(), wave = threadIdx.x / 64;
+      float4* wDir = sDir[wave];
+      uint2* wCell = sCell[wave];
       for (uint32_t pos0 = gridStart() - lane; pos0 < nCells; pos0 += gridStride()) {  // uniform per wave
         const uint32_t pos = pos0 + lane;
-        RowCell me{0, 0, 0.f, 0.f, 0.f, 0.f};
         uint32_t b = 0, n = 0;
-        float4 ph{0.f, 0.f, 0.f, 0.f};
         if (pos < nCells) {
-          const uint2 rec = cells[pos];
-          me.c = rec.x;
-          const uint32_t h = rec.y >> 16, hi = rec.y & 0xffff;
+          const uint32_t h = cells[pos].y >> 16;
           b = off[h];
           n = off[h + 1] - b;
-          ph = H.pos[h];
-          const float4 pi = H.pos[hi];
-          me.x = pi.x - ph.x;
-          me.y = pi.y - ph.y;
-          me.z = pi.z - ph.z;
-          me.n = me.x * me.x + me.y * me.y + me.z * me.z;
-          me.det = uint32_t(H.detInd[hi]) | (isUsed(me.c) << 16);
         }
-        sDir[threadIdx.x] = make_float4(me.x, me.y, me.z, me.n);
-        sCell[threadIdx.x] = make_uint2(me.c, me.det);
-        waveLdsSync();
-        const uint32_t ci = me.c;
-        if (pos < nCells && (me.det >> 16) && n >= 2) {
-          // with more than 128 cells only the 128 lowest indices take part
-          uint32_t limit = 0xffffffffu;
-          bool listed = true;
-          if (n > kCaMaxCellsPerHit) {
-            listed = listedInOuterHit(cells, b, n, ci);
-            for (uint32_t k = 0; listed && k < n; ++k) {
-              const uint32_t c = cells[b + k].x;
-              uint32_t rank = 0;
-              for (uint32_t m = 0; m < n; ++m)
-                rank += cells[b + m].x < c;
-              if (rank == kCaMaxCellsPerHit)
-                limit = c;
-            }
+        // the window: from the first list's start to the last list's end
+        const uint32_t last = min(63u, nCells - 1 - pos0);
+        const uint32_t wb = __shfl(b, 0);
+        const uint32_t we = __shfl(b + n, int(last));
+        const bool staged = we - wb <= kFbWindow;
+        if (staged) {
+          for (uint32_t k = lane; k < we - wb; k += 64) {
+            const RowCell d = entry(wb + k);
+            wDir[k] = make_float4(d.x, d.y, d.z, d.n);
+            wCell[k] = make_uint2(d.c, d.det);
           }
-          auto fromMemory = [&](uint32_t k) {
-            RowCell d;
-            const uint2 rec = cells[b + k];
-            d.c = rec.x;
-            const uint32_t hj = rec.y & 0xffff;
-            const float4 pj = H.pos[hj];
-            d.x = pj.x - ph.x;
-            d.y = pj.y - ph.y;
-            d.z = pj.z - ph.z;
-            d.n = d.x * d.x + d.y * d.y + d.z * d.z;
-            d.det = uint32_t(H.detInd[hj]) | (isUsed(d.c) << 16);
-            return d;
-          };
-          const uint32_t lo = wbase + (b - pos0);
-          auto fromLds = [&](uint32_t k) {
-            const float4 v = sDir[lo + k];
-            const uint2 c = sCell[lo + k];
-            return RowCell{c.x, c.y, v.x, v.y, v.z, v.w};
-          };
-          // one pass: the aligned later cells, and the first closer one among them
-          auto row = [&](auto get) {
-            uint32_t killer = limit;
-            uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
-            for (uint32_t k = 0; k < n; ++k) {
-              const RowCell d = get(k);
-              if (d.c <= ci || d.c >= limit || !(d.det >> 16))
-                continue;
-              const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
-              if (!((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n))
-                continue;
-              if (me.n > d.n && d.c < killer)
-                killer = d.c;
-              if (k < 64)
-                alignedMask |= 1ull << k;
+        }
+        waveLdsSync();
+        if (pos < nCells && n >= 2) {
+          const RowCell me = staged ? RowCell{wCell[pos - wb].x, wCell[pos - wb].y, wDir[pos - wb].x, wDir[pos - wb].y,
+                                              wDir[pos - wb].z, wDir[pos - wb].w}
+                                    : entry(pos);
+          const uint32_t ci = me.c;
+          if (me.det >> 16) {
+            // with more than 128 cells only the 128 lowest indices take part
+            uint32_t limit = 0xffffffffu;
+            bool listed = true;
+            if (n > kCaMaxCellsPerHit) {
+              listed = listedInOuterHit(cells, b, n, ci);
+              for (uint32_t k = 0; listed && k < n; ++k) {
+                const uint32_t c = cells[b + k].x;
+                uint32_t rank = 0;
+                for (uint32_t m = 0; m < n; ++m)
+                  rank += cells[b + m].x < c;
+                if (rank == kCaMaxCellsPerHit)
+                  limit = c;
+              }
             }
-            // the aligned cells before the killer are killed (several rows may
-            // kill a cell: they store the same byte)
-            for (uint64_t m = alignedMask; m; m &= m - 1) {
-              const uint32_t cj = get(__ffsll((unsigned long long)m) - 1).c;
-              if (cj < killer)
-                ca.cellFlags[cb + cj] |= 2;
+            auto fromMemory = [&](uint32_t k) { return entry(b + k); };
+            const uint32_t lo = b - wb;
+            auto fromLds = [&](uint32_t k) {
+              const float4 v = wDir[lo + k];
+              const uint2 c = wCell[lo + k];
+              return RowCell{c.x, c.y, v.x, v.y, v.z, v.w};
+            };
+            // one pass: the aligned later cells, and the first closer one among them
+            auto row = [&](auto get) {
+              uint32_t killer = limit;
+              uint64_t alignedMask = 0;  // list positions (lists of up to 64 cells)
+              for (uint32_t k = 0; k < n; ++k) {
+                const RowCell d = get(k);
+                if (d.c <= ci || d.c >= limit || !(d.det >> 16))
+                  continue;
+                const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
+                if (!((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n))
+                  continue;
+                if (me.n > d.n && d.c < killer)
+                  killer = d.c;
+                if (k < 64)
+                  alignedMask |= 1ull << k;
+              }
+              // the aligned cells before the killer are killed (several rows may
+              // kill a cell: they store the same byte)
+              for (uint64_t m = alignedMask; m; m &= m - 1) {
+                const uint32_t cj = get(__ffsll((unsigned long long)m) - 1).c;
+                if (cj < killer)
+                  ca.cellFlags[cb + cj] |= 2;
+              }
+              for (uint32_t k = 64; k < n; ++k) {  // positions past 64: second pass
+                const RowCell d = get(k);
+                if (d.c <= ci || d.c >= killer || !(d.det >> 16))
+                  continue;
+                const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
+                if ((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n)
+                  ca.cellFlags[cb + d.c] |= 2;
+              }
+              if (killer != limit)
+                ca.cellFlags[cb + ci] |= 2;
+            };
+            if (listed) {
+              if (staged)
+                row(fromLds);
+              else
+                row(fromMemory);
             }
-            for (uint32_t k = 64; k < n; ++k) {  // positions past 64: second pass
-              const RowCell d = get(k);
-              if (d.c <= ci || d.c >= killer || !(d.det >> 16))
-                continue;
-              const float cos12 = me.x * d.x + me.y * d.y + me.z * d.z;
-              if ((me.det & 0xffff) != (d.det & 0xffff) && cos12 * cos12 >= 0.99999f * me.n * d.n)
-                ca.cellFlags[cb + d.c] |= 2;
-            }
-            if (killer != limit)
-              ca.cellFlags[cb + ci] |= 2;
-          };
-          if (listed) {
-            if (n <= 64 && b >= pos0 && b + n <= pos0 + 64)
-              row(fromLds);
-            else
-              row(fromMemory);
           }
         }
         waveLdsSync();
