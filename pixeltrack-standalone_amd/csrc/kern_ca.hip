@@ -425,7 +425,7 @@ namespace pxg {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
-    constexpr uint32_t kStage = 320;  // staged candidate cells per wave
+    constexpr uint32_t kStage = 256;  // staged candidate cells per wave
     constexpr uint32_t kSegs = 64;    // staged candidate ranges per wave
 
     __device__ __forceinline__ uint32_t waveExclusiveSum(uint32_t v) {
