@@ -306,7 +306,7 @@ namespace pxg {
           ca.cellInner[cb + c] = uint16_t(q.i);
           ca.cellOuter[cb + c] = uint16_t(oi);
           ca.cellPair[cb + c] = uint8_t(q.pairId);
-          atomicAdd(&ohc[oi], 1u);
+          ca.ohRank[cb + c] = atomicAdd(&ohc[oi], 1u);
         };
         // the first 128 candidates: the count pass recorded their verdicts; the
         // r-th accepted one is the slot's r-th cell, written by lane r % G
@@ -349,23 +349,22 @@ namespace pxg {
         const uint32_t v = h < nHits ? cnt[h] : 0;
         uint32_t tot;
         const uint32_t ex = wgExclusiveScan<kEventBlock>(v, tot, lds);
-        if (h < nHits) {
+        if (h < nHits)
           off[h] = carry + ex;
-          cnt[h] = 0;  // fill cursor
-        }
         carry += tot;
       }
       if (threadIdx.x == 0)
         off[nHits] = carry;
     }
 
+    // each cell at its outer hit's offset plus the position the doublet fill's
+    // count gave it
     __global__ void __launch_bounds__(kCaBlock) ohFillKernel(CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const uint32_t nCells = ca.cnt[e].nCells;
       const size_t cb = ca.cellBase(e);
-      uint32_t* cur = ca.ohCount + size_t(e) * kHitStride;
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
-      // four cells per iteration: their atomics are issued together
+      // four cells per iteration: their loads are issued together
       const uint32_t stride = gridStride();
       for (uint32_t c0 = gridStart(); c0 < nCells; c0 += 4 * stride) {
         uint32_t h[4], hi[4], pos[4];
@@ -375,12 +374,9 @@ namespace pxg {
           if (c < nCells) {
             h[u] = ca.cellOuter[cb + c];
             hi[u] = ca.cellInner[cb + c];
+            pos[u] = ca.ohRank[cb + c];
           }
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (c0 + u * stride < nCells)
-            pos[u] = atomicAdd(&cur[h[u]], 1u);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t c = c0 + u * stride;

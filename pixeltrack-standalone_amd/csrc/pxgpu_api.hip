@@ -119,6 +119,7 @@ namespace pxg::detail {
     f(ca.ohCells, D);
     f(ca.startCells, D);
     f(ca.nbSlot, D);
+    f(ca.ohRank, D);
     f(ca.nbCount, D);
     f(ca.nbPool, L * kCaMaxNeighbors);
     f(ca.startMask, E * (ca.caps.maxDoublets / 64 + 64));
@@ -225,6 +226,7 @@ namespace pxg::detail {
     v.ohCells += D;
     v.startCells += D;
     v.nbSlot += D;
+    v.ohRank += D;
     v.nbCount += D;
     v.nbPool += L * kCaMaxNeighbors;
     v.startMask += size_t(first) * (c.caps.maxDoublets / 64 + 64);

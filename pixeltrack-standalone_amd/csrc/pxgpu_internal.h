@@ -176,6 +176,7 @@ namespace pxg {
     uint2* ohCells;          // [maxDoublets] cells by outer hit (fill order): x cell, y inner hit | outer hit << 16
     uint32_t* startCells;    // [maxDoublets] start cells of the ntuplet walk, ascending
     int32_t* nbSlot;         // [maxDoublets] neighbour list slot (-1 none, -2 store full)
+    uint32_t* ohRank;        // [maxDoublets] position of the cell in its outer hit's list
     uint8_t* nbCount;        // [maxDoublets] neighbours stored (<= 36)
     uint32_t* nbPool;        // [maxCellLists * 36]
     uint64_t* startMask;     // [maxDoublets / 64 + 64] start cells of the ntuplet walk
