@@ -967,19 +967,47 @@ namespace pxg {
       uint32_t* trCells = ca.dfsT + cb;
       uint32_t* trTotal = ca.dfsH + cb;
       uint32_t status = 0;
-      for (uint32_t c = gridStart(); c < cnt.nCells; c += gridStride()) {
-        const uint32_t n = ca.trCount[cb + c];
-        ca.trCount[cb + c] = 0;
-        const uint32_t s = waveAlloc(&ca.cnt[e].trLists, n > 0);
-        if (n == 0)
+      // four cells per lane and iteration, one counter update per wave for all four
+      const uint32_t stride = gridStride();
+      for (uint32_t c0 = gridStart(); c0 < cnt.nCells; c0 += 4 * stride) {
+        uint32_t n[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t c = c0 + u * stride;
+          n[u] = c < cnt.nCells ? ca.trCount[cb + c] : 0;
+        }
+        uint64_t m[4];
+        uint32_t total = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t c = c0 + u * stride;
+          if (c < cnt.nCells)
+            ca.trCount[cb + c] = 0;
+          m[u] = __ballot(n[u] > 0);
+          total += __popcll(m[u]);
+        }
+        if (total == 0)
           continue;
-        if (s + 1 < ca.caps.maxCellLists) {
-          ca.trSlot[cb + c] = int32_t(s);
-          trCells[s] = c;
-          trTotal[s] = n;
-        } else {
-          ca.trSlot[cb + c] = -2;
-          status |= kCaTrackStoreOverflow;
+        const uint32_t leader = __ffsll((unsigned long long)__ballot(true)) - 1;
+        uint32_t base = 0;
+        if (laneId() == leader)
+          base = atomicAdd(&ca.cnt[e].trLists, total);
+        base = __shfl(base, leader);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t s = base + __popcll(m[u] & lanesBelow());
+          base += __popcll(m[u]);
+          if (n[u] == 0)
+            continue;
+          const uint32_t c = c0 + u * stride;
+          if (s + 1 < ca.caps.maxCellLists) {
+            ca.trSlot[cb + c] = int32_t(s);
+            trCells[s] = c;
+            trTotal[s] = n[u];
+          } else {
+            ca.trSlot[cb + c] = -2;
+            status |= kCaTrackStoreOverflow;
+          }
         }
       }
       if (status)
