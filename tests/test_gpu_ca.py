@@ -149,6 +149,21 @@ def test_high_occupancy_profile(detector, conditions, gpu_ctx):
     compare_ca(o, _gpu_event(gpu_ctx, s, raw), "pu200/high-occupancy")
 
 
+def test_reference_capacities_overflow(detector, oracle, gpu_ctx):
+    """PU200 under the reference's capacities (CAConstants.h): more doublets than
+    maxNumberOfDoublets, more tuples than maxNumberOfTuples, a full multiplicity
+    container.  The products keep what the serial program keeps (its first cells and
+    tuples in serial order) and the status bits are the same."""
+    s = gpu_ctx.stream(max_events=1, max_words=400000)
+    for idx in (1, 2):
+        raw, _ = event(detector, "pu200", idx)
+        o = oracle.process(raw, upto=5)
+        print(f"pu200[{idx}] reference caps: cells={o['nCells']} tuples={o['nTuples']} status={o['caStatus']:#x}",
+              flush=True)
+        assert o["caStatus"] & 1  # the cell container overflowed
+        compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"pu200[{idx}]/reference")
+
+
 @pytest.mark.parametrize("algo", [1, 2])
 def test_alternative_vertex_clustering(detector, conditions, gpu_ctx, algo):
     """DBSCAN (1) and iterative (2) vertex clustering, the reference's off-by-default
