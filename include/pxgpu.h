@@ -129,7 +129,7 @@ typedef struct {
   uint32_t* outerHitOff;
   uint32_t* outerHitCells;  /* cells by outer hit in fill order, as u32 pairs (cell, inner hit | outer hit << 16) */
   int32_t* neighborSlot; uint8_t* neighborCount; uint32_t* neighborPool;  /* 36 per slot */
-  int32_t* trackSlot; uint32_t* trackCount; uint16_t* trackPool;          /* 48 per slot */
+  int32_t* trackSlot; uint16_t* trackCount; uint16_t* trackPool;          /* 48 per slot */
   uint32_t* multOff; uint16_t* multBins;   /* tuple multiplicity */
   uint32_t *hitToTupleOff; uint16_t* hitToTupleBins;
   uint32_t counts[16];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,

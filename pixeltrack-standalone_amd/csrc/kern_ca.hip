@@ -52,6 +52,14 @@ namespace pxg {
     // Counter allocation with one atomic per wave: the active lanes asking for
     // a slot get consecutive values (lane order).  Lanes that diverged away do
     // not take part, so it may be called in divergent code.
+    // increments the 16-bit counter i of a 4-byte aligned array through the 32-bit word
+    // holding it and returns its old value (no carry: the counts stay below 65536)
+    __device__ __forceinline__ uint32_t atomicIncHalf(uint16_t* base, uint32_t i) {
+      uint32_t* w = reinterpret_cast<uint32_t*>(base) + (i >> 1);
+      const uint32_t sh = (i & 1) * 16;
+      return (atomicAdd(w, 1u << sh) >> sh) & 0xffffu;
+    }
+
     __device__ __forceinline__ uint32_t waveAlloc(uint32_t* counter, bool need) {
       const uint64_t m = __ballot(need);
       if (!m)
@@ -649,7 +657,7 @@ namespace pxg {
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       const uint2* cells = ca.ohCells + cb;
       const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
-      const uint32_t* trCount = ca.trCount + cb;
+      const uint16_t* trCount = ca.trCount + cb;
       auto isUsed = [&](uint32_t c) {
         return ((used[c >> 5] >> (c & 31)) & 1) && (!kCheckTrack || trCount[c] > 0);
       };
@@ -819,7 +827,7 @@ namespace pxg {
               for (int k = 0; k < depth; ++k) {
                 bins[k] = ca.cellInner[cb + stack[k]];
                 ca.tupleCells[4 * tb + 4 * size_t(m) + k] = stack[k];
-                atomicAdd(&ca.trCount[cb + stack[k]], 1u);
+                atomicIncHalf(ca.trCount + cb, stack[k]);
               }
               bins[depth] = ca.cellOuter[cb + cell];
               for (int k = depth; k < 4; ++k)
@@ -1028,7 +1036,7 @@ namespace pxg {
           const int32_t slot = ca.trSlot[cb + c];
           if (slot < 0)
             continue;  // no list: the store is full
-          const uint32_t pos = atomicAdd(&ca.trCount[cb + c], 1u);
+          const uint32_t pos = atomicIncHalf(ca.trCount + cb, c);
           if (pos < kCaMaxCellTracks)
             pool[size_t(slot) * kCaMaxCellTracks + pos] = uint16_t(t);
         }
@@ -1078,7 +1086,7 @@ namespace pxg {
             list[m] = v;
           }
         }
-        ca.trCount[cb + c] = n;
+        ca.trCount[cb + c] = uint16_t(n);
         if (n < 2)
           continue;
         uint32_t maxNh = 0;

@@ -182,7 +182,7 @@ namespace pxg {
     uint64_t* startMask;     // [maxDoublets / 64 + 64] start cells of the ntuplet walk
     uint32_t *dfsT, *dfsH;   // [maxDoublets] tuples / hits per start cell, then offsets
     int32_t* trSlot;         // [maxDoublets] track list slot (-1 none, -2 store full)
-    uint32_t* trCount;       // [maxDoublets] tracks stored (<= 48)
+    uint16_t* trCount;       // [maxDoublets] tracks of the cell (< maxTuples < 65536), then stored (<= 48)
     uint16_t* trPool;        // [maxCellLists * 48]
     uint32_t* multiCells;    // [maxDoublets] cells with two or more tracks (any order)
     uint32_t* dupPreds;      // [maxCellLists * 8] per multiCells entry: its lower cells sharing a tuple (first 8)

@@ -319,7 +319,7 @@ class Stream:
         out["cellNeighborOff"], out["cellNeighbors"] = csr(ci.neighborSlot, ci.neighborCount, ci.neighborPool, 36,
                                                            np.uint32, np.uint8)
         out["cellTrackOff"], out["cellTracks"] = csr(ci.trackSlot, ci.trackCount, ci.trackPool, 48, np.uint16,
-                                                     np.uint32)
+                                                     np.uint16)
         out["multOff"] = self._copy(ci.multOff, np.uint32, 9)
         out["multBins"] = self._copy(ci.multBins, np.uint16, int(out["multOff"][8]))
         out["hitToTupleOff"] = self._copy(ci.hitToTupleOff, np.uint32, n_hits + 1)
