@@ -448,7 +448,7 @@ namespace pxg {
       return v;
     }
 
-    __global__ void __launch_bounds__(kCaBlock) connectKernel(Pool p, CAPool ca) {
+    __global__ void __launch_bounds__(kCaBlock, 7) connectKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
       __shared__ uint32_t sMe[kCaBlock / 64][kStage];
@@ -638,9 +638,9 @@ namespace pxg {
       float x, y, z, n;  // inner hit minus the list's hit, squared norm
     };
 
-    // lists that the wave's 64 positions touch, staged whole: at most 128 cells before
-    // the first position, the 64 positions and 128 cells after the last
-    constexpr uint32_t kFbWindow = 320;
+    // lists that the wave's 64 positions touch, staged whole (up to 256 entries: six
+    // workgroups per CU; a wave whose lists span more reads them from memory)
+    constexpr uint32_t kFbWindow = 256;
 
     template <bool kCheckTrack>
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
