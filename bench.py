@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--mix-events", type=int, default=200)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay each batch's stage sequence as a captured HIP graph (batched path)")
     ap.add_argument("--riemann-fit", action="store_true",
                     help="fit with the Riemann fit instead of the broken line (batched path)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -238,6 +240,8 @@ def main():
         st = ctx.stream(max_events=len(part), max_words=int(sum(len(w) for w, _ in part)))
         if args.riemann_fit:
             st.set_riemann_fit(True)
+        if args.graphs:
+            st.set_graphs(True)
         st.load(part)
         streams.append(st)
     stream = streams[0]
@@ -416,6 +420,7 @@ def main():
         "config": {"workload": f"{n} PU50 events ({STAGES[args.upto]}), batched path, {B} events per launch, "
                                f"{S} streams" + (", Riemann fit" if args.riemann_fit else ""),
                    "fit": "riemann" if args.riemann_fit else "broken line",
+                   "graphs": bool(args.graphs),
                    "events": n, "batch": B, "streams": S, "mean_words": round(float(W.mean()), 1),
                    "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
                    "stages": STAGES[args.upto]},

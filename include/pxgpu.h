@@ -315,6 +315,11 @@ int pxg_get_counters(pxg_stream* s, uint64_t* out); /* PXG_N_COUNTERS values */
 int pxg_reset_counters(pxg_stream* s);
 int pxg_print_counters(pxg_stream* s);
 int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target);
+/* With graphs on, pxg_run captures the stage sequence of each (first, count, upto) it is
+ * called with as a HIP graph once and replays it afterwards.  Loading events and changing a
+ * setting drop the captured graphs; a profiled stream runs eagerly.  No counterpart in the
+ * reference, which launches the kernels of a stage one by one. */
+int pxg_set_graphs(pxg_stream* s, int on);
 /* Times the chain on resident events [first, first+count) for each launch shape of a fixed
  * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */
 int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* best_lanes,

@@ -418,9 +418,16 @@ namespace pxg::detail {
 
 using namespace pxg::detail;
 
+void pxg::detail::dropGraphs(pxg_stream* s) {
+  for (auto& g : s->graphs)
+    (void)hipGraphExecDestroy(g.second);
+  s->graphs.clear();
+}
+
 int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync) {
   if (!s || !raws || !bs || n == 0)
     return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
+  dropGraphs(s);
   if (n > s->pool.maxEvents)
     return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
   uint32_t off = 0, blocks = 0;
@@ -576,6 +583,7 @@ void pxg_stream_destroy(pxg_stream* s) {
     return;
   if (s->stream)
     (void)hipStreamSynchronize(s->stream);
+  dropGraphs(s);
   for (auto& m : s->marks) {
     (void)hipEventDestroy(m.start);
     (void)hipEventDestroy(m.stop);
@@ -607,7 +615,42 @@ int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const p
 int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
   if (!s || upto < 1 || upto > 5)
     return fail(PXG_ERR_ARG, "pxg_run: bad arguments");
-  return runStages(s, first, count, 1, upto);
+  if (!s->useGraphs || s->profiling)
+    return runStages(s, first, count, 1, upto);
+  if (count == 0 || first + count > s->nLoaded)
+    return fail(PXG_ERR_ARG, "events [%u, %u) not loaded (%u resident)", first, first + count, s->nLoaded);
+  const auto key = std::make_tuple(first, count, upto);
+  auto it = s->graphs.find(key);
+  if (it == s->graphs.end()) {
+    // the sequence is kernels only (counters are cleared by kernels, not memsets)
+    PXG_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = runStages(s, first, count, 1, upto);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+      if (g)
+        (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return fail(PXG_ERR_HIP, "pxg_run: stage sequence capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return fail(PXG_ERR_HIP, "pxg_run: graph instantiation failed: %s", hipGetErrorString(e));
+    it = s->graphs.emplace(key, x).first;
+  }
+  PXG_HIP(hipGraphLaunch(it->second, s->stream));
+  return 0;
+}
+
+int pxg_set_graphs(pxg_stream* s, int on) {
+  if (!s)
+    return fail(PXG_ERR_ARG, "pxg_set_graphs: null stream");
+  s->useGraphs = on != 0;
+  dropGraphs(s);
+  return 0;
 }
 
 int pxg_fetch_counts(pxg_stream* s, uint32_t first, uint32_t count) {
@@ -779,6 +822,7 @@ int pxg_set_tuning(pxg_stream* s, uint32_t doubletLanes, uint32_t gridTarget) {
     return fail(PXG_ERR_ARG, "pxg_set_tuning: lanes must be a power of two in [1, 64], grid target >= 4");
   s->doubletLanes = doubletLanes;
   s->gridTarget = gridTarget;
+  dropGraphs(s);
   return 0;
 }
 
@@ -790,6 +834,7 @@ int pxg_set_vertex_algorithm(pxg_stream* s, int algo) {
   static_assert(PXG_VERTEX_DENSITY == kVertexDensity && PXG_VERTEX_DBSCAN == kVertexDBSCAN &&
                 PXG_VERTEX_ITERATIVE == kVertexIterative);
   s->vertexAlgo = algo;
+  dropGraphs(s);
   return 0;
 }
 
@@ -798,6 +843,7 @@ int pxg_set_fishbone(pxg_stream* s, int early, int late) {
     return fail(PXG_ERR_ARG, "pxg_set_fishbone: null stream");
   s->earlyFishbone = early != 0;
   s->lateFishbone = late != 0;
+  dropGraphs(s);
   return 0;
 }
 
@@ -805,6 +851,7 @@ int pxg_set_riemann_fit(pxg_stream* s, int riemann) {
   if (!s)
     return fail(PXG_ERR_ARG, "pxg_set_riemann_fit: null stream");
   s->riemannFit = riemann != 0;
+  dropGraphs(s);
   return 0;
 }
 
@@ -812,6 +859,7 @@ int pxg_set_stats(pxg_stream* s, int on) {
   if (!s)
     return fail(PXG_ERR_ARG, "pxg_set_stats: null stream");
   s->doStats = on != 0;
+  dropGraphs(s);
   return 0;
 }
 
@@ -867,6 +915,7 @@ int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32
     return fail(PXG_ERR_ARG, "pxg_autotune: bad arguments");
   static const uint32_t lanes[] = {2, 4, 8, 16, 32, 64};
   static const uint32_t grids[] = {1024, 2048, 4096, 8192};
+  dropGraphs(s);  // captured with the shape it replaces
   const bool wasProfiling = s->profiling;
   s->profiling = false;
   hipEvent_t a, b;

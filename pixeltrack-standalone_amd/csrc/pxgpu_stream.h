@@ -6,6 +6,7 @@
 
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pxgpu.h"
@@ -42,6 +43,11 @@ struct pxg_stream {
   bool riemannFit = false;     // the reference's default: broken-line fit
   bool doStats = false;        // the reference's default: no counters
   unsigned long long* counters = nullptr;  // device, PXG_N_COUNTERS
+  // pxg_run's stage sequences captured as HIP graphs, by (first, count, upto); their
+  // grid sizes come from the loaded events and the settings, so loading events or
+  // changing a setting drops them
+  bool useGraphs = false;
+  std::map<std::tuple<uint32_t, uint32_t, int>, hipGraphExec_t> graphs;
   std::vector<ProfileMark> marks;
   std::map<std::string, std::pair<uint32_t, double>> profile;
 };
@@ -60,4 +66,6 @@ namespace pxg::detail {
   int loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync);
   // bytes of device memory a pool of maxEvents events and maxWords words takes
   size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps);
+  // destroys the stream's captured stage sequences
+  void dropGraphs(pxg_stream* s);
 }  // namespace pxg::detail

@@ -115,6 +115,7 @@ def _gpu():
         lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
         lib.pxg_set_riemann_fit.argtypes = [vp, C.c_int]
         lib.pxg_set_stats.argtypes = [vp, C.c_int]
+        lib.pxg_set_graphs.argtypes = [vp, C.c_int]
         lib.pxg_get_counters.argtypes = [vp, C.POINTER(C.c_uint64)]
         lib.pxg_reset_counters.argtypes = [vp]
         lib.pxg_print_counters.argtypes = [vp]
@@ -442,6 +443,10 @@ class Stream:
 
     COUNTERS = ("nEvents", "nHits", "nCells", "nTuples", "nFitTracks", "nGoodTracks", "nUsedHits", "nDupHits",
                 "nKilledCells", "nEmptyCells", "nZeroTrackCells")
+
+    def set_graphs(self, on=True):
+        """Capture run()'s stage sequences as HIP graphs and replay them."""
+        _check(self.lib.pxg_set_graphs(self._h, int(on)), "pxg_set_graphs")
 
     def set_stats(self, on=True):
         """Accumulate the ntuplet generator's counters (the reference's doStats)."""

@@ -238,6 +238,53 @@ def test_stats_counters(detector, oracle, gpu_ctx):
     assert all(v == 0 for v in s.counters().values())
 
 
+def test_graph_replays_equal_eager_runs(detector, oracle, gpu_ctx, gpu_stream):
+    """pxg_set_graphs: the captured stage sequences give the eager run's products on
+    every replay, with other eager work and synchronisations between the replays, and
+    for two batch shapes; a setting change drops them (the next run captures again)."""
+    from pxtrack import gpu
+    evs = [event(detector, "pu20_80", 50 + i)[0] for i in range(3)]
+    unpacked = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in evs]
+    s = gpu_ctx.stream(max_events=3, max_words=sum(w.size for w, _ in unpacked) + 1024)
+    s.load(unpacked)
+
+    def products(first, count):
+        s.fetch_counts(first, count)
+        got = [s.event_products(e, upto=5) for e in range(first, first + count)]
+        for k, e in enumerate(range(first, first + count)):
+            got[k].update(s.ca_internals(e, int(got[k]["nHits"])))
+        return got
+
+    def same(ref, got, ctx):
+        for e in range(len(ref)):
+            nt, nv = int(ref[e]["nTuples"]), int(ref[e]["nVertices"])
+            assert (int(got[e]["nTuples"]), int(got[e]["nVertices"])) == (nt, nv), ctx
+            used = {"quality": nt, "state": 5 * nt, "idv": nt, "zv": nv}
+            for k in ("tupleOff", "tupleBins", "quality", "state", "cellInner", "cellOuter", "cellNeighbors",
+                      "cellTracks", "idv", "zv", "xx", "adc", "clus", "xg", "zg"):
+                n = used.get(k, len(ref[e][k]))
+                assert np.array_equal(ref[e][k][:n], got[e][k][:n]), f"{ctx} event {e}: {k}"
+
+    s.run(0, 3, upto=5)
+    ref = products(0, 3)
+    compare_ca(oracle.process(evs[0], upto=5), ref[0], "eager event 0")
+    s.run(1, 2, upto=5)
+    ref12 = products(1, 2)
+    s.set_graphs(True)
+    other = gpu.unpack_raw(event(detector, "pu50", 9)[0], gpu_ctx.fed_ids)
+    for rep in range(4):
+        print(f"graph replay {rep}", flush=True)
+        s.run(0, 3, upto=5)
+        same(ref, products(0, 3), f"replay {rep}")
+        gpu_stream.process(*other, upto=5)  # eager work on another stream in between
+        s.run(1, 2, upto=5)
+        same(ref12, products(1, 2), f"replay {rep} of events 1-2")
+    s.set_tuning(8, 1024)  # drops the graphs: the next run captures with the new shape
+    s.run(0, 3, upto=5)
+    same(ref, products(0, 3), "after a setting change")
+    s.set_graphs(False)
+
+
 def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
     """Every launch shape the autotuner may pick gives the same products."""
     from pxtrack import gpu
