@@ -1,9 +1,13 @@
 // Stage B: clusters -> rec-hits, layer offsets and the per-layer phi histogram.
 //
-//   hitsKernel      one wavefront per (module run, event), four per workgroup:
-//                   cluster extents and integer charges accumulated with LDS
-//                   atomics (exact in any order), then one cluster per lane for
-//                   the cluster position estimate and the local -> global
+//   hitsKernel      G lanes per (module run, event): 16 for batches (a module
+//                   holds ~30 digis and ~7 clusters at PU50, so a wavefront per
+//                   run would leave most lanes idle and make four times as many
+//                   latency chains), 64 for one event (more workgroups to fill
+//                   the device): cluster extents and integer charges
+//                   accumulated with LDS atomics (exact in any order), G
+//                   clusters per pass, then one cluster per lane for the
+//                   cluster position estimate and the local -> global
 //                   transform.  The float
 //                   arithmetic keeps the serial program's operation order and
 //                   the library is compiled without FMA contraction, so the
@@ -25,8 +29,6 @@ namespace pxg {
 
   namespace {
     constexpr int kBlock = 256;
-    constexpr int kWaves = kBlock / 64;  // module runs per workgroup
-    constexpr int kChunk = 64;           // clusters per pass of a wavefront
 
     // orders the wavefront's LDS accesses (lanes read what other lanes wrote)
     __device__ __forceinline__ void waveSync() {
@@ -88,10 +90,14 @@ namespace pxg {
     }
   }  // namespace
 
+  template <int G>
   __global__ void __launch_bounds__(kBlock) hitsKernel(DevConditions c, Pool p) {
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int kGroup = G;              // lanes per module run
+    constexpr int kRuns = kBlock / kGroup;  // module runs per workgroup
+    constexpr int kChunk = kGroup;          // clusters per pass of a run
+    const uint32_t lane = threadIdx.x % kGroup, run = threadIdx.x / kGroup;
     const LogicalBlock lb = logicalBlock();
-    const uint32_t k = lb.x * kWaves + wv;
+    const uint32_t k = lb.x * kRuns + run;
     const uint32_t e = lb.y;
     const EventDesc& ev = p.events[e];
     const BeamSpotPOD bs = ev.bs;
@@ -100,7 +106,7 @@ namespace pxg {
     if (k == 0) {  // average geometry corrected for the beam spot
       const AverageGeometry& ag = *c.cpeAverage;
       AverageGeometry& agc = p.avg[e];
-      for (uint32_t il = lane; il < kLaddersInBarrel; il += 64) {
+      for (uint32_t il = lane; il < kLaddersInBarrel; il += kGroup) {
         const float lx = ag.ladderX[il] - bs.x, ly = ag.ladderY[il] - bs.y;
         agc.ladderZ[il] = ag.ladderZ[il] - bs.z;
         agc.ladderX[il] = lx;
@@ -132,12 +138,12 @@ namespace pxg {
     const CPEDetParams& det = c.cpeDet[me];
     const CPECommonParams com = *c.cpeCommon;
 
-    __shared__ uint32_t sMinRow[kWaves][kChunk], sMaxRow[kWaves][kChunk], sMinCol[kWaves][kChunk],
-        sMaxCol[kWaves][kChunk];
-    __shared__ int32_t sQsum[kWaves][kChunk], sQfx[kWaves][kChunk], sQlx[kWaves][kChunk], sQfy[kWaves][kChunk],
-        sQly[kWaves][kChunk];
-    uint32_t *minRow = sMinRow[wv], *maxRow = sMaxRow[wv], *minCol = sMinCol[wv], *maxCol = sMaxCol[wv];
-    int32_t *qsum = sQsum[wv], *qfx = sQfx[wv], *qlx = sQlx[wv], *qfy = sQfy[wv], *qly = sQly[wv];
+    __shared__ uint32_t sMinRow[kRuns][kChunk], sMaxRow[kRuns][kChunk], sMinCol[kRuns][kChunk],
+        sMaxCol[kRuns][kChunk];
+    __shared__ int32_t sQsum[kRuns][kChunk], sQfx[kRuns][kChunk], sQlx[kRuns][kChunk], sQfy[kRuns][kChunk],
+        sQly[kRuns][kChunk];
+    uint32_t *minRow = sMinRow[run], *maxRow = sMaxRow[run], *minCol = sMinCol[run], *maxCol = sMaxCol[run];
+    int32_t *qsum = sQsum[run], *qfx = sQfx[run], *qlx = sQlx[run], *qfy = sQfy[run], *qly = sQly[run];
 
     for (uint32_t start = 0; start < nclus; start += kChunk) {
       const uint32_t nIn = min(uint32_t(kChunk), nclus - start);
@@ -147,7 +153,7 @@ namespace pxg {
         qsum[lane] = qfx[lane] = qlx[lane] = qfy[lane] = qly[lane] = 0;
       }
       waveSync();
-      for (uint32_t i = first + lane; i < end; i += 64) {
+      for (uint32_t i = first + lane; i < end; i += kGroup) {
         const uint32_t g = wo + i;
         if (p.moduleInd[g] != me)
           continue;  // invalid (or dropped) pixel
@@ -162,7 +168,7 @@ namespace pxg {
         atomicMax(&maxCol[j], y);
       }
       waveSync();
-      for (uint32_t i = first + lane; i < end; i += 64) {
+      for (uint32_t i = first + lane; i < end; i += kGroup) {
         const uint32_t g = wo + i;
         if (p.moduleInd[g] != me)
           continue;
@@ -183,7 +189,7 @@ namespace pxg {
           atomicAdd(&qly[j], ch);
       }
       waveSync();
-      for (uint32_t ic = lane; ic < nIn; ic += 64) {
+      for (uint32_t ic = lane; ic < nIn; ic += kGroup) {
         const uint32_t h = hitBase + start + ic;
         if (h >= kMaxNumClusters)
           break;
@@ -366,7 +372,10 @@ namespace pxg {
   }
 
   void launchHits(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
-    hitsKernel<<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(c, p);
+    if (nEvents < 8)
+      hitsKernel<64><<<dim3((kMaxNumModules + kBlock / 64 - 1) / (kBlock / 64), nEvents), kBlock, 0, s>>>(c, p);
+    else
+      hitsKernel<16><<<dim3((kMaxNumModules + kBlock / 16 - 1) / (kBlock / 16), nEvents), kBlock, 0, s>>>(c, p);
   }
 
   void launchPhiBinner(const DevConditions& c, const Pool& p, uint32_t nEvents, hipStream_t s) {
