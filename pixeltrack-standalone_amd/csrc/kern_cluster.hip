@@ -8,9 +8,10 @@
 // result depends only on the connected components, so any exact algorithm
 // gives the same labels; two are used here, picked per module run:
 //
-//   clusterSmallKernel  one wavefront per module run of at most 64 digis
-//                       (most modules at PU50 carry ~30): a 64-bit neighbour
-//                       mask per lane from shuffles, min-label propagation
+//   clusterSmallKernel  two module runs of at most 64 digis per wavefront
+//                       (most modules at PU50 carry ~30: on the two halves when
+//                       both fit 32 lanes): a neighbour mask per lane from
+//                       shuffles, min-label propagation
 //                       with pointer jumping through a per-wave LDS row until
 //                       a ballot sees no change, roots ranked with popcounts.
 //                       No workgroup barrier, no atomics but the charge sums.
@@ -294,103 +295,138 @@ namespace pxg {
     __device__ __forceinline__ uint64_t lanesBelow(uint32_t lane) { return (1ull << lane) - 1ull; }
   }  // namespace
 
-  // ---- module runs of at most 64 digis: one wavefront each -------------------------
+  // ---- module runs of at most 64 digis ---------------------------------------------
+  // One run on the lanes [base, base + G) of the wavefront (G = 64, or G = 32 with the
+  // two halves on two runs at once).  Every lane of the wavefront calls this together:
+  // the propagation loop ends on a wavefront-wide ballot.  `live`: this group has a run.
+  namespace {
+    template <int G>
+    __device__ void clusterRunLanes(const Pool& p, uint32_t e, uint32_t k, uint32_t first, uint32_t n,
+                                    const EventDesc& ev, int32_t* rowWave, uint32_t base, bool live) {
+      const uint32_t lane = (threadIdx.x & 63) - base;
+      constexpr uint64_t kGroupMask = G == 64 ? ~0ull : (1ull << G) - 1;
+      auto ballot = [&](bool v) { return (__ballot(v) >> base) & kGroupMask; };
+      int32_t* row = rowWave + base;
+      const uint32_t wo = ev.wordOffset;
+      const uint16_t me = live ? p.moduleInd[wo + first] : kInvId;
+      // a run always starts on a valid digi; never false for valid conditions
+      live = live && me < kNumModules;
+      const bool act = live && lane < n;
+      const uint32_t g = wo + first + lane;
+      uint32_t x = 0, y = 0;
+      int32_t adc = 0;
+      bool valid = false;
+      if (act) {
+        x = p.xx[g];
+        y = p.yy[g];
+        adc = p.adc[g];
+        valid = p.moduleInd[g] != kInvId && x < kRowsPerModule && y < kColsPerModule;
+      }
+      // 8-neighbourhood as a lane mask (duplicated pixels are neighbours too); a
+      // valid pixel is x | y << 16 with x < 160, y < 416, an invalid one ~0
+      uint64_t nb = 0;
+      const uint32_t xy = valid ? x | y << 16 : 0xffffffffu;
+      const uint32_t nMax = live ? n : 0;
+      const uint32_t nLoop = max(nMax, uint32_t(__shfl_xor(int(nMax), G == 64 ? 0 : 32)));  // both halves' runs
+      for (uint32_t j = 0; j < nLoop; ++j) {
+        const uint32_t xyj = __shfl(xy, int(base + j));
+        const int dx = int(xyj & 0xffff) - int(x), dy = int(xyj >> 16) - int(y);
+        if (j < n && valid && xyj != 0xffffffffu && j != lane && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
+          nb |= 1ull << j;
+      }
+      // min-label propagation with pointer jumping; every label is a lane of the same component
+      uint32_t label = lane;
+      while (true) {
+        row[lane] = int32_t(label);
+        waveSync();
+        uint32_t m = label;
+        for (uint64_t b = nb; b; b &= b - 1) {
+          const uint32_t l = uint32_t(row[__ffsll((unsigned long long)b) - 1]);
+          m = l < m ? l : m;
+        }
+        const uint32_t jump = uint32_t(row[m]);
+        m = jump < m ? jump : m;
+        waveSync();
+        const bool changed = m != label;
+        label = m;
+        if (__ballot(changed) == 0)
+          break;
+      }
+      // roots (label == own index) numbered in index order
+      const uint64_t roots = ballot(valid && label == lane);
+      const uint32_t nclus = __popcll(roots);
+      const uint32_t cid = __popcll(roots & lanesBelow(label));
+      // charge cut
+      row[lane] = 0;
+      waveSync();
+      if (valid)
+        atomicAdd(&row[cid], adc);
+      waveSync();
+      const bool isRoot = (roots >> lane) & 1;
+      const int cut = me < 96 ? 2000 : 4000;
+      const uint64_t okRoots = ballot(isRoot && row[__popcll(roots & lanesBelow(lane))] > cut);
+      const uint32_t kept = __popcll(okRoots);
+      const bool renumber = nclus > 0 && kept != nclus;
+      if (act) {
+        if (!valid) {
+          p.clus[g] = -9999;
+        } else if (!renumber) {
+          p.clus[g] = int32_t(cid);
+        } else if ((okRoots >> label) & 1) {
+          p.clus[g] = int32_t(__popcll(okRoots & lanesBelow(label)));
+        } else {
+          p.clus[g] = 9999;
+          p.moduleInd[g] = kInvId;
+        }
+      }
+      if (live && lane == 0) {
+        p.clusInModule[size_t(e) * kMaxModuleRanges + me] = renumber ? kept : nclus;
+        p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
+      }
+      waveSync();  // the row is reused by the next run of the wavefront
+    }
+  }  // namespace
+
+  // kRuns (2) consecutive module runs per wavefront: on its two halves when both hold
+  // at most 32 digis (most at PU50), else one after the other on the whole wavefront
+  // (one run per wavefront for a few events, to fill the device); runs of more than
+  // 64 digis go to the work list of the workgroup kernel
+  template <uint32_t kRuns>
   __global__ void __launch_bounds__(kBlock) clusterSmallKernel(Pool p) {
     __shared__ int32_t rowLds[kWaves][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const LogicalBlock lb = logicalBlock();
-    const uint32_t k = lb.x * kWaves + wave;
+    const uint32_t k0 = kRuns * (lb.x * kWaves + wave);
     const uint32_t e = lb.y;
     const EventDesc& ev = p.events[e];
     const uint32_t nModules = p.counts[e].nModules;
-    if (k >= nModules)
+    if (k0 >= nModules)
       return;  // uniform per wavefront; this kernel has no workgroup barrier
     const uint32_t* ms = p.moduleStart + size_t(e) * kMaxModuleRanges;
-    const uint32_t first = ms[1 + k];
-    const uint32_t end = (k + 1 < nModules) ? ms[2 + k] : ev.nWords;
-    const uint32_t n = end - first;
-    if (n > 64) {
-      if (lane == 0) {
-        const uint32_t slot = atomicAdd(&p.work[0], 1u);
-        p.bigList[slot] = (e << 16) | k;
-      }
-      return;
-    }
-    const uint32_t wo = ev.wordOffset;
-    const uint16_t me = p.moduleInd[wo + first];
-    if (me >= kNumModules)
-      return;  // a run always starts on a valid digi; never taken for valid conditions
+    auto runOf = [&](uint32_t k, uint32_t& first) -> uint32_t {
+      if (k >= nModules)
+        return 0;
+      first = ms[1 + k];
+      const uint32_t end = (k + 1 < nModules) ? ms[2 + k] : ev.nWords;
+      return end - first;
+    };
+    uint32_t f0 = 0, f1 = 0;
+    const uint32_t n0 = runOf(k0, f0), n1 = kRuns > 1 ? runOf(k0 + 1, f1) : 0;
     int32_t* row = rowLds[wave];
-
-    const bool act = lane < n;
-    const uint32_t g = wo + first + lane;
-    uint32_t x = 0, y = 0;
-    int32_t adc = 0;
-    bool valid = false;
-    if (act) {
-      x = p.xx[g];
-      y = p.yy[g];
-      adc = p.adc[g];
-      valid = p.moduleInd[g] != kInvId && x < kRowsPerModule && y < kColsPerModule;
-    }
-    // 8-neighbourhood as a lane mask (duplicated pixels are neighbours too); a
-    // valid pixel is x | y << 16 with x < 160, y < 416, an invalid one ~0
-    uint64_t nb = 0;
-    const uint32_t xy = valid ? x | y << 16 : 0xffffffffu;
-    for (uint32_t j = 0; j < n; ++j) {
-      const uint32_t xyj = __shfl(xy, int(j));
-      const int dx = int(xyj & 0xffff) - int(x), dy = int(xyj >> 16) - int(y);
-      if (valid && xyj != 0xffffffffu && j != lane && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
-        nb |= 1ull << j;
-    }
-    // min-label propagation with pointer jumping; every label is a lane of the same component
-    uint32_t label = lane;
-    while (true) {
-      row[lane] = int32_t(label);
-      waveSync();
-      uint32_t m = label;
-      for (uint64_t b = nb; b; b &= b - 1) {
-        const uint32_t l = uint32_t(row[__ffsll((unsigned long long)b) - 1]);
-        m = l < m ? l : m;
-      }
-      const uint32_t jump = uint32_t(row[m]);
-      m = jump < m ? jump : m;
-      waveSync();
-      const bool changed = m != label;
-      label = m;
-      if (__ballot(changed) == 0)
-        break;
-    }
-    // roots (label == own index) numbered in index order
-    const uint64_t roots = __ballot(valid && label == lane);
-    const uint32_t nclus = __popcll(roots);
-    const uint32_t cid = __popcll(roots & lanesBelow(label));
-    // charge cut
-    row[lane] = 0;
-    waveSync();
-    if (valid)
-      atomicAdd(&row[cid], adc);
-    waveSync();
-    const bool isRoot = (roots >> lane) & 1;
-    const int cut = me < 96 ? 2000 : 4000;
-    const uint64_t okRoots = __ballot(isRoot && row[__popcll(roots & lanesBelow(lane))] > cut);
-    const uint32_t kept = __popcll(okRoots);
-    const bool renumber = nclus > 0 && kept != nclus;
-    if (act) {
-      if (!valid) {
-        p.clus[g] = -9999;
-      } else if (!renumber) {
-        p.clus[g] = int32_t(cid);
-      } else if ((okRoots >> label) & 1) {
-        p.clus[g] = int32_t(__popcll(okRoots & lanesBelow(label)));
-      } else {
-        p.clus[g] = 9999;
-        p.moduleInd[g] = kInvId;
+    for (uint32_t u = 0; u < 2; ++u) {
+      const uint32_t n = u ? n1 : n0;
+      if (n > 64 && lane == 0) {
+        const uint32_t slot = atomicAdd(&p.work[0], 1u);
+        p.bigList[slot] = (e << 16) | (k0 + u);
       }
     }
-    if (lane == 0) {
-      p.clusInModule[size_t(e) * kMaxModuleRanges + me] = renumber ? kept : nclus;
-      p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
+    if (n0 <= 32 && n1 <= 32) {
+      const uint32_t half = lane >> 5;
+      clusterRunLanes<32>(p, e, k0 + half, half ? f1 : f0, half ? n1 : n0, ev, row, 32 * half,
+                          (half ? n1 : n0) > 0);
+    } else {
+      clusterRunLanes<64>(p, e, k0, f0, n0, ev, row, 0, n0 > 0 && n0 <= 64);
+      clusterRunLanes<64>(p, e, k0 + 1, f1, n1, ev, row, 0, n1 > 0 && n1 <= 64);
     }
   }
 
@@ -458,7 +494,10 @@ namespace pxg {
   }
 
   void launchClusterize(const Pool& p, uint32_t nEvents, hipStream_t s) {
-    clusterSmallKernel<<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(p);
+    if (nEvents < 8)
+      clusterSmallKernel<1><<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(p);
+    else
+      clusterSmallKernel<2><<<dim3((kMaxNumModules + 2 * kWaves - 1) / (2 * kWaves), nEvents), kBlock, 0, s>>>(p);
     clusterBigKernel<<<kBigWorkgroups, kBlock, 0, s>>>(p);
     clusStartKernel<<<nEvents, 1024, 0, s>>>(p);
   }
