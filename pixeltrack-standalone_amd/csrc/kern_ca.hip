@@ -783,40 +783,46 @@ namespace pxg {
       const size_t cb = ca.cellBase(e);
       const uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
       const size_t tb = ca.tupleBase(e);
-      uint32_t stack[4];
-      uint8_t pos[4];
-      bool any[4];
+      // the stack (at most 4 cells) in registers: the cells as four values read through
+      // selects, the list positions as bytes of one word, the "any child" flags as bits
+      uint32_t st0 = c, st1 = 0, st2 = 0, st3 = 0;
+      auto cellAt = [&](int d) { return d == 0 ? st0 : d == 1 ? st1 : d == 2 ? st2 : st3; };
+      auto setCell = [&](int d, uint32_t v) {
+        st1 = d == 1 ? v : st1;
+        st2 = d == 2 ? v : st2;
+        st3 = d == 3 ? v : st3;
+      };
+      uint32_t posBytes = 0, anyBits = 0;
       int depth = 1;
-      stack[0] = c;
-      pos[0] = 0;
-      any[0] = false;
       nT = nH = 0;
       while (depth > 0) {
         const int top = depth - 1;
-        const uint32_t cell = stack[top];
+        const uint32_t cell = cellAt(top);
         const int32_t slot = ca.nbSlot[cb + cell];
         const uint32_t cnt = slot >= 0 ? ca.nbCount[cb + cell] : 0;
         bool pushed = false;
-        while (pos[top] < cnt) {
-          const uint32_t other = pool[size_t(slot) * kCaMaxNeighbors + pos[top]];
-          ++pos[top];
+        uint32_t pos = (posBytes >> (8 * top)) & 0xff;
+        while (pos < cnt) {
+          const uint32_t other = pool[size_t(slot) * kCaMaxNeighbors + pos];
+          ++pos;
           if (ca.cellFlags[cb + other] & 2)
             continue;
-          any[top] = true;
+          anyBits |= 1u << top;
           if (depth >= 4) {
             status |= kCaDepthOverflow;
             continue;
           }
-          stack[depth] = other;
-          pos[depth] = 0;
-          any[depth] = false;
+          setCell(depth, other);
+          posBytes &= ~(0xffu << (8 * depth));  // the child's position starts at 0
+          anyBits &= ~(1u << depth);
           ++depth;
           pushed = true;
           break;
         }
+        posBytes = (posBytes & ~(0xffu << (8 * top))) | (pos << (8 * top));
         if (pushed)
           continue;
-        if (!any[top] && depth >= 2) {
+        if (!((anyBits >> top) & 1) && depth >= 2) {
           const uint32_t nh = uint32_t(depth) + 1;
           if (kFill) {
             const uint32_t m = tBase + nT;
@@ -825,9 +831,10 @@ namespace pxg {
               ca.tupleOff[ca.offBase(e) + m] = first;
               uint16_t* bins = ca.tupleBins + 5 * tb + first;
               for (int k = 0; k < depth; ++k) {
-                bins[k] = ca.cellInner[cb + stack[k]];
-                ca.tupleCells[4 * tb + 4 * size_t(m) + k] = stack[k];
-                atomicIncHalf(ca.trCount + cb, stack[k]);
+                const uint32_t ck = cellAt(k);
+                bins[k] = ca.cellInner[cb + ck];
+                ca.tupleCells[4 * tb + 4 * size_t(m) + k] = ck;
+                atomicIncHalf(ca.trCount + cb, ck);
               }
               bins[depth] = ca.cellOuter[cb + cell];
               for (int k = depth; k < 4; ++k)
