@@ -202,6 +202,8 @@ namespace pxg::detail {
     v.ysize = p.ysize + h;
     v.detInd = p.detInd + h;
     v.phiBins = p.phiBins + h;
+    v.pos = p.pos + h;
+    v.attr = p.attr + h;
     v.hitsLayerStart = p.hitsLayerStart + size_t(first) * kLayerStartStride;
     v.phiOff = p.phiOff + size_t(first) * kPhiOffStride;
     v.avg = p.avg + first;
