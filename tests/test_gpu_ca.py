@@ -211,6 +211,24 @@ def test_fishbone_switches(detector, conditions, gpu_ctx, early, late):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"fishbone {early}/{late} pu50[{idx}]")
 
 
+def test_alternative_switches_together_at_pu200(detector, conditions, gpu_ctx):
+    """The Riemann fit, the late fishbone without the early one and DBSCAN vertexing at
+    once, on a PU200 event under the high-occupancy capacities, against the oracle."""
+    import pyoracle
+    from pxtrack import gpu
+    orc = pyoracle.Oracle(conditions, profile=1, vertex_algo=1, early_fishbone=False, late_fishbone=True,
+                          riemann_fit=True)
+    s = gpu_ctx.stream(max_events=1, max_words=400000, profile=gpu.PROFILE_HIGH_OCCUPANCY)
+    s.set_riemann_fit(True)
+    s.set_fishbone(False, True)
+    s.set_vertex_algorithm(1)
+    raw, _ = event(detector, "pu200", 3)
+    o = orc.process(raw, upto=5)
+    print(f"pu200 switches: cells={o['nCells']} tuples={o['nTuples']} vertices={o['nVertices']} "
+          f"status={o['caStatus']:#x}", flush=True)
+    compare_ca(o, _gpu_event(gpu_ctx, s, raw), "pu200/switches")
+
+
 def test_stats_counters(detector, oracle, gpu_ctx):
     """doStats (CAHitNtupletGeneratorKernels.h:14-26): the counters the GPU accumulates
     over a batch equal the oracle's per-event counts summed; off by default, a second
