@@ -41,12 +41,13 @@ def algorithmic_bytes(kernel, c):
         # arrays (20 B) and reads ~2 B of gains per digi; clusterize reads x, y, adc,
         # moduleInd (8 B) and writes clus + moduleInd (6 B); hits reads 12 B per digi and
         # writes 44 B per hit; moduleStarts reads 16 B of masks per 64 words and writes
-        # 4 B per module run; phiBinner reads iphi and writes the bin contents + offsets
+        # 4 B per module run; phiBinner reads iphi and writes the bin contents + offsets,
+        # and copies each hit's packed position and attributes (24 B) into histogram order
         "decode": 24.5 * W + 2 * D,
         "moduleStarts": W / 64 * 16 + 4 * M,
         "clusterize": 14 * D,
         "hits": 12 * D + 44 * H,
-        "phiBinner": 4 * H + 5124,
+        "phiBinner": 4 * H + 48 * H + 5124,
         # doublets: hits (z, r, iphi, ysize, detInd: 14 B) + phi histogram (2 B per hit
         # + 5 kB) in, one count (4 B) per slot out; the fill pass writes the cells
         "doubletCount": 16 * H + 5124 + 4 * S,

@@ -26,6 +26,8 @@ namespace pxg {
       const uint16_t* phiBins;
       const float4* pos;
       const HitAttr* attr;
+      const float4* posPhi;     // pos[phiBins[k]] at k
+      const HitAttr* attrPhi;   // attr[phiBins[k]] at k
     };
 
     __device__ __forceinline__ HitView hitsOf(const Pool& p, uint32_t e) {
@@ -41,7 +43,9 @@ namespace pxg {
               p.phiOff + size_t(e) * kPhiOffStride,
               p.phiBins + h,
               p.pos + h,
-              p.attr + h};
+              p.attr + h,
+              p.posPhi + h,
+              p.attrPhi + h};
     }
 
     __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
@@ -152,18 +156,19 @@ namespace pxg {
       return q;
     }
 
-    // the cuts of one (inner, outer) hit pair: gpuPixelDoubletsAlgos.h:141-215
-    __device__ __forceinline__ bool acceptDoublet(const HitView& H, const SlotScan& q, uint32_t oi) {
+    // the cuts of one (inner, outer) hit pair: gpuPixelDoubletsAlgos.h:141-215; the
+    // outer hit is the one at position k of the phi histogram's contents
+    __device__ __forceinline__ bool acceptDoublet(const HitView& H, const SlotScan& q, uint32_t k) {
       constexpr int maxDYsize12 = 28, maxDYsize = 20, maxDYPred = 20;
       constexpr float dzdrFact = 8 * 0.0285 / 0.015;
       constexpr float z0cut = 12.f;
       constexpr float minRadius = 0.5f * 87.78f;
       constexpr float minRadius2T4 = 4.f * minRadius * minRadius;
       constexpr float i2p = 3.14159265358979323846 / 32768;
-      const HitAttr a = H.attr[oi];
+      const HitAttr a = H.attrPhi[k];
       if (a.detInd > 2000)
         return false;
-      const float4 po = H.pos[oi];
+      const float4 po = H.posPhi[k];
       const float zo = po.z, ro = po.w;
       const float dr = ro - q.mer;
       if (dr > q.maxr || dr < 0 || fabsf(q.mez * ro - q.mer * zo) > z0cut * dr)
@@ -185,8 +190,12 @@ namespace pxg {
       return !(dphi * dphi * (minRadius2T4 - q.mer * ro) > (ro - q.mer) * (ro - q.mer));
     }
 
+    // position in the phi histogram's contents of the slot's k-th candidate
+    __device__ __forceinline__ uint32_t windowPos(const SlotScan& q, uint32_t k) {
+      return k < q.n1 ? q.s1 + k : q.s2 + (k - q.n1);
+    }
     __device__ __forceinline__ uint32_t candidate(const HitView& H, const SlotScan& q, uint32_t k) {
-      return H.phiBins[k < q.n1 ? q.s1 + k : q.s2 + (k - q.n1)];
+      return H.phiBins[windowPos(q, k)];
     }
 
     // position of the r-th set bit of w (r < popcount(w))
@@ -238,7 +247,7 @@ namespace pxg {
         if (q.live) {
           for (uint32_t k0 = 0; k0 < q.n; k0 += G) {
             const uint32_t k = k0 + sub;
-            const bool ok = k < q.n && acceptDoublet(H, q, candidate(H, q, k));
+            const bool ok = k < q.n && acceptDoublet(H, q, windowPos(q, k));
             const uint64_t bits = groupBits<G>(__ballot(ok));
             n += __popcll(bits);
             if (k0 < 64)
@@ -332,8 +341,9 @@ namespace pxg {
           uint32_t oi = 0;
           bool ok = false;
           if (k < q.n) {
-            oi = candidate(H, q, k);
-            ok = acceptDoublet(H, q, oi);
+            const uint32_t w = windowPos(q, k);
+            oi = H.phiBins[w];
+            ok = acceptDoublet(H, q, w);
           }
           const uint64_t m = groupBits<G>(__ballot(ok));
           const uint32_t c = base + run + __popcll(m & below);

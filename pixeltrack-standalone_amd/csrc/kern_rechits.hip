@@ -18,6 +18,9 @@
 //                   counters, which lists every bin in decreasing hit index;
 //                   each wavefront reproduces that order deterministically by
 //                   ranking equal bins inside a 64-hit chunk with 7 ballots.
+//                   Beside each hit index it writes the hit's packed position
+//                   and attributes (posPhi, attrPhi), so the doublet search
+//                   reads a phi window's candidates from contiguous memory.
 //
 // Reference behaviour: plugin-SiPixelRecHits/gpuPixelRecHits.h:16-207,
 // CondFormats/pixelCPEforGPU.h:106-340, DataFormats/SOARotation.h:55-94,
@@ -286,6 +289,10 @@ namespace pxg {
     uint32_t* off = p.phiOff + size_t(e) * kPhiOffStride;
     uint16_t* bins = p.phiBins + size_t(e) * kMaxNumClusters;
     const int16_t* iphi = p.iphi + size_t(e) * kMaxNumClusters;
+    const float4* pos = p.pos + size_t(e) * kMaxNumClusters;
+    const HitAttr* attr = p.attr + size_t(e) * kMaxNumClusters;
+    float4* posPhi = p.posPhi + size_t(e) * kMaxNumClusters;
+    HitAttr* attrPhi = p.attrPhi + size_t(e) * kMaxNumClusters;
 
     __shared__ uint32_t ls[kNumLayers + 1];
     __shared__ uint32_t cnt[kPhiHistTotBins + 1];
@@ -340,10 +347,28 @@ namespace pxg {
     for (uint32_t b = lane; b < kPhiBins; b += 64)
       taken[layer * kPhiBins + b] = 0;
     __builtin_amdgcn_wave_barrier();
+    // the chunk's inputs are loaded one chunk ahead, so the loads overlap the
+    // ranking of the previous chunk
+    int16_t ipNext = 0;
+    float4 posNext{};
+    HitAttr attrNext{};
+    if (h0 + lane < h1) {
+      ipNext = iphi[h0 + lane];
+      posNext = pos[h0 + lane];
+      attrNext = attr[h0 + lane];
+    }
     for (uint32_t c0 = h0; c0 < h1; c0 += 64) {
       const uint32_t i = c0 + lane;
       const bool act = i < h1;
-      const uint32_t key = act ? ((int(iphi[i]) >> 9) & 127) : 0;
+      const int16_t ip = ipNext;
+      const float4 pv = posNext;
+      const HitAttr av = attrNext;
+      if (i + 64 < h1) {
+        ipNext = iphi[i + 64];
+        posNext = pos[i + 64];
+        attrNext = attr[i + 64];
+      }
+      const uint32_t key = act ? ((int(ip) >> 9) & 127) : 0;
       uint64_t peers = __ballot(act);
 #pragma unroll
       for (int bit = 0; bit < 7; ++bit) {
@@ -359,7 +384,10 @@ namespace pxg {
         before = taken[bin];
       __builtin_amdgcn_wave_barrier();
       if (act) {
-        bins[cnt[bin] - 1 - (before + rank)] = uint16_t(i);
+        const uint32_t k = cnt[bin] - 1 - (before + rank);
+        bins[k] = uint16_t(i);
+        posPhi[k] = pv;
+        attrPhi[k] = av;
         if (rank == group - 1)
           taken[bin] = before + group;
       }

@@ -95,6 +95,8 @@ namespace pxg::detail {
     f(p.rg, H);
     f(p.pos, H);
     f(p.attr, H);
+    f(p.posPhi, H);
+    f(p.attrPhi, H);
     f(p.iphi, H);
     f(p.charge, H);
     f(p.xsize, H);
@@ -204,6 +206,8 @@ namespace pxg::detail {
     v.phiBins = p.phiBins + h;
     v.pos = p.pos + h;
     v.attr = p.attr + h;
+    v.posPhi = p.posPhi + h;
+    v.attrPhi = p.attrPhi + h;
     v.hitsLayerStart = p.hitsLayerStart + size_t(first) * kLayerStartStride;
     v.phiOff = p.phiOff + size_t(first) * kPhiOffStride;
     v.avg = p.avg + first;

@@ -101,6 +101,8 @@ namespace pxg {
     uint32_t* hitsLayerStart;  // stride 16
     uint32_t* phiOff;          // stride kPhiHistTotBins (padded to 1288)
     uint16_t* phiBins;         // stride kMaxNumClusters
+    float4* posPhi;            // pos of phiBins[k] at k: a phi window's candidates in one contiguous range
+    HitAttr* attrPhi;          // attr of phiBins[k] at k
     AverageGeometry* avg;      // per event
   };
 
