@@ -137,7 +137,8 @@ CA_FIELDS = ["cellInner", "cellOuter", "cellDoubletId", "cellLayerPair", "cellUs
              "covariance", "pt", "eta", "chi2", "idv", "zv", "wv", "chi2v", "ptv2", "ndof", "sortInd", "stats"]
 FIELDS = FIELDS + CA_FIELDS
 AB_SCALARS = ["nWords", "nModules", "nDigis", "nClusters", "nHits", "fedCounter"]
-SCALARS = AB_SCALARS + ["nCells", "nTuples", "caStatus", "nVertices", "vtxTracks", "nNeighborLists", "nTrackLists"]
+SCALARS = AB_SCALARS + ["nCells", "nTuples", "caStatus", "nVertices", "nVerticesFound", "vtxTracks", "nNeighborLists",
+                        "nTrackLists"]
 
 
 class Oracle(_Chain):

@@ -104,6 +104,7 @@ namespace orc {
   struct VertexState {
     uint32_t status = 0;
     uint32_t ntrks = 0, nvFinal = 0;
+    uint32_t nFound = 0;  // vertices found by the clustering, before splitVertices adds its own
     std::vector<int16_t> idv;
     std::vector<int32_t> ndof;
     std::vector<float> zv, wv, chi2, ptv2;

@@ -269,6 +269,8 @@ uint32_t orc_scalar(const orc_event* e, const char* name) {
     return ev.vtx.nvFinal;
   if (n == "vtxTracks")
     return ev.vtx.ntrks;
+  if (n == "nVerticesFound")
+    return ev.vtx.nFound;
   if (n == "nNeighborLists")
     return uint32_t(ev.ca.neighborStore.size());
   if (n == "nTrackLists")

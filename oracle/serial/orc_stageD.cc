@@ -253,6 +253,7 @@ namespace orc {
       iv[i] = -iv[i] - 1;
     uint32_t nvIntermediate = found;
     uint32_t nvFinal = found;
+    V.nFound = found;
     if (found >= kMaxVertices)
       V.status |= 1;
 

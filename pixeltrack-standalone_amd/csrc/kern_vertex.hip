@@ -480,73 +480,110 @@ namespace pxg {
       fit(50.f);
 
       // splitVertices: each vertex's 2-means runs on its own tracks; new
-      // vertices are numbered in the order of the vertices they split from
+      // vertices are numbered in the order of the vertices they split from.
+      // A vertex to split gets a wave: every lane runs the same sequential sums
+      // over the vertex's tracks in ascending order (the serial sums, bit for
+      // bit), and the lanes reassign the tracks in parallel.
       {
         const uint32_t nvf = L.nvFinal;
+        const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         for (uint32_t base = 0; base < nvf; base += kVB) {
           const uint32_t kv = base + threadIdx.x;
-          bool split = false;
           // the vertex's assigned tracks in ascending order, compacted into its own
           // segment [b, b + m) of arrays that are free here (the z-ordered copies
-          // and tmp): offset from the vertex, weight, side, track index
+          // and tmp): offset from the vertex, weight, offset x weight, and the
+          // track index with the side in bit 31
           const uint32_t b = kv < nvf && kv < kMaxVertices ? voff[kv] : 0;
           float* const czz = zS;
           float* const cww = eS;
-          int32_t* const cside = nnS;
-          int32_t* const cidx = tmp;
+          float* const cprod = reinterpret_cast<float*>(nnS);
+          uint32_t* const cidx = reinterpret_cast<uint32_t*>(tmp);
           uint32_t m = 0;
-          if (kv < nvf && kv < kMaxVertices && nn[kv] >= 4 && !(chi2[kv] < 9.f * float(nn[kv])) && nn[kv] < 512) {
+          const bool cand =
+              kv < nvf && kv < kMaxVertices && nn[kv] >= 4 && !(chi2[kv] < 9.f * float(nn[kv])) && nn[kv] < 512;
+          if (cand) {
             forEachOf(kv, [&](uint32_t i) {
               if (iv[i] != int32_t(kv))
                 return;
               const float zz = zt[i] - zv[kv];
+              const float ww = 1.f / ezt2[i];
               czz[b + m] = zz;
-              cww[b + m] = 1.f / ezt2[i];
-              cside[b + m] = zz < 0 ? 0 : 1;
-              cidx[b + m] = int32_t(i);
+              cww[b + m] = ww;
+              cprod[b + m] = zz * ww;
+              cidx[b + m] = i | (zz < 0 ? 0u : 0x80000000u);
               ++m;
             });
+          }
+          // L.counts: the vertex's track count (bit 31: split); L.running: the candidates
+          if (kv < kMaxVertices)
+            L.counts[kv] = m;
+          uint32_t nCand;
+          const uint32_t ci = wgExclusiveScan<kVB>(cand ? 1u : 0u, nCand, L.scan);
+          if (cand)
+            L.running[ci] = kv;
+          __syncthreads();
+          for (uint32_t c = wid; c < nCand; c += kVB / 64) {
+            const uint32_t v = L.running[c];
+            const uint32_t vb = voff[v], vm = L.counts[v];
             float znew[2], wnew[2];
             int maxiter = 20;
             bool more = true;
             while (more) {
-              more = false;
               znew[0] = znew[1] = 0;
               wnew[0] = wnew[1] = 0;
-              for (uint32_t u = 0; u < m; ++u) {
-                const int32_t sd = cside[b + u];
-                const float ww = cww[b + u];
-                znew[sd] += czz[b + u] * ww;
-                wnew[sd] += ww;
+              for (uint32_t u = 0; u < vm; ++u) {
+                const float pr = cprod[vb + u], ww = cww[vb + u];
+                if (cidx[vb + u] >> 31) {
+                  znew[1] += pr;
+                  wnew[1] += ww;
+                } else {
+                  znew[0] += pr;
+                  wnew[0] += ww;
+                }
               }
               znew[0] /= wnew[0];
               znew[1] /= wnew[1];
-              for (uint32_t u = 0; u < m; ++u) {
-                const float zz = czz[b + u];
+              bool changed = false;
+              for (uint32_t u = lane; u < vm; u += 64) {
+                const float zz = czz[vb + u];
                 const float d0 = fabsf(zz - znew[0]);
                 const float d1 = fabsf(zz - znew[1]);
-                const int32_t newer = d0 < d1 ? 0 : 1;
-                more |= newer != cside[b + u];
-                cside[b + u] = newer;
+                const uint32_t newer = d0 < d1 ? 0u : 0x80000000u;
+                const uint32_t x = cidx[vb + u];
+                changed |= newer != (x & 0x80000000u);
+                cidx[vb + u] = (x & 0x7fffffffu) | newer;
               }
+              // the next pass's sums read what other lanes wrote (in LDS, or in
+              // global memory for events with more than kLdsTracks tracks)
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+              more = __ballot(changed) != 0;
               --maxiter;
               if (maxiter <= 0)
                 more = false;
             }
+            bool split = false;
             if (!(0 == wnew[0] || 0 == wnew[1])) {
               const float dist2 = (znew[0] - znew[1]) * (znew[0] - znew[1]);
               const float chi2Dist = dist2 / (1.f / wnew[0] + 1.f / wnew[1]);
               split = !(chi2Dist < 4);
             }
+            if (lane == 0 && split)
+              L.counts[v] = vm | 0x80000000u;
           }
+          __syncthreads();
+          const bool split = kv < kMaxVertices && (L.counts[kv] >> 31) != 0;
           uint32_t tot;
           const uint32_t ex = wgExclusiveScan<kVB>(split ? 1u : 0u, tot, L.scan);
           if (split) {
             // the new id is >= nvf, so no other vertex of this pass sees the moved tracks
             const int32_t igv = int32_t(L.nvIntermediate + ex);
-            for (uint32_t u = 0; u < m; ++u)
-              if (cside[b + u] == 1)
-                iv[cidx[b + u]] = igv;
+            for (uint32_t u = 0; u < m; ++u) {
+              const uint32_t x = cidx[b + u];
+              if (x >> 31)
+                iv[x & 0x7fffffffu] = igv;
+            }
           }
           __syncthreads();
           if (threadIdx.x == 0)

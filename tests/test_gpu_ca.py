@@ -25,6 +25,8 @@ CASES = [
     ("pu50", 0, dict(pu_lo=5, pu_hi=5)),
     ("pu50", 0, {}),
     ("pu50", 3, {}),
+    ("pu50", 2, {}),  # 3 vertices split
+    ("pu50", 4, {}),  # 2 vertices split
     ("pu20_80", 2, {}),
     ("pu50", 5, {"error_rate": 0.1}),
 ]

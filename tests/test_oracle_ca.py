@@ -420,6 +420,25 @@ def test_vertex_split(oracle):
     assert abs(got[0] - 1.0) < 0.006 and abs(got[1] - 1.04) < 0.006
 
 
+def test_gpu_vertex_cases_split_vertices(detector, conditions, oracle):
+    """The events of the GPU vertex parity tests include vertices that splitVertices
+    splits, so those tests compare the device's split with the oracle's and not only
+    the clustering and the fits: the PU50 cases of test_gpu_ca.CASES (per-track
+    arrays in LDS), and the PU200 event of test_high_occupancy_profile (more than
+    2048 tracks: per-track arrays in global memory)."""
+    import pyoracle
+    from test_gpu_ca import CASES
+    splits = 0
+    for preset, index, kw in CASES:
+        o = oracle.process(event(detector, preset, index, **kw)[0], upto=5)
+        assert o["nVertices"] >= o["nVerticesFound"]
+        splits += o["nVertices"] - o["nVerticesFound"]
+    assert splits >= 5
+    o = pyoracle.Oracle(conditions, profile=1).process(event(detector, "pu200", 1)[0], upto=5)
+    assert o["vtxTracks"] > 2048
+    assert o["nVertices"] - o["nVerticesFound"] >= 5
+
+
 def test_chain_finds_generated_vertices(small_events):
     for name, o, truth in small_events:
         nv = o["nVertices"]
