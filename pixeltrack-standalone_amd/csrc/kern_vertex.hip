@@ -465,9 +465,13 @@ namespace pxg {
           float c = 0;
           int32_t k = -1;
           forEachOf(v, [&](uint32_t i) {
-            if (iv[i] != int32_t(v))
+            // both loads unconditional (i < nt), so a batch's loads issue together;
+            // tmp[i] is used only for the vertex's own tracks
+            const int32_t ivi = iv[i];
+            const float ci = __int_as_float(tmp[i]);
+            if (ivi != int32_t(v))
               return;
-            c += __int_as_float(tmp[i]);
+            c += ci;
             ++k;
           });
           chi2[v] = c;
@@ -531,7 +535,29 @@ namespace pxg {
             while (more) {
               znew[0] = znew[1] = 0;
               wnew[0] = wnew[1] = 0;
-              for (uint32_t u = 0; u < vm; ++u) {
+              // eight tracks' inputs are loaded before they are added, in order
+              uint32_t u = 0;
+              for (; u + 8 <= vm; u += 8) {
+                float pr[8], ww[8];
+                uint32_t sd[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                  pr[t] = cprod[vb + u + t];
+                  ww[t] = cww[vb + u + t];
+                  sd[t] = cidx[vb + u + t] >> 31;
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                  if (sd[t]) {
+                    znew[1] += pr[t];
+                    wnew[1] += ww[t];
+                  } else {
+                    znew[0] += pr[t];
+                    wnew[0] += ww[t];
+                  }
+                }
+              }
+              for (; u < vm; ++u) {
                 const float pr = cprod[vb + u], ww = cww[vb + u];
                 if (cidx[vb + u] >> 31) {
                   znew[1] += pr;
