@@ -5,7 +5,8 @@ Workload: 1000 synthetic PU50 events (50 vertices x Poisson(35) tracks, 1/pT^2
 spectrum above 0.5 GeV, ~50k digis and ~13k clusters per event), words
 resident in HBM before the timed region.  One step = one pass of the whole
 chain (raw words -> digis -> clusters -> rec-hits -> ntuplets with
-broken-line fits -> vertices) over all events.  The batched path runs B events per launch of each stage;
+broken-line fits -> vertices) over all events.  The batched path spreads the events over S
+HIP streams and runs each stream's share as equal launches of at least B events per stage;
 the plain path runs one launch sequence per event with the reference's
 per-event host synchronisation.  Prints one JSON line.
 """
@@ -177,7 +178,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--events", type=int, default=1000)
-    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--batch", type=int, default=128,
+                    help="at least this many events per launch (each stream's share is cut into equal launches)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -235,6 +237,11 @@ def main():
     # a pass enqueues every stream's batches, which then overlap on the device
     S = max(1, min(args.streams, (n + B - 1) // B))
     bounds = [round(k * n / S) for k in range(S + 1)]
+    # each stream's share is cut into equal launches of at least --batch events: a
+    # short last launch would cost a whole launch's latency for a few events
+    share = max(bounds[k + 1] - bounds[k] for k in range(S))
+    per = max(1, share // B)
+    B = (share + per - 1) // per
     streams = []
     for k in range(S):
         part = inputs[bounds[k]:bounds[k + 1]]
