@@ -315,9 +315,15 @@ namespace pxg::detail {
     return 0;
   }
 
+  void markStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto) {
+    for (uint32_t e = first; e < first + count && e < s->stagesRun.size(); ++e)
+      s->stagesRun[e] = uint8_t(from <= 1 ? upto : std::max<int>(s->stagesRun[e], upto));
+  }
+
   int runStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto) {
     if (count == 0 || first + count > s->nLoaded)
       return fail(PXG_ERR_ARG, "events [%u, %u) not loaded (%u resident)", first, first + count, s->nLoaded);
+    markStages(s, first, count, from, upto);
     const Pool v = viewFrom(s->pool, first);
     const DevConditions& c = s->ctx->cond;
     const uint32_t blockBase = s->hostEvents[first].blockOffset;
@@ -430,8 +436,9 @@ void pxg::detail::dropGraphs(pxg_stream* s) {
   s->graphs.clear();
 }
 
-int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync) {
-  if (!s || !raws || !bs || n == 0)
+int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs,
+                             uint32_t nBeamspots, bool sync) {
+  if (!s || !raws || !bs || n == 0 || (nBeamspots != 1 && nBeamspots != n))
     return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
   dropGraphs(s);
   if (n > s->pool.maxEvents)
@@ -449,7 +456,7 @@ int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws,
     d.nWords = raws[e].nWords;
     d.blockOffset = blocks;
     d.pad = 0;
-    std::memcpy(&d.bs, bs, sizeof(BeamSpotPOD));
+    std::memcpy(&d.bs, bs + (nBeamspots == 1 ? 0 : e), sizeof(BeamSpotPOD));
     if (raws[e].nWords) {
       PXG_HIP(hipMemcpyAsync(s->pool.words + off, raws[e].words, size_t(raws[e].nWords) * 4, hipMemcpyHostToDevice,
                              s->stream));
@@ -463,6 +470,7 @@ int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws,
   if (sync)
     PXG_HIP(hipStreamSynchronize(s->stream));
   s->nLoaded = n;
+  std::fill(s->stagesRun.begin(), s->stagesRun.end(), uint8_t(0));
   return 0;
 }
 
@@ -580,6 +588,7 @@ int pxg_stream_create_profile(pxg_context* ctx, uint32_t maxEvents, uint32_t max
   }
   std::memset(s->hostCounts, 0, E * sizeof(EventCounts));
   std::memset(s->hostCA, 0, E * sizeof(CACounts));
+  s->stagesRun.assign(E, 0);
   *out = s;
   return 0;
 }
@@ -615,7 +624,7 @@ int pxg_sync(pxg_stream* s) {
 }
 
 int pxg_load_events(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs) {
-  return loadEvents(s, n, raws, bs, true);
+  return loadEvents(s, n, raws, bs, 1, true);
 }
 
 int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
@@ -648,6 +657,7 @@ int pxg_run(pxg_stream* s, uint32_t first, uint32_t count, int upto) {
     it = s->graphs.emplace(key, x).first;
   }
   PXG_HIP(hipGraphLaunch(it->second, s->stream));
+  markStages(s, first, count, 1, upto);
   return 0;
 }
 
@@ -670,8 +680,10 @@ int pxg_fetch_counts(pxg_stream* s, uint32_t first, uint32_t count) {
   uint32_t st = 0;
   for (uint32_t e = first; e < first + count; ++e) {
     st |= s->hostCounts[e].status;
-    st |= s->hostCA[e].status << PXG_STATUS_CA_SHIFT;
-    st |= s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u;
+    if (s->stagesRun[e] >= 4)
+      st |= s->hostCA[e].status << PXG_STATUS_CA_SHIFT;
+    if (s->stagesRun[e] >= 5 && s->hostCA[e].vstatus)
+      st |= uint32_t(PXG_STATUS_VERTEX_OVERFLOW);
   }
   return int(st);
 }

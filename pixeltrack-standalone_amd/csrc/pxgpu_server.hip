@@ -171,14 +171,16 @@ struct pxg_server {
 int pxg_server::launch(Slot& sl) {
   const uint32_t n = uint32_t(sl.batch.size());
   std::vector<pxg_raw_view> views(n);
+  std::vector<pxg_beamspot> beamspots(n);  // each request's own beam spot
   const uint64_t t = nowNs();
   for (uint32_t e = 0; e < n; ++e) {
     Request& r = *sl.batch[e];
     views[e] = {reinterpret_cast<const uint32_t*>(r.chunk), r.chunk + 4 * kChunkWords, r.nWords};
+    beamspots[e] = r.bs;
     r.tLaunch = t;
     r.batchSize = n;
   }
-  int rc = pxg::detail::loadEvents(sl.s, n, views.data(), &sl.batch[0]->bs, false);
+  int rc = pxg::detail::loadEvents(sl.s, n, views.data(), beamspots.data(), n, false);
   if (!rc)
     rc = pxg::detail::runStages(sl.s, 0, n, 1, cfg.upto);
   if (rc)
@@ -235,8 +237,10 @@ void pxg_server::finish(Slot& sl) {
     r->counts[3] = cfg.upto >= 5 ? s->hostCA[e].nVertices : 0;
     freeChunks.push_back(r->chunk);
     r->chunk = nullptr;
-    r->status = int(s->hostCounts[e].status | (s->hostCA[e].status << PXG_STATUS_CA_SHIFT) |
-                    (s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u));
+    // counters of the stages the server runs only (the others hold an earlier batch's)
+    const uint32_t caBits = cfg.upto >= 4 ? s->hostCA[e].status << PXG_STATUS_CA_SHIFT : 0u;
+    const uint32_t vBits = cfg.upto >= 5 && s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u;
+    r->status = int(s->hostCounts[e].status | caBits | vBits);
     r->tDone = t;
     done.push_back(std::move(r));
   }

@@ -34,6 +34,9 @@ struct pxg_stream {
   pxg::EventCounts* hostCounts = nullptr;  // pinned
   uint32_t nLoaded = 0;
   uint32_t wordCapacity = 0;
+  // per resident event: the last stage run on it since it was loaded (0 none), so that
+  // counters of stages not run (left over from earlier events) are never reported
+  std::vector<uint8_t> stagesRun;
   bool profiling = false;
   uint32_t doubletLanes = 8;   // tuning: lanes per doublet slot
   uint32_t gridTarget = 2048;  // tuning: workgroups per grid-stride launch (divided among the events)
@@ -61,9 +64,13 @@ namespace pxg::detail {
   CAPool caViewFrom(const CAPool& c, uint32_t first);
   // enqueues stages [from, upto] on resident events [first, first + count)
   int runStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto);
+  // records that stages [from, upto] ran on events [first, first + count)
+  void markStages(pxg_stream* s, uint32_t first, uint32_t count, int from, int upto);
   // copies n events into the pool; host arrays must stay valid until the copies
-  // complete (pinned and kept alive by the caller when sync is false)
-  int loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, bool sync);
+  // complete (pinned and kept alive by the caller when sync is false).  bs holds
+  // nBeamspots entries: 1 (every event gets bs[0]) or n (event e gets bs[e]).
+  int loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, uint32_t nBeamspots,
+                 bool sync);
   // bytes of device memory a pool of maxEvents events and maxWords words takes
   size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps);
   // destroys the stream's captured stage sequences

@@ -361,8 +361,9 @@ class Stream:
         st = _check(self.lib.pxg_find_vertices(self._h, C.byref(t), C.byref(v)), "pxg_find_vertices")
         return st, v
 
-    def process(self, words, fed_ids, upto=3, internals=False):
-        """Plain path on one event; returns a dict of host arrays named like the oracle's fields."""
+    def process(self, words, fed_ids, upto=3, internals=False, beamspot=None):
+        """Plain path on one event; returns a dict of host arrays named like the oracle's fields.
+        beamspot: a BeamSpot (default: the conditions' beam spot)."""
         if upto == 1:
             st, d, e = self.raw_to_digi(words, fed_ids)
             out = self.digis_to_host(d)
@@ -371,7 +372,7 @@ class Stream:
             out = self.digis_to_host(d)
             out.update(self.clusters_to_host(c))
             if upto >= 3:
-                st2, h = self.rechits(d, c)
+                st2, h = self.rechits(d, c, beamspot)
                 st |= st2
                 out.update(self.hits_to_host(h))
             if upto >= 4:
@@ -574,3 +575,42 @@ class Server:
 def pool_bytes(max_events, max_words, profile=PROFILE_REFERENCE):
     """Device memory of a stream pool (pxg_pool_bytes)."""
     return int(_gpu().pxg_pool_bytes(max_events, max_words, profile))
+
+
+PRODUCT_KEYS = ("xx", "yy", "adc", "moduleInd", "clus", "pdigi", "rawIdArr", "errors", "clusInModule", "moduleId",
+                "clusModuleStart", "xl", "yl", "xerr", "yerr", "xg", "yg", "zg", "rg", "iphi", "charge", "xsize",
+                "ysize", "detInd", "hitsLayerStart", "phiOff", "phiBins")
+
+
+def stored_products(p):
+    """The products of one event (a Stream.process / event_products dict with upto=5)
+    trimmed to what the event stores: the containers' tails past them are not products."""
+    nt, nv = int(p["nTuples"]), int(p["nVertices"])
+    nb = int(p["tupleOff"][nt])
+    out = {k: p[k] for k in PRODUCT_KEYS}
+    out["moduleStart"] = p["moduleStart"][: int(p["nModules"]) + 1]
+    out["tupleOff"] = p["tupleOff"][: nt + 1]
+    out["tupleBins"] = p["tupleBins"][:nb]
+    out["detIndices"] = p["detIndices"][:nb]
+    for k in ("quality", "pt", "eta", "chi2", "idv"):
+        out[k] = p[k][:nt]
+    out["state"] = p["state"][: 5 * nt]
+    out["covariance"] = p["covariance"][: 15 * nt]
+    for k in ("zv", "wv", "chi2v", "ptv2", "sortInd", "ndof"):
+        out[k] = p[k][:nv]
+    out["counts"] = np.array([p["nDigis"], p["nModules"], p["nClusters"], nt, nv], np.int64)
+    return out
+
+
+def differing_products(a, b):
+    """Names of the stored products in which two events differ (floats compared as bit
+    patterns); empty when the two are identical."""
+    ua, ub = stored_products(a), stored_products(b)
+    bad = []
+    for k in ua:
+        x, y = np.asarray(ua[k]), np.asarray(ub[k])
+        if x.dtype.kind == "f":
+            x, y = x.view(np.uint32), y.view(np.uint32)
+        if x.shape != y.shape or not np.array_equal(x, y):
+            bad.append(k)
+    return bad

@@ -74,3 +74,72 @@ def test_server_budget_limits_batch_words(detector, gpu_ctx, gpu_stream):
         assert r["n_words"] == inputs[ids[r["id"]]][0].size
         _same(_plain(gpu_stream, *inputs[ids[r["id"]]]), sv.products(r["id"]), f"budget event {ids[r['id']]}")
     sv.close()
+
+
+def test_server_keeps_each_requests_beamspot(detector, gpu_ctx, gpu_stream):
+    """Requests batched together keep their own beam spot: the same event submitted with
+    two beam spots gives, for each, the plain path's products under that beam spot, and
+    the two results differ."""
+    from pxtrack import gpu
+    words, fed = gpu.unpack_raw(event(detector, "pu50", 21, pu_lo=20, pu_hi=20)[0], gpu_ctx.fed_ids)
+    bs0 = gpu_ctx.beamspot
+    bs1 = gpu.BeamSpot(*[getattr(bs0, n) for n, _ in gpu.BeamSpot._fields_])
+    bs1.x, bs1.y, bs1.z = bs0.x + 0.05, bs0.y - 0.03, bs0.z + 0.4
+    sv = gpu.Server(gpu_ctx, max_batch=8, n_streams=1)
+    which = {}
+    for k in range(9):  # the first request occupies the slot, the next ones wait and share a batch
+        which[sv.submit(words, fed, beamspot=bs1 if k % 2 else bs0)] = k % 2
+    results = []
+    deadline = time.time() + 60
+    while len(results) < len(which) and time.time() < deadline:
+        results += sv.poll(wait_ms=100)
+    assert len(results) == len(which)
+    launches = {}
+    for r in results:
+        launches.setdefault(r["t_launch_ns"], set()).add(which[r["id"]])
+    assert any(v == {0, 1} for v in launches.values()), "no batch held both beam spots"
+    alone = [_plain(gpu_stream, words, fed), gpu_stream.process(words, fed, upto=5, beamspot=bs1)]
+    assert not np.array_equal(alone[0]["state"].view(np.uint32), alone[1]["state"].view(np.uint32))
+    for r in results:
+        _same(alone[which[r["id"]]], sv.products(r["id"]), f"beam spot {which[r['id']]} (batch of {r['batch_size']})")
+    sv.close()
+
+
+def _serve(sv, inputs, rate, seed):
+    """Submits inputs with Poisson arrivals at `rate` events/s; returns the results by input."""
+    rng = np.random.default_rng(seed)
+    ids, t = {}, time.perf_counter()
+    for i in rng.permutation(len(inputs)):
+        t += rng.exponential(1.0 / rate) if rate else 0.0
+        while time.perf_counter() < t:
+            pass
+        ids[sv.submit(*inputs[i])] = i
+    results = []
+    deadline = time.time() + 120
+    while len(results) < len(ids) and time.time() < deadline:
+        results += sv.poll(wait_ms=100)
+    assert len(results) == len(ids), f"{len(results)} of {len(ids)} completed"
+    return {ids[r["id"]]: r for r in results}
+
+
+@pytest.mark.parametrize("budget_gb,n_events,rate", [(64, 256, 10000), (256, 96, None)])
+def test_server_poisson60_under_budget(detector, gpu_ctx, gpu_stream, budget_gb, n_events, rate):
+    """The serving configuration of the benchmark: pileup Poisson(60) clipped to [10, 200],
+    batches of up to 128 events on 3 slots under an HBM budget of 64 GB (Poisson arrivals
+    at 10k events/s) and 256 GB (all at once); every event's products equal the plain path's."""
+    from pxtrack import gen, gpu
+    cfg = gen.preset("pu_poisson60")
+    raws = [detector.event(cfg, 1000 + i)[0] for i in range(n_events)]
+    inputs = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in raws]
+    sv = gpu.Server(gpu_ctx, max_batch=128, n_streams=3, hbm_budget=int(budget_gb * 1e9))
+    res = _serve(sv, inputs, rate, seed=budget_gb)
+    sizes = [r["batch_size"] for r in res.values()]
+    print(f"budget {budget_gb} GB: {len(res)} events in {sv.info()['batches']} batches, largest {max(sizes)}",
+          flush=True)
+    assert max(sizes) > 1
+    for i, r in res.items():
+        assert r["n_words"] == inputs[i][0].size
+        alone = _plain(gpu_stream, *inputs[i])
+        assert r["status"] == alone["status"], f"event {i}: status {r['status']} != {alone['status']}"
+        _same(alone, sv.products(r["id"]), f"poisson60 event {i} (batch of {r['batch_size']}, {budget_gb} GB)")
+    sv.close()

@@ -83,3 +83,16 @@ def test_batched_equals_plain_and_oracle(detector, conditions, oracle, gpu_ctx):
         g = s.event_products(e)
         _compare(g, oracle.process(raws[e]), f"sub-batch event {e}")
     s.close()
+
+
+def test_status_reports_only_the_stages_run(detector, conditions, gpu_ctx):
+    """A raw-to-cluster call reports the status of its own stages, not the overflow
+    bits an earlier event left in the stream's ntuplet counters."""
+    from pxtrack import gpu
+    s = gpu_ctx.stream(max_events=1, max_words=400000)
+    big = gpu.unpack_raw(event(detector, "pu200", 1)[0], conditions["fed_ids"])
+    assert s.process(*big, upto=5)["status"] >> 8 & 1, "the PU200 event overflows the reference cell capacity"
+    small = gpu.unpack_raw(event(detector, "serial100", 4)[0], conditions["fed_ids"])
+    assert s.process(*small, upto=2)["status"] == 0
+    assert s.process(*small, upto=3)["status"] == 0
+    s.close()
