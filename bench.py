@@ -5,13 +5,25 @@ Workload: 1000 synthetic PU50 events (50 vertices x Poisson(35) tracks, 1/pT^2
 spectrum above 0.5 GeV, ~50k digis and ~13k clusters per event), words
 resident in HBM before the timed region.  One step = one pass of the whole
 chain (raw words -> digis -> clusters -> rec-hits -> ntuplets with
-broken-line fits -> vertices) over all events.  The batched path spreads the events over S
-HIP streams and runs each stream's share as equal launches of at least B events per stage;
-the plain path runs one launch sequence per event with the reference's
-per-event host synchronisation.  Prints one JSON line.
+broken-line fits -> vertices) over all events.  The batched path spreads the
+events over S HIP streams and runs each stream's share as equal launches of at
+least B events per stage; the plain path runs one launch sequence per event
+with the reference's per-event host synchronisation.  Prints one JSON line.
+
+Phases: autotune, warmup, the timed steps (`value`, no instrumentation), the
+same steps again with a HIP event pair around every kernel on its own stream
+(per-kernel durations under overlap), the first stream's launches alone
+(durations without overlap), a parity check of one event per launch against
+the plain path, the plain path, the serving loop and the CPU baseline.
+Each phase is a ROCTx range, so that a `rocprofv3 --kernel-trace
+--marker-trace` run of this script can be summarised per phase with
+`python bench.py --summarize-trace DIR` (writes the profiles' markdown).
 """
 import argparse
 import concurrent.futures as cf
+import csv
+import ctypes
+import glob
 import json
 import os
 import sys
@@ -26,69 +38,82 @@ METRIC = "events/s at PU50-like load; p50/p99 latency, batched vs plain per-even
 STAGES = {1: "raw->digis", 2: "raw->clusters", 3: "raw->rec-hits", 4: "raw->ntuplets+fits",
           5: "raw->ntuplets+fits->vertices"}
 HBM_PEAK_GBS = 8000.0
+# host threads of the GPU box's CPU share for one GPU (the rest of the host's cores
+# belong to the other GPUs' jobs)
+CPU_SHARE = 16
 
-# Algorithmic bytes per event of each kernel: the unique input it must read and
-# the output it must write, with the reference's element widths, from the mean
-# per-event counters (c): W words, D digis, H hits, M module runs, S doublet
-# slots (layer pair x inner hit), C cells, N neighbour entries, T tuples, TH
-# tuple hits, F fitted tuples, V vertex-finder tracks.  Per hit the CA kernels
-# read x, y, z, r (16 B), iphi, ysize, detInd (6 B); a cell is inner + outer hit
-# (4 B), pair and flags (2 B), list slots and counts (13 B).
-def algorithmic_bytes(kernel, c):
-    W, D, H, M = c["W"], c["D"], c["H"], c["M"]
-    S, C, N, T, TH, F, V = c["S"], c["C"], c["N"], c["T"], c["TH"], c["F"], c["V"]
-    table = {
-        # stage A/B: decode reads words (4 B) + fedIds (0.5 B) per word, writes seven digi
-        # arrays (20 B) and reads ~2 B of gains per digi; clusterize reads x, y, adc,
-        # moduleInd (8 B) and writes clus + moduleInd (6 B); hits reads 12 B per digi and
-        # writes 44 B per hit; moduleStarts reads 16 B of masks per 64 words and writes
-        # 4 B per module run; phiBinner reads iphi and writes the bin contents + offsets,
-        # and copies each hit's packed position and attributes (24 B) into histogram order
-        "decode": 24.5 * W + 2 * D,
-        "moduleStarts": W / 64 * 16 + 4 * M,
-        "clusterize": 14 * D,
-        "hits": 12 * D + 44 * H,
-        "phiBinner": 4 * H + 48 * H + 5124,
-        # doublets: hits (z, r, iphi, ysize, detInd: 14 B) + phi histogram (2 B per hit
-        # + 5 kB) in, one count (4 B) per slot out; the fill pass writes the cells
-        "doubletCount": 16 * H + 5124 + 4 * S,
-        "doubletScan": 8 * S,
-        "doubletFill": 16 * H + 5124 + 4 * S + 19 * C + 4 * H,
-        "outerHitScan": 8 * H,
-        "outerHitFill": 2 * C + 4 * H + 4 * C,
-        "outerHitSort": 8 * C + C,
-        # connect: every cell (6 B), the hits' x, y, z, r, detInd (18 B), the slot offsets,
-        # the neighbour lists written (4 B per entry) and the used bits
-        "connect": 6 * C + 18 * H + 4 * S + 4 * N + 5 * C + C / 8,
-        "fishbone": 4 * C + 2 * C + 16 * H + C / 8 + C,
-        "ntupletCount": 7 * C + 4 * N + 8 * C,
-        "ntupletScan": 16 * C,
-        "ntupletFill": 7 * C + 4 * N + 8 * C + 4 * T + 2 * TH + 16 * T + T,
-        "trackListAlloc": 12 * C,
-        "trackListFill": 16 * T + 8 * C + 2 * TH,
-        "trackListSort": 8 * C + 2 * TH + 4 * T + T,
-        "multiplicity": 5 * T + 2 * T + 4 * TH + 92 * T,
-        # fit: per fitted tuple the offsets and hit indices, per hit x, y, z, xerr, yerr,
-        # detInd (22 B) and its 36 B local frame, 92 B of results out
-        "fit": F * (4 + 2 * 4 + 4 * (22 + 36) + 92),
-        "classify": T * (4 + 1 + 4 * 5 + 12 + 1),
-        # dupPreds: per cell with two or more tuples (MU) its list (2 B per tuple, ~3), the
-        # tuples' cells (16 B each) and those cells' counts (4 B, ~3 per tuple), 33 B out
-        "dupPreds": c.get("MU", 0.0) * (4 + 4 + 3 * (2 + 16 + 12) + 33),
-        "fastDuplicate": 8 * C + 2 * TH + 4 * T + T,
-        "hitToTupleCount": T * 5 + 2 * TH + 4 * H,
-        "hitToTupleScan": 12 * H,
-        "hitToTupleFill": T * 5 + 2 * TH + 8 * H + 2 * TH,
-        "tripletCleaner": 8 * H + 2 * TH + 4 * T + T,
-        "vertices": T * (4 + 1 + 4 + 4 + 4) + V * 24 + 4096 * 5,
-    }
-    return table.get(kernel)
+# Canonical algorithmic bytes per event of each stage: the unique input a stage must
+# read and the output it must write, with the reference's element widths, from the
+# per-event counters W raw words, D digis, H hits, C doublets, L accepted cell links,
+# T ntuplets, N hits per ntuplet, T' tracks given to the vertexer, V vertices.  CA
+# records count at a minimal 8 bytes per cell, whatever this library's layout is.
+# Each row lists the profiler labels of the kernels that implement it.
+ROWS = {
+    "K1-K3 decode+calib+countModules": (("decode",), lambda c: 4.5 * c["W"] + 16 * c["W"] + 12 * c["D"] + 6 * c["D"]),
+    "K6 module scan (ordered starts)": (("moduleStarts",), lambda c: 16384.0),
+    "K4+K5 findClus+chargeCut (+K6 clamp scan)": (("clusterize",), lambda c: 10 * c["D"] + 14 * c["D"] + 16384),
+    "K7 getHits": (("hits",), lambda c: 12 * c["D"] + 44 * c["H"]),
+    "K8 layer start + phi binner": (("phiBinner",), lambda c: 6 * c["H"] + 5120),
+    "K9 doublets": (("doubletCount", "doubletScan", "doubletFill", "outerHitScan", "outerHitFill"),
+                    lambda c: 14 * c["H"] + 12 * c["C"]),
+    "K10 connect": (("connect",), lambda c: 8 * c["C"] + 16 * c["H"] + 4 * c["L"]),
+    "K11 fishbone": (("fishbone",), lambda c: 12 * c["C"] + 16 * c["H"]),
+    "K12 find_ntuplets": (("startMask", "startCompact", "ntupletCount", "ntupletScan", "ntupletFill"),
+                          lambda c: 8 * c["C"] + 4 * c["L"] + 14 * c["T"]),
+    "K13 cleaners/multiplicity/classify": (("trackListAlloc", "trackListFill", "trackListSort", "multiplicity",
+                                            "classify", "dupPreds", "fastDuplicate", "hitToTupleCount",
+                                            "hitToTupleScan", "hitToTupleFill", "tripletCleaner"),
+                                           lambda c: 8 * c["C"] + 300 * c["T"]),
+    "K14 broken-line fits": (("fit",), lambda c: 22 * c["Nbar"] * c["T"] + 92 * c["T"]),
+    "K15 vertexer": (("vertices",), lambda c: 30 * c["T"] + 60 * c["Tp"] + 16 * c["V"]),
+}
+LABEL_ROW = {lab: row for row, (labs, _) in ROWS.items() for lab in labs}
+# device kernel symbol -> profiler label (several kernels make up "clusterize")
+KERNEL_LABEL = {
+    "decodeKernel": "decode", "moduleStartKernel": "moduleStarts", "clusterSmallKernel": "clusterize",
+    "clusterBigKernel": "clusterize", "clusStartKernel": "clusterize", "hitsKernel": "hits",
+    "phiBinnerKernel": "phiBinner", "doubletCountKernel": "doubletCount", "doubletScanKernel": "doubletScan",
+    "doubletFillKernel": "doubletFill", "ohScanKernel": "outerHitScan", "ohFillKernel": "outerHitFill",
+    "connectKernel": "connect", "fishboneKernel": "fishbone", "startMaskKernel": "startMask",
+    "startCompactKernel": "startCompact", "ntupletCountKernel": "ntupletCount", "ntupletScanKernel": "ntupletScan",
+    "ntupletFillKernel": "ntupletFill", "trackListAllocKernel": "trackListAlloc",
+    "trackListFillKernel": "trackListFill", "trackListSortKernel": "trackListSort",
+    "multiplicityKernel": "multiplicity", "fitKernel": "fit", "classifyKernel": "classify",
+    "dupPredsKernel": "dupPreds", "fastDuplicateKernel": "fastDuplicate", "hitToTupleCountKernel": "hitToTupleCount",
+    "hitToTupleScanKernel": "hitToTupleScan", "hitToTupleFillKernel": "hitToTupleFill",
+    "tripletCleanerKernel": "tripletCleaner", "vertexKernel": "vertices",
+}
 
 
-def gen_events(detector, cfg, n, threads):
-    from pxtrack import gen  # noqa: F401
+def row_bytes(row, c):
+    return ROWS[row][1](c)
+
+
+class Roctx:
+    """ROCTx ranges around the phases (no-ops when the library is absent)."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("libroctx64.so", "/opt/rocm/lib/libroctx64.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+                break
+            except OSError:
+                continue
+
+    def push(self, name):
+        if self.lib:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib:
+            self.lib.roctxRangePop()
+
+
+def gen_events(detector, cfg, n, threads, first=0):
     with cf.ThreadPoolExecutor(threads) as ex:
-        return list(ex.map(lambda i: detector.event(cfg, i)[0], range(n)))
+        return list(ex.map(lambda i: detector.event(cfg, i)[0], range(first, first + n)))
 
 
 def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
@@ -138,7 +163,7 @@ def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
 def serving_sweep(ctx, det, cond, args, tuned):
     """Continuous-batching serving loop vs the plain per-event path at the same arrival rates."""
     from pxtrack import gen, gpu
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(CPU_SHARE, os.cpu_count() or 1)
     raws = gen_events(det, gen.preset("pu_poisson60"), args.serve_events, threads)
     ev = [gpu.unpack_raw(r, cond["fed_ids"]) for r in raws]
     budget = int(args.serve_budget_gb * 1e9)
@@ -172,6 +197,22 @@ def serving_sweep(ctx, det, cond, args, tuned):
     return out
 
 
+def roofline_of(label, prof, counters, steps_events):
+    """Roofline entry of one profiled kernel label: its row's algorithmic bytes per event x
+    events per launch / average launch duration."""
+    calls, total_ms = prof[label]
+    row = LABEL_ROW[label]
+    per_launch = steps_events / calls
+    alg = row_bytes(row, counters) * per_launch
+    avg_s = total_ms / 1e3 / calls
+    achieved = alg / avg_s / 1e9
+    return {"bound": "hbm", "kernel": label, "row": row, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "algorithmic_bytes_per_event": round(row_bytes(row, counters)),
+            "algorithmic_bytes_per_launch": round(alg), "avg_launch_ms": round(avg_s * 1e3, 4), "launches": calls,
+            "events_per_launch": round(per_launch, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,7 +241,22 @@ def main():
     ap.add_argument("--isolated-passes", type=int, default=2,
                     help="passes over the first stream's batches with the other streams idle, for the "
                          "per-kernel durations without overlap")
+    ap.add_argument("--only-timed", action="store_true",
+                    help="stop after the timed steps (for counter runs: every chain dispatch is a timed one)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="summarise a rocprofv3 --kernel-trace --marker-trace csv output directory of this "
+                         "script per phase (markdown on stdout) and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc csv output directories (FETCH_SIZE, WRITE_SIZE) of an "
+                         "--only-timed --calibrate-copy run of this script into HBM bytes per event and kernel "
+                         "(json on stdout, markdown on stderr) and exit; needs --events and --steps of that run")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.events * args.steps, args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: each box here has one MI355X and events shard across GPUs "
               f"without any exchange, so --gpus {args.gpus} is not run.", file=sys.stderr)
@@ -209,6 +265,7 @@ def main():
     from pxtrack import gen, gpu
     import torch
 
+    rtx = Roctx()
     if args.calibrate_copy > 0:
         nbytes = args.calibrate_copy << 20
         a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -223,7 +280,7 @@ def main():
     det = gen.Detector()
     cond = det.conditions()
     cfg = gen.preset("pu50")
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(CPU_SHARE, os.cpu_count() or 1)
     raws = gen_events(det, cfg, args.events, threads)
     inputs = [gpu.unpack_raw(r, cond["fed_ids"]) for r in raws]
     W = np.array([len(w) for w, _ in inputs], dtype=np.int64)
@@ -253,12 +310,12 @@ def main():
         st.load(part)
         streams.append(st)
     stream = streams[0]
+    launches = [(k, first, min(B, bounds[k + 1] - bounds[k] - first)) for k in range(S)
+                for first in range(0, bounds[k + 1] - bounds[k], B)]
 
     def batched_pass():
-        for k, st in enumerate(streams):
-            m = bounds[k + 1] - bounds[k]
-            for first in range(0, m, B):
-                st.run(first, min(B, m - first), args.upto)
+        for k, first, count in launches:
+            streams[k].run(first, count, args.upto)
         status = 0
         for k, st in enumerate(streams):
             status |= st.fetch_counts(0, bounds[k + 1] - bounds[k])
@@ -267,21 +324,21 @@ def main():
     # ---- launch shape chosen by the autotuner on the first batch
     tuned = None
     if not args.no_tune:
+        rtx.push("autotune")
         lanes, grid, tms = stream.autotune(0, min(B, bounds[1]), args.upto)
+        rtx.pop()
         for st in streams[1:]:
             st.set_tuning(lanes, grid)
         tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": round(tms, 4)}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
 
-    # ---- batched path: warmup, then K timed steps (one step = all events)
+    # ---- batched path: warmup, then K timed steps (one step = all events), nothing instrumented
+    rtx.push("warmup")
     for _ in range(args.warmup):
         batched_pass()
     torch.cuda.synchronize()
-    for st in streams:
-        st.profile(True)
-        st.profile_read()
-        st.sync()
-    torch.cuda.synchronize()
+    rtx.pop()
+    rtx.push("timed")
     t_start = time.perf_counter()
     for _ in range(args.steps):
         status = batched_pass()
@@ -289,75 +346,134 @@ def main():
         st.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    rtx.pop()
+    ms_per_step = 1e3 * elapsed / args.steps
+    evps = n * args.steps / elapsed
+    print(f"[bench] timed: {evps:.1f} events/s, {ms_per_step:.3f} ms per step, status {status}", file=sys.stderr,
+          flush=True)
+    if args.only_timed:
+        print(json.dumps({"metric": METRIC, "value": round(evps, 1), "unit": "events/s", "only_timed": True,
+                          "events": n, "steps": args.steps, "status": status}))
+        return
+
+    # ---- the same steps with every kernel bracketed by HIP events on its stream
+    for st in streams:
+        st.profile(True)
+        st.profile_read()
+    rtx.push("profiled")
+    t_prof = time.perf_counter()
+    for _ in range(args.steps):
+        batched_pass()
+    for st in streams:
+        st.sync()
+    elapsed_prof = time.perf_counter() - t_prof
+    rtx.pop()
     prof = {}
     for st in streams:
         for k, (c, ms) in st.profile_read().items():
             a = prof.get(k, (0, 0.0))
             prof[k] = (a[0] + c, a[1] + ms)
         st.profile(False)
-    ms_per_step = 1e3 * elapsed / args.steps
-    evps = n * args.steps / elapsed
 
-    # ---- the first stream's batches alone (no overlap with the other streams), for the
-    # kernels' own durations
+    # ---- the first stream's batches alone (no overlap with the other streams)
     iso = {}
+    iso_events = 0
     if args.isolated_passes > 0:
         st = streams[0]
         st.profile(True)
         st.profile_read()
+        rtx.push("isolated")
         for _ in range(args.isolated_passes):
-            for first in range(0, bounds[1], B):
-                st.run(first, min(B, bounds[1] - first), args.upto)
+            for k, first, count in launches:
+                if k == 0:
+                    st.run(first, count, args.upto)
         st.sync()
+        rtx.pop()
         iso = st.profile_read()
         st.profile(False)
         iso_events = bounds[1] * args.isolated_passes
+
+    # ---- parity: one event of every launch of the last pass against the plain path
+    parity = None
+    if args.upto >= 5:
+        rtx.push("parity")
+        plain_stream = ctx.stream(max_events=1, max_words=int(W.max()))
+        checked, bad = 0, []
+        for k, first, count in launches:
+            e = first + count // 2
+            got = streams[k].event_products(e, upto=5)
+            alone = plain_stream.process(*inputs[bounds[k] + e], upto=5)
+            diff = gpu.differing_products(alone, got)
+            checked += 1
+            if diff:
+                bad.append({"stream": k, "event": e, "differ": diff})
+        plain_stream.close()
+        rtx.pop()
+        parity = {"events_checked": checked, "identical_to_plain": not bad, "mismatches": bad}
+        if bad:
+            print(f"[bench] PARITY FAILURE: batched products differ from the plain path: {bad}", file=sys.stderr,
+                  flush=True)
+            print(json.dumps({"metric": METRIC, "error": "batched products differ from the plain path",
+                              "parity": parity}))
+            sys.exit(3)
 
     # per-event counters for the algorithmic bytes
     where = [(st, e) for k, st in enumerate(streams) for e in range(bounds[k + 1] - bounds[k])]
     cnts = np.array([st.event_counts(e) for st, e in where], dtype=np.float64)
     mean = {"W": float(W.mean()), "D": float(W.mean()), "M": float(cnts[:, 0].mean()), "H": float(cnts[:, 1].mean())}
+    for key in ("C", "L", "T", "TH", "F", "Tp", "V", "Nbar"):
+        mean[key] = 0.0
     if args.upto >= 4:
         cac = [st.ca_counts(e) for st, e in where]
-        for key, name in (("S", "nSlots"), ("C", "nCells"), ("N", "nNeighbors"), ("T", "nTuples"), ("TH", "apcN"),
-                          ("F", "nMult"), ("V", "vtxTracks"), ("MU", "nMulti")):
+        for key, name in (("C", "nCells"), ("L", "nNeighbors"), ("T", "nTuples"), ("TH", "apcN"), ("F", "nMult"),
+                          ("Tp", "vtxTracks"), ("V", "nVertices")):
             mean[key] = float(np.mean([c[name] for c in cac]))
-        mean["vertices"] = float(np.mean([c["nVertices"] for c in cac]))
-    else:
-        for key in ("S", "C", "N", "T", "TH", "F", "V", "MU"):
-            mean[key] = 0.0
-    launches_per_kernel = {k: v[0] for k, v in prof.items()}
-    # the dominant kernel: the most time with no other stream running (durations in the
-    # timed region depend on what the other streams overlap them with)
-    rank = iso if iso else prof
-    kern = max((k for k in rank if k in prof), key=lambda k: rank[k][1])
-    calls, total_ms = prof[kern]
-    avg_s = total_ms / 1e3 / calls
-    events_per_launch = n * args.steps / calls
-    alg = algorithmic_bytes(kern, mean) * events_per_launch
-    achieved = alg / avg_s / 1e9
-    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-    kernel_ms_iso = {k: round(v[1] / v[0], 4) for k, v in sorted(iso.items(), key=lambda kv: -kv[1][1])}
-    traffic, iso_roof = None, None
-    tfile = os.path.join(REPO, "profiles", "traffic_chain_pu50_batched128.json")
-    per_event = json.load(open(tfile))["kernels"] if os.path.exists(tfile) else {}
-    if kern in per_event:
-        traffic = round(per_event[kern]["hbm_bytes_per_event"] * events_per_launch)
-    if kern in iso:
-        ic, ims = iso[kern]
-        iso_per_launch = iso_events / ic
-        ialg = algorithmic_bytes(kern, mean) * iso_per_launch
-        iso_roof = {"avg_launch_ms": round(ims / ic, 4), "launches": ic,
-                    "events_per_launch": round(iso_per_launch, 1), "algorithmic_bytes_per_launch": round(ialg),
-                    "achieved": round(ialg / (ims / ic / 1e3) / 1e9, 2),
-                    "frac": round(ialg / (ims / ic / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
-        if kern in per_event:
-            iso_roof["traffic"] = round(per_event[kern]["hbm_bytes_per_event"] * iso_per_launch)
-    share = {k: round(v[1] / sum(x[1] for x in prof.values()), 4) for k, v in prof.items()}
+        mean["Nbar"] = mean["TH"] / mean["T"] if mean["T"] else 0.0
+
+    # roofline: the kernel with the largest share of the profiled timed steps, and connect
+    steps_events = n * args.steps
+    single = [lab for lab in prof if lab in LABEL_ROW and len(ROWS[LABEL_ROW[lab]][0]) == 1]
+    total_prof = sum(v[1] for v in prof.values())
+    share = {k: round(v[1] / total_prof, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    kern = max(single, key=lambda k: prof[k][1])
+    tfile = os.path.join(REPO, "profiles", "traffic_chain_pu50_batched167.json")
+    traffic_per_event = json.load(open(tfile))["labels"] if os.path.exists(tfile) else {}
+
+    def with_extras(label):
+        r = roofline_of(label, prof, mean, steps_events)
+        r["traffic"] = (round(traffic_per_event[label] * r["events_per_launch"]) if label in traffic_per_event
+                        else None)
+        if label in iso:
+            ic, ims = iso[label]
+            per_launch = iso_events / ic
+            alg = row_bytes(LABEL_ROW[label], mean) * per_launch
+            r["isolated"] = {"avg_launch_ms": round(ims / ic, 4), "launches": ic,
+                             "events_per_launch": round(per_launch, 1),
+                             "achieved": round(alg / (ims / ic / 1e3) / 1e9, 2),
+                             "frac": round(alg / (ims / ic / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
+        return r
+
+    roof = with_extras(kern)
+    others = {lab: with_extras(lab) for lab in ("connect", "fit") if lab in prof and lab != kern}
+    # every row of the canonical table: summed kernel time per launch of the row's kernels
+    rows = {}
+    for row, (labs, _) in ROWS.items():
+        have = [lab for lab in labs if lab in prof]
+        if not have:
+            continue
+        per_launch_ms = sum(prof[lab][1] / prof[lab][0] for lab in have)
+        per_launch_events = steps_events / prof[have[0]][0]
+        alg = row_bytes(row, mean) * per_launch_events
+        rows[row] = {"ms_per_launch": round(per_launch_ms, 4), "bytes_per_event": round(row_bytes(row, mean)),
+                     "frac": round(alg / (per_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
+    chain_bytes = sum(row_bytes(row, mean) for row in ROWS) * n
+    chain = {"algorithmic_bytes_per_step": round(chain_bytes), "frac_of_hbm_peak_per_step":
+             round(chain_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
 
     # ---- plain per-event path: one launch sequence + host sync per event
     plain = None
     if not args.no_plain:
+        rtx.push("plain")
         lat = []
         for st, e in where[:50]:  # warmup
             st.run(e, 1, args.upto)
@@ -369,6 +485,7 @@ def main():
             st.fetch_counts(e, 1)
             lat.append(time.perf_counter() - a)
         tp = time.perf_counter() - t1
+        rtx.pop()
         lat = np.array(lat) * 1e3
         plain = {"events_per_s": round(n / tp, 1), "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
                  "latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
@@ -377,7 +494,9 @@ def main():
     # ---- serving loop: continuous batching under Poisson arrivals
     serving = None
     if not args.no_serve:
+        rtx.push("serving")
         serving = serving_sweep(ctx, det, cond, args, tuned)
+        rtx.pop()
 
     # ---- CPU baseline: the oracle (serial restatement) on the host cores, bounded sample
     cpu = None
@@ -386,7 +505,7 @@ def main():
         import pyoracle
         orc = pyoracle.Oracle(cond)
         sample = raws[:100]
-        ncores = min(16, os.cpu_count() or 1)
+        ncores = min(CPU_SHARE, os.cpu_count() or 1)
         a = time.perf_counter()
         done = orc.run_many(sample, upto=args.upto, threads=ncores, repeat=1)
         one = time.perf_counter() - a
@@ -409,9 +528,13 @@ def main():
         cpu = {"value": round(done / ct, 1), "unit": "events/s", "cores": ncores, "kind": "port",
                "sample": f"{len(sample)} of the PU50 events x {rep} passes, {STAGES[args.upto]}, one event per thread "
                          f"at a time ({ncores} threads), {ct:.1f} s",
+               "host_cpu_count": os.cpu_count(),
+               "cores_note": f"{CPU_SHARE} threads: the host CPU share of one GPU on the box",
                "single_thread_ms_per_event": round(one_ms, 3), "cpu_model": model}
 
-    batch_lat_ms = ms_per_step / ((n + B - 1) // B)
+    batch_lat_ms = ms_per_step / len(launches)
+    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    kernel_ms_iso = {k: round(v[1] / v[0], 4) for k, v in sorted(iso.items(), key=lambda kv: -kv[1][1])}
     out = {
         "metric": METRIC,
         "value": round(evps, 1),
@@ -430,26 +553,134 @@ def main():
                    "fit": "riemann" if args.riemann_fit else "broken line",
                    "graphs": bool(args.graphs),
                    "events": n, "batch": B, "streams": S, "mean_words": round(float(W.mean()), 1),
-                   "mean_per_event": {k: (round(v, 1) if v is not None else None) for k, v in mean.items()},
+                   "mean_per_event": {k: round(v, 2) for k, v in mean.items()},
                    "stages": STAGES[args.upto]},
         "batched": {"events_per_s": round(evps, 1), "batch_latency_ms": round(batch_lat_ms, 4),
-                    "status": status, "tuning": tuned},
+                    "status": status, "tuning": tuned,
+                    "profiled_pass_events_per_s": round(steps_events / elapsed_prof, 1)},
+        "parity": parity,
         "plain": plain,
         "speedup_batched_vs_plain": round(evps / plain["events_per_s"], 2) if plain else None,
         "serving": serving,
         "kernel_ms_per_launch": kernel_ms,
         "kernel_ms_per_launch_isolated": kernel_ms_iso,
         "kernel_time_share": share,
-        "launches": launches_per_kernel,
-        "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(alg), "avg_launch_ms": round(avg_s * 1e3, 4),
-                     "launches": calls, "events_per_launch": round(events_per_launch, 1),
-                     "isolated": iso_roof},
+        "roofline": roof,
+        "roofline_other_kernels": others,
+        "roofline_rows": rows,
+        "chain": chain,
         "cpu_baseline": cpu,
         "device": gpu.device_name(),
     }
     print(json.dumps(out))
+
+
+# ---- rocprofv3 trace summary ------------------------------------------------------------
+
+def _read_csv(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def summarize_trace(d):
+    """Markdown summary of a `rocprofv3 --kernel-trace --marker-trace -f csv` run of this
+    script: per kernel symbol, the launches and average duration in each phase (the ROCTx
+    ranges this script opens; every phase ends with a device synchronisation, so each
+    dispatch belongs to the phase it started in)."""
+    kfile = sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True))
+    mfile = sorted(glob.glob(os.path.join(d, "**", "*marker_api_trace.csv"), recursive=True))
+    if not kfile:
+        sys.exit(f"no kernel_trace.csv under {d}")
+    ks = _read_csv(kfile[0])
+    phases = []
+    if mfile:
+        for m in _read_csv(mfile[0]):
+            name = m.get("Function") or m.get("Marker_Name") or m.get("Name") or ""
+            if name in ("autotune", "warmup", "timed", "profiled", "isolated", "parity", "plain", "serving"):
+                phases.append((int(m["Start_Timestamp"]), int(m["End_Timestamp"]), name))
+    order = ["timed", "profiled", "isolated", "warmup", "autotune", "parity", "plain", "serving", "other"]
+    stats = {}
+    for k in ks:
+        sym = k["Kernel_Name"].split("(")[0].split("<")[0].replace("void ", "").split("::")[-1].strip()
+        t0, t1 = int(k["Start_Timestamp"]), int(k["End_Timestamp"])
+        ph = next((p for a, b, p in phases if a <= t0 <= b), "other")
+        s = stats.setdefault(sym, {})
+        c, tot = s.get(ph, (0, 0.0))
+        s[ph] = (c + 1, tot + (t1 - t0) / 1e3)
+    tot_timed = sum(s.get("timed", (0, 0.0))[1] for s in stats.values())
+    print("| kernel | label | timed launches | timed avg us | timed share | profiled avg us | isolated avg us |")
+    print("|---|---|---|---|---|---|---|")
+    for sym, s in sorted(stats.items(), key=lambda kv: -kv[1].get("timed", (0, 0.0))[1]):
+        def avg(ph):
+            c, t = s.get(ph, (0, 0.0))
+            return f"{t / c:.2f}" if c else ""
+        c, t = s.get("timed", (0, 0.0))
+        share = f"{100 * t / tot_timed:.2f} %" if tot_timed and c else ""
+        print(f"| {sym} | {KERNEL_LABEL.get(sym, '')} | {c} | {avg('timed')} | {share} | {avg('profiled')} | "
+              f"{avg('isolated')} |")
+    if phases:
+        print()
+        for a, b, p in phases:
+            print(f"- phase `{p}`: {(b - a) / 1e6:.2f} ms")
+    else:
+        print("\n(no ROCTx phases in the trace: every dispatch counted as `other`)")
+
+
+def _pmc_per_kernel(d, counter):
+    """Sum of one counter over the dispatches of each kernel symbol in a --pmc csv run."""
+    f = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    if not f:
+        sys.exit(f"no counter_collection.csv under {d}")
+    out = {}
+    for r in _read_csv(f[0]):
+        if r.get("Counter_Name") != counter:
+            continue
+        sym = r["Kernel_Name"].split("(")[0].split("<")[0].replace("void ", "").split("::")[-1].strip()
+        c, v, m = out.get(sym, (0, 0.0, 0.0))
+        x = float(r["Counter_Value"])
+        out[sym] = (c + 1, v + x, max(m, x))
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir, events, copy_mib):
+    """HBM bytes per event and profiler label from FETCH_SIZE and WRITE_SIZE runs.  The
+    counters are scaled by what they read for a plain device copy of copy_mib MiB run three
+    times at the start of the same program (the largest dispatch of the copy kernel)."""
+    fetch, write = _pmc_per_kernel(fetch_dir, "FETCH_SIZE"), _pmc_per_kernel(write_dir, "WRITE_SIZE")
+    nbytes = copy_mib << 20
+    copy_sym = [k for k in fetch if "copy" in k.lower() or "elementwise" in k.lower()]
+    if not copy_mib or not copy_sym:
+        sys.exit("the runs hold no copy-calibration dispatch (--calibrate-copy)")
+    cf_ = max(fetch[k][2] for k in copy_sym)
+    cw_ = max(write[k][2] for k in copy_sym if k in write)
+    fpu, wpu = nbytes / cf_, nbytes / cw_
+    labels = {}
+    for sym, (c, v, _) in fetch.items():
+        lab = KERNEL_LABEL.get(sym)
+        if not lab:
+            continue
+        a = labels.setdefault(lab, [0.0, 0.0, 0])
+        a[0] += v * fpu / events
+        a[2] += c
+    for sym, (c, v, _) in write.items():
+        lab = KERNEL_LABEL.get(sym)
+        if lab:
+            labels.setdefault(lab, [0.0, 0.0, 0])[1] += v * wpu / events
+    out = {"what": "HBM bytes per event of each profiler label: FETCH_SIZE and WRITE_SIZE summed over its "
+                   "dispatches, scaled by the bytes per counter unit of a plain device copy of known size",
+           "events": events,
+           "copy_calibration": {"bytes": nbytes, "fetch_units": cf_, "write_units": cw_,
+                                "fetch_bytes_per_unit": round(fpu, 2), "write_bytes_per_unit": round(wpu, 2)},
+           "labels": {k: round(v[0] + v[1]) for k, v in sorted(labels.items(), key=lambda kv: -(kv[1][0] + kv[1][1]))},
+           "detail": {k: {"fetch_bytes_per_event": round(v[0]), "write_bytes_per_event": round(v[1]),
+                          "dispatches": v[2]} for k, v in labels.items()}}
+    print(json.dumps(out, indent=1))
+    print(f"Copy calibration: a {nbytes} B device copy reads {cf_:.0f} FETCH_SIZE units and writes {cw_:.0f} "
+          f"WRITE_SIZE units: {fpu:.1f} and {wpu:.1f} bytes per unit.  Events: {events}.\n", file=sys.stderr)
+    print("| label | fetch B/event | write B/event | total B/event | dispatches |", file=sys.stderr)
+    print("|---|---|---|---|---|", file=sys.stderr)
+    for k, v in sorted(labels.items(), key=lambda kv: -(kv[1][0] + kv[1][1])):
+        print(f"| {k} | {v[0]:.0f} | {v[1]:.0f} | {v[0] + v[1]:.0f} | {v[2]} |", file=sys.stderr)
 
 
 if __name__ == "__main__":
