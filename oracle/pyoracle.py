@@ -60,6 +60,19 @@ def _load(path, prefix):
             lib.orc_conditions_set_fishbone.restype = C.c_int
             lib.orc_conditions_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
             lib.orc_circle.argtypes = [C.c_float] * 6 + [C.POINTER(C.c_float)] * 2
+            lib.orc_fit_helix_d.restype = C.c_int
+            lib.orc_fit_helix_d.argtypes = [C.c_int, vp, vp, C.c_double] + [vp] * 6
+            lib.orc_mult_scatt.restype = C.c_double
+            lib.orc_mult_scatt.argtypes = [C.c_double] * 4
+            lib.orc_cluster_tracks.restype = C.c_int
+            lib.orc_cluster_tracks.argtypes = [C.c_int, C.c_uint32, vp, vp, vp, vp]
+            lib.orc_set_mutation.argtypes = [C.c_int]
+            lib.orc_set_variant.argtypes = [C.c_int]
+            lib.orc_circle_debug.restype = C.c_int
+            lib.orc_circle_debug.argtypes = [C.c_int, vp, vp, C.c_double, vp, vp]
+            lib.orc_fast_fit.restype = C.c_int
+            lib.orc_fast_fit.argtypes = [C.c_int, vp, vp]
+            lib.orc_translate_karimaki.argtypes = [vp, vp, C.c_double, C.c_double]
             lib.orc_run_many.restype = C.c_uint32
             lib.orc_run_many.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_uint32]
         else:
@@ -173,6 +186,63 @@ class Oracle(_Chain):
             raise ValueError(f"fit_helix: unsupported number of hits {n}")
         return st, cv, float(sc[0]), float(sc[1]), float(sc[2])
 
+    def fit_helix_d(self, hits, ge, B):
+        """Broken-line fit in double precision before the float conversion:
+        (circle (phi, d, k), its 3x3 covariance, line (cot theta, zip), its 2x2 covariance,
+        chi2 of the circle, chi2 of the line)."""
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ge = np.ascontiguousarray(ge, dtype=np.float32)
+        n = hits.shape[1]
+        c, cc, ln, lc, x2 = np.zeros(3), np.zeros(9), np.zeros(2), np.zeros(4), np.zeros(2)
+        a = lambda v: v.ctypes.data_as(C.c_void_p)  # noqa: E731
+        rc = self.lib.orc_fit_helix_d(n, a(hits), a(ge), B, a(c), a(cc), a(ln), a(lc), a(x2[0:]), a(x2[1:]))
+        if rc != 0:
+            raise ValueError("fit_helix_d: 3 to 5 hits")
+        return c, cc.reshape(3, 3), ln, lc.reshape(2, 2), x2[0], x2[1]
+
+    def fast_fit(self, hits):
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ff = np.zeros(4)
+        if self.lib.orc_fast_fit(hits.shape[1], hits.ctypes.data_as(C.c_void_p), ff.ctypes.data_as(C.c_void_p)):
+            raise ValueError("fast_fit: 3 or 4 hits")
+        return ff
+
+    def circle_debug(self, hits, ge, B, ff=None):
+        """The circle fit's intermediate values: dict with d, e (corrected first two hits),
+        par_mid, cov_mid ((phi, d, k) and covariance at the chord midpoint), ms (variance added
+        to phi at the first hit), S0, S1, R, slope.  ff: a fast fit (X0, Y0, R, tan theta) to
+        linearise about instead of the hits' own."""
+        hits = np.ascontiguousarray(hits, dtype=np.float64)
+        ge = np.ascontiguousarray(ge, dtype=np.float32)
+        out = np.zeros(21)
+        a = lambda v: v.ctypes.data_as(C.c_void_p)  # noqa: E731
+        ffp = None if ff is None else np.ascontiguousarray(ff, dtype=np.float64)
+        if self.lib.orc_circle_debug(hits.shape[1], a(hits), a(ge), B, None if ffp is None else a(ffp), a(out)) != 0:
+            raise ValueError("circle_debug: 3 or 4 hits")
+        return {"d": out[0:2], "e": out[2:4], "par_mid": out[4:7], "cov_mid": out[7:16].reshape(3, 3),
+                "ms": out[16], "S0": out[17], "S1": out[18], "R": out[19], "slope": out[20]}
+
+    def translate_karimaki(self, par, cov, x0, y0):
+        par = np.array(par, dtype=np.float64)
+        cov = np.array(cov, dtype=np.float64).reshape(9).copy()
+        a = lambda v: v.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.lib.orc_translate_karimaki(a(par), a(cov), float(x0), float(y0))
+        return par, cov.reshape(3, 3)
+
+    def mult_scatt(self, length, B, R, slope):
+        return float(self.lib.orc_mult_scatt(length, B, R, slope))
+
+    def cluster_tracks(self, zt, ezt2, algo=0):
+        """The vertex finder's track clustering alone: (cluster of each track, 9997 for
+        noise; neighbour counts; number of clusters)."""
+        zt = np.ascontiguousarray(zt, dtype=np.float32)
+        ezt2 = np.ascontiguousarray(ezt2, dtype=np.float32)
+        iv = np.zeros(zt.size, np.int32)
+        nn = np.zeros(zt.size, np.int32)
+        a = lambda v: v.ctypes.data_as(C.c_void_p)  # noqa: E731
+        found = self.lib.orc_cluster_tracks(algo, zt.size, a(zt), a(ezt2), a(iv), a(nn))
+        return iv, nn, found
+
     def fit_helix_riemann(self, hits, ge, B):
         """Riemann fit of one track, same arguments and results as fit_helix."""
         hits = np.ascontiguousarray(hits, dtype=np.float64)
@@ -267,3 +337,38 @@ class Reference(_Chain):
 
     def localY(self, n):
         return int(self.lib.ref_localY(n))
+
+
+class mutation:
+    """Context manager planting one of the oracle's mistakes (oracle.h kMut*) for the
+    tests that show their checks catch it."""
+    NONE, XXI0, CHI2_NDF, VARBETA_ONE_SIDE, NO_FIRST_MS_CIRCLE, NO_SLOPE_FACTOR, NO_FIRST_MS_LINE, LINE_WEIGHT, \
+        DENSITY_TIE, DENSITY_NO_CHI2, CU_BORDER, PHI_JACOBIAN = range(12)
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        _load(ORACLE_LIB, "orc").orc_set_mutation(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        _load(ORACLE_LIB, "orc").orc_set_mutation(0)
+        return False
+
+
+class variant:
+    """Context manager switching the oracle away from one of the reference's own expressions
+    (oracle.h kVar*), for the tests that show what those expressions do."""
+    HALF_CHARGE_REAL, PHI_JACOBIAN_SIGN = 1, 2
+
+    def __init__(self, bits):
+        self.bits = bits
+
+    def __enter__(self):
+        _load(ORACLE_LIB, "orc").orc_set_variant(self.bits)
+        return self
+
+    def __exit__(self, *exc):
+        _load(ORACLE_LIB, "orc").orc_set_variant(0)
+        return False

@@ -177,6 +177,12 @@ namespace orc {
   int fitHelix(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt, float* eta,
                float* chi2);
   void invertSPDForTest(const double* a, double* inv, int n);
+  int fitHelixD(int n, const double* hits, const float* ge, double B, double* circle, double* ccov, double* line,
+                double* lcov, double* chi2c, double* chi2l);
+  double multScattForTest(double length, double B, double R, double slope);
+  int circleDebug(int n, const double* hits, const float* ge, double B, const double* ffIn, double* out);
+  int fastFitForTest(int n, const double* hits, double* ff);
+  void translateKarimakiForTest(double* par, double* cov, double x0, double y0);
   // a18/a19 alternative: the Riemann fit (orc_riemann.cc)
   void fitTracksRiemann(const Conditions& c, Event& ev);
   int fitHelixRiemann(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt,
@@ -187,6 +193,7 @@ namespace orc {
   double rmathForTest(int fn, double x, double y);
   // a23-a25 (orc_stageD.cc)
   void findVertices(Event& ev);
+  uint32_t clusterTracks(int algo, uint32_t nt, const float* zt, const float* ezt2, int32_t* iv, int32_t* nn);
   void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
                      const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo = 0);
   int verticesDirect(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv,
@@ -198,5 +205,30 @@ namespace orc {
 
   // exposed for unit tests
   int16_t atan2s7(float y, float x);
+
+  // Planted mistakes, for the tests that show a check catches them (0: none).  Each
+  // changes one term of the restatement the way a plausible transcription error would.
+  enum : int {
+    kMutNone = 0,
+    kMutXXI0 = 1,            // multiple scattering: material per cm doubled
+    kMutChi2Ndf = 2,         // fit chi2 normalised by 2N-4 instead of 2N-5
+    kMutVarBetaOneSide = 3,  // kink variance from the next segment only
+    kMutNoFirstMSCircle = 4, // circle fit: first segment's scattering not added to phi
+    kMutNoSlopeFactor = 5,   // circle fit: kink variances without the (1 + cot^2) factor
+    kMutNoFirstMSLine = 6,   // line fit: first segment's scattering not added to the slope
+    kMutLineWeight = 7,      // line fit: hit weight from the transverse instead of the z element
+    kMutDensityTie = 8,      // density clustering: equal-density tie broken towards higher z
+    kMutDensityNoChi2 = 9,   // density clustering: neighbour count without the chi2 cut
+    kMutCuBorder = 10,       // circle fit: a term of the curvature border of C_u dropped
+    kMutPhiJacobian = 11,    // circle fit: x and y of the first radius swapped in d(phi)/d(u0)
+  };
+  extern int g_mutation;
+  // Departures from the reference's own expressions, for the tests that show what those
+  // expressions do (bits, 0: the reference's arithmetic).
+  enum : int {
+    kVarHalfChargeReal = 1,   // the circle fit's (q / 2) Jacobian factor in real, not integer, arithmetic
+    kVarPhiJacobianSign = 2,  // d(phi)/d(u0) with the sign of the derivative of atan2(e - d)
+  };
+  extern int g_variant;
 
 }  // namespace orc

@@ -14,6 +14,11 @@
 
 using namespace orc;
 
+namespace orc {
+  int g_mutation = kMutNone;
+  int g_variant = 0;
+}
+
 extern "C" {
 
 struct orc_conditions {
@@ -284,6 +289,45 @@ int orc_fit_helix(int n, const double* hits, const float* ge, double B, float* s
 }
 
 void orc_invert_spd(const double* a, double* inv, int n) { invertSPDForTest(a, inv, n); }
+
+// the broken-line fit's double-precision results (circle and line parameters and covariances,
+// the two chi2) for the tests of its covariance and chi2
+int orc_fit_helix_d(int n, const double* hits, const float* ge, double B, double* circle, double* ccov, double* line,
+                    double* lcov, double* chi2c, double* chi2l) {
+  return fitHelixD(n, hits, ge, B, circle, ccov, line, lcov, chi2c, chi2l);
+}
+
+// BrokenLine.h MultScatt: planar-angle variance of a segment of the given length
+double orc_mult_scatt(double length, double B, double R, double slope) {
+  return multScattForTest(length, B, R, slope);
+}
+
+// the track clustering of the vertex finder alone: iv (cluster per track, 9997 noise), nn
+int orc_cluster_tracks(int algo, uint32_t nt, const float* zt, const float* ezt2, int32_t* iv, int32_t* nn) {
+  if (algo < 0 || algo > 2)
+    return -1;
+  return int(clusterTracks(algo, nt, zt, ezt2, iv, nn));
+}
+
+// the broken-line circle fit's intermediate values (21 doubles: d, e, (phi, d, k) at the chord
+// midpoint, their covariance, the scattering variance added at the first hit, S0, S1, R, slope)
+int orc_circle_debug(int n, const double* hits, const float* ge, double B, const double* ff, double* out) {
+  return circleDebug(n, hits, ge, B, ff, out);
+}
+
+// BrokenLine.h BL_Fast_fit: (X0, Y0, R, tan theta) of the hits
+int orc_fast_fit(int n, const double* hits, double* ff) { return fastFitForTest(n, hits, ff); }
+
+// BrokenLine.h TranslateKarimaki on (phi, d, k) and its 3x3 covariance, in place
+void orc_translate_karimaki(double* par, double* cov, double x0, double y0) {
+  translateKarimakiForTest(par, cov, x0, y0);
+}
+
+// planted mistakes for the tests that show a check catches them (oracle.h kMut*)
+void orc_set_mutation(int m) { g_mutation = m; }
+
+// departures from the reference's own expressions (oracle.h kVar* bits)
+void orc_set_variant(int bits) { g_variant = bits; }
 
 int orc_fit_helix_riemann(int n, const double* hits, const float* ge, double B, float* state, float* cov, float* pt,
                           float* eta, float* chi2) {

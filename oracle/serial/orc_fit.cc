@@ -59,7 +59,7 @@ namespace orc {
     double multScatt(double length, double B, double R, double slope) {
       double pt2 = std::min(20., B * R);
       pt2 *= pt2;
-      constexpr double XXI_0 = 0.06 / 16.;
+      const double XXI_0 = g_mutation == kMutXXI0 ? 0.12 / 16. : 0.06 / 16.;
       constexpr double fact = 0.7 * (13.6 / 1000.) * (13.6 / 1000.);
       return fact / (pt2 * (1. + sq(slope))) * (std::abs(length) * XXI_0) *
              sq(1. + 0.038 * std::log(std::abs(length) * XXI_0));
@@ -114,6 +114,13 @@ namespace orc {
       double cov[9];
       int q;
       float chi2;
+    };
+    // intermediate values of the circle fit, for the tests that pin its covariance:
+    // the corrected first two hits d and e, (phi, d, k) and their covariance with the chord
+    // midpoint as the origin, the scattering variance added at the first hit, the first
+    // two hits' path lengths S0, S1, and the fast fit's radius and slope
+    struct CircleDebug {
+      double d[2], e[2], parMid[3], covMid[9], ms, S0, S1, R, slope;
     };
     struct LineFit {
       double par[2];
@@ -222,7 +229,8 @@ namespace orc {
       }
       d.VarBeta[0] = d.VarBeta[n - 1] = 0;
       for (int i = 1; i < n - 1; ++i)
-        d.VarBeta[i] = multScatt(d.S[i + 1] - d.S[i], B, ff[2], slope) + multScatt(d.S[i] - d.S[i - 1], B, ff[2], slope);
+        d.VarBeta[i] = multScatt(d.S[i + 1] - d.S[i], B, ff[2], slope) +
+                       (g_mutation == kMutVarBetaOneSide ? 0. : multScatt(d.S[i] - d.S[i - 1], B, ff[2], slope));
     }
 
     // BrokenLine.h:401-490; ge[6][N] float (xx, xy, yy, xz, yz, zz)
@@ -249,7 +257,7 @@ namespace orc {
         for (int b = 0; b < 2; ++b)
           RJVJt[b] = RJV[0] * J[b][0] + RJV[1] * J[b][1] + RJV[2] * J[b][2];
         const double m11 = RJVJt[0] * R.r10 + RJVJt[1] * R.r11;
-        w[i] = 1. / m11;
+        w[i] = g_mutation == kMutLineWeight ? 1. / V[2][2] : 1. / m11;
       }
       for (int i = 0; i < n; ++i)
         ru[i] = w[i] * d.Z[i];
@@ -266,7 +274,8 @@ namespace orc {
       L.par[0] = (u[1] - u[0]) / (d.S[1] - d.S[0]);
       L.par[1] = u[0];
       const double idiff = 1. / (d.S[1] - d.S[0]);
-      L.cov[0] = (I[0] - 2 * I[1] + I[n + 1]) * sq(idiff) + multScatt(d.S[1] - d.S[0], B, ff[2], slope);
+      L.cov[0] = (I[0] - 2 * I[1] + I[n + 1]) * sq(idiff) +
+                 (g_mutation == kMutNoFirstMSLine ? 0. : multScatt(d.S[1] - d.S[0], B, ff[2], slope));
       L.cov[1] = L.cov[2] = (I[1] - I[0]) * idiff;
       L.cov[3] = I[0];
       double J[4] = {1., 0, -d.S[0], 1.};
@@ -294,12 +303,12 @@ namespace orc {
     // BrokenLine.h:256-382
     template <int N>
     void circleFit(const double (&h)[3][N], const float (&ge)[6][N], const double* ff, double B, BLData<N>& d,
-                   CircleFit& c) {
+                   CircleFit& c, CircleDebug* dbg = nullptr) {
       constexpr int n = N;
       c.q = d.q;
       const double slope = -c.q / ff[3];
       for (int i = 0; i < n; ++i)
-        d.VarBeta[i] *= 1. + sq(slope);
+        d.VarBeta[i] *= g_mutation == kMutNoSlopeFactor ? 1. : 1. + sq(slope);
       for (int i = 0; i < n; ++i)
         d.Z[i] = std::sqrt(d.radii[0][i] * d.radii[0][i] + d.radii[1][i] * d.radii[1][i]) - ff[2];
       double w[N];
@@ -328,7 +337,7 @@ namespace orc {
         double cin = 0;
         if (i > 0 && i < n - 1)
           cin += -(s[i + 1] - s[i - 1]) * (s[i + 1] - s[i - 1]) / (2. * VB[i] * (s[i + 1] - s[i]) * (s[i] - s[i - 1]));
-        if (i > 1)
+        if (i > 1 && g_mutation != kMutCuBorder)
           cin += (s[i] - s[i - 2]) / (2. * VB[i - 1] * (s[i] - s[i - 1]));
         if (i < n - 2)
           cin += (s[i + 2] - s[i]) / (2. * VB[i + 1] * (s[i + 1] - s[i]));
@@ -358,8 +367,15 @@ namespace orc {
       c.par[0] = std::atan2(emdy, emdx);
       c.par[1] = -c.q * (ff[2] - std::sqrt(sq(ff[2]) - 0.25 * tmp1));
       c.par[2] = c.q * (1. / ff[2] + u[n]);
-      const int qHalf = c.q / 2;  // integer division, as in the reference: always 0
-      const double J[9] = {(d.radii[1][0] * emdx - emdy * d.radii[0][0]) / tmp1,
+      // integer division, as in the reference (BrokenLine.h:351,353): always 0, so the
+      // sagitta's variance at the chord midpoint drops out of the covariance; the
+      // kVarHalfChargeReal switch evaluates q / 2 in real arithmetic for the tests
+      const double qHalf = (g_variant & kVarHalfChargeReal) ? 0.5 * c.q : double(c.q / 2);
+      // d(phi)/d(u0) as the reference writes it (BrokenLine.h:348); the derivative of
+      // atan2(e - d) has the opposite sign, which the kVarPhiJacobianSign variant applies
+      const double dPhi0 = (g_mutation == kMutPhiJacobian ? d.radii[0][0] * emdx - emdy * d.radii[1][0]
+                                                          : d.radii[1][0] * emdx - emdy * d.radii[0][0]) / tmp1;
+      const double J[9] = {(g_variant & kVarPhiJacobianSign) ? -dPhi0 : dPhi0,
                            (d.radii[1][1] * emdx - emdy * d.radii[0][1]) / tmp1,
                            0,
                            qHalf * (emdx * d.radii[0][0] + emdy * d.radii[1][0]) / std::sqrt(sq(2 * ff[2]) - tmp1),
@@ -378,8 +394,16 @@ namespace orc {
       c.cov[7] = I[n * m + 1];
       c.cov[8] = I[n * m + n];
       sandwich3(J, c.cov);
+      if (dbg) {
+        *dbg = {{dx, dy}, {ex, ey}, {c.par[0], c.par[1], c.par[2]}, {}, 0, d.S[0], d.S[1], ff[2], slope};
+        std::memcpy(dbg->covMid, c.cov, sizeof(c.cov));
+      }
       translateKarimaki(c, 0.5 * emdx, 0.5 * emdy);
-      c.cov[0] += (1 + sq(slope)) * multScatt(d.S[1] - d.S[0], B, ff[2], slope);
+      const double msFirst = (1 + sq(slope)) * multScatt(d.S[1] - d.S[0], B, ff[2], slope);
+      if (dbg)
+        dbg->ms = msFirst;
+      if (g_mutation != kMutNoFirstMSCircle)
+        c.cov[0] += msFirst;
       translateKarimaki(c, dx, dy);
       c.chi2 = 0;
       for (int i = 0; i < n; ++i) {
@@ -444,7 +468,7 @@ namespace orc {
       cv[14] = float(L.cov[3]);
       C.pt[tkid] = float(B) / float(std::abs(Ci.par[2]));
       C.eta[tkid] = asinhf(float(L.par[0]));
-      C.chi2[tkid] = float((Ci.chi2 + L.chi2) / (2 * N - 5));
+      C.chi2[tkid] = float((Ci.chi2 + L.chi2) / (2 * N - (g_mutation == kMutChi2Ndf ? 4 : 5)));
     }
   }  // namespace
 
@@ -519,7 +543,7 @@ namespace orc {
       cov[14] = float(L.cov[3]);
       *pt = float(B) / float(std::abs(Ci.par[2]));
       *eta = asinhf(float(L.par[0]));
-      *chi2 = float((Ci.chi2 + L.chi2) / (2 * N - 5));
+      *chi2 = float((Ci.chi2 + L.chi2) / (2 * N - (g_mutation == kMutChi2Ndf ? 4 : 5)));
     };
     if (n == 3)
       run(std::integral_constant<int, 3>{});
@@ -530,6 +554,114 @@ namespace orc {
     else
       return -1;
     return 0;
+  }
+
+  // the fit's double-precision results before the float conversion: circle (phi, d, k) and
+  // its covariance at the origin, line (cot theta, zip) and its covariance, the two chi2
+  int fitHelixD(int n, const double* hits, const float* ge, double B, double* circle, double* ccov, double* line,
+                double* lcov, double* chi2c, double* chi2l) {
+    auto run = [&](auto tag) {
+      constexpr int N = decltype(tag)::value;
+      double h[3][N];
+      float g[6][N];
+      for (int r = 0; r < 3; ++r)
+        for (int i = 0; i < N; ++i)
+          h[r][i] = hits[r * n + i];
+      for (int r = 0; r < 6; ++r)
+        for (int i = 0; i < N; ++i)
+          g[r][i] = ge[r * n + i];
+      double ff[4];
+      fastFit<N>(h, ff);
+      BLData<N> d;
+      prepare<N>(h, ff, B, d);
+      LineFit L;
+      lineFit<N>(g, ff, B, d, L);
+      CircleFit Ci;
+      circleFit<N>(h, g, ff, B, d, Ci);
+      for (int k = 0; k < 3; ++k)
+        circle[k] = Ci.par[k];
+      for (int k = 0; k < 9; ++k)
+        ccov[k] = Ci.cov[k];
+      line[0] = L.par[0];
+      line[1] = L.par[1];
+      for (int k = 0; k < 4; ++k)
+        lcov[k] = L.cov[k];
+      *chi2c = Ci.chi2;
+      *chi2l = L.chi2;
+    };
+    if (n == 3)
+      run(std::integral_constant<int, 3>{});
+    else if (n == 4)
+      run(std::integral_constant<int, 4>{});
+    else if (n == 5)
+      run(std::integral_constant<int, 5>{});
+    else
+      return -1;
+    return 0;
+  }
+
+  double multScattForTest(double length, double B, double R, double slope) { return multScatt(length, B, R, slope); }
+
+  int fastFitForTest(int n, const double* hits, double* ff) {
+    auto run = [&](auto tag) {
+      constexpr int N = decltype(tag)::value;
+      double h[3][N];
+      for (int r = 0; r < 3; ++r)
+        for (int i = 0; i < N; ++i)
+          h[r][i] = hits[r * n + i];
+      fastFit<N>(h, ff);
+    };
+    if (n == 3)
+      run(std::integral_constant<int, 3>{});
+    else if (n == 4)
+      run(std::integral_constant<int, 4>{});
+    else
+      return -1;
+    return 0;
+  }
+
+  // the circle fit's intermediate values (CircleDebug, 21 doubles in field order); with ffIn
+  // the fit linearises about that fast fit (X0, Y0, R, tan theta) instead of the hits' own
+  int circleDebug(int n, const double* hits, const float* ge, double B, const double* ffIn, double* out) {
+    auto run = [&](auto tag) {
+      constexpr int N = decltype(tag)::value;
+      double h[3][N];
+      float g[6][N];
+      for (int r = 0; r < 3; ++r)
+        for (int i = 0; i < N; ++i)
+          h[r][i] = hits[r * n + i];
+      for (int r = 0; r < 6; ++r)
+        for (int i = 0; i < N; ++i)
+          g[r][i] = ge[r * n + i];
+      double ff[4];
+      fastFit<N>(h, ff);
+      if (ffIn)
+        std::memcpy(ff, ffIn, sizeof(ff));
+      BLData<N> d;
+      prepare<N>(h, ff, B, d);
+      CircleFit Ci;
+      CircleDebug dbg;
+      circleFit<N>(h, g, ff, B, d, Ci, &dbg);
+      static_assert(sizeof(CircleDebug) == 21 * sizeof(double), "debug layout");
+      std::memcpy(out, &dbg, sizeof(dbg));
+    };
+    if (n == 3)
+      run(std::integral_constant<int, 3>{});
+    else if (n == 4)
+      run(std::integral_constant<int, 4>{});
+    else
+      return -1;
+    return 0;
+  }
+
+  // TranslateKarimaki on (phi, d, k) and its covariance (row-major 3x3)
+  void translateKarimakiForTest(double* par, double* cov, double x0, double y0) {
+    CircleFit c{};
+    std::memcpy(c.par, par, sizeof(c.par));
+    std::memcpy(c.cov, cov, sizeof(c.cov));
+    translateKarimaki(c, x0, y0);
+    std::memcpy(par, c.par, sizeof(c.par));
+    std::memcpy(cov, c.cov, sizeof(c.cov));
   }
 
   void invertSPDForTest(const double* a, double* inv, int n) {

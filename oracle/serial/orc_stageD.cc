@@ -44,21 +44,14 @@ namespace orc {
     clusterAndFit(V, itrk, zt, ezt2, ptt2, ev.vertexAlgo);
   }
 
-  // density clustering, fit, split, refit and sort over loaded tracks
-  void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
-                     const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo) {
-    V.zv.assign(kMaxVertices, 0);
-    V.wv.assign(kMaxVertices, 0);
-    V.chi2.assign(kMaxVertices, 0);
-    V.ptv2.assign(kMaxVertices, 0);
-    V.sortInd.assign(kMaxVertices, 0);
+  // clusterTracksByDensity (gpuClusterTracksByDensity.h:20-207), clusterTracksDBSCAN
+  // (gpuClusterTracksDBSCAN.h:17-215) or clusterTracksIterative (gpuClusterTracksIterative.h:17-193)
+  // on nt tracks: iv[i] becomes the track's cluster (0 ... found-1, numbered by seed index) or
+  // 9997 (noise), nn[i] its neighbour count; returns the number of clusters found
+  uint32_t clusterTracks(int algo, uint32_t nt, const float* zt, const float* ezt2, int32_t* iv, int32_t* nn) {
     constexpr int minT = 2;
     constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f;
-    const uint32_t nt = uint32_t(itrk.size());
     std::vector<uint8_t> izt(nt);
-    std::vector<int32_t> iv(nt);
-    int32_t* nn = V.ndof.data();
-    V.ntrks = nt;
 
     // clusterTracksByDensity (gpuClusterTracksByDensity.h:20-207)
     const float er2mx = errmax * errmax;
@@ -190,7 +183,7 @@ namespace orc {
           float dist = std::abs(zt[i] - zt[j]);
           if (dist > eps)
             return;
-          if (dist * dist > chi2max * (ezt2[i] + ezt2[j]))
+          if (g_mutation != kMutDensityNoChi2 && dist * dist > chi2max * (ezt2[i] + ezt2[j]))
             return;
           nn[i]++;
         });
@@ -200,7 +193,7 @@ namespace orc {
         forEachNear(izt[i], [&](uint32_t j) {
           if (nn[j] < nn[i])
             return;
-          if (nn[j] == nn[i] && zt[j] >= zt[i])
+          if (nn[j] == nn[i] && (g_mutation == kMutDensityTie ? zt[j] <= zt[i] : zt[j] >= zt[i]))
             return;
           float dist = std::abs(zt[i] - zt[j]);
           if (dist > mdist)
@@ -251,6 +244,22 @@ namespace orc {
         iv[i] = iv[iv[i]];
     for (uint32_t i = 0; i < nt; ++i)
       iv[i] = -iv[i] - 1;
+    return found;
+  }
+
+  // density clustering, fit, split, refit and sort over loaded tracks
+  void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
+                     const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo) {
+    V.zv.assign(kMaxVertices, 0);
+    V.wv.assign(kMaxVertices, 0);
+    V.chi2.assign(kMaxVertices, 0);
+    V.ptv2.assign(kMaxVertices, 0);
+    V.sortInd.assign(kMaxVertices, 0);
+    const uint32_t nt = uint32_t(itrk.size());
+    std::vector<int32_t> iv(nt);
+    int32_t* nn = V.ndof.data();
+    V.ntrks = nt;
+    const uint32_t found = clusterTracks(algo, nt, zt.data(), ezt2.data(), iv.data(), nn);
     uint32_t nvIntermediate = found;
     uint32_t nvFinal = found;
     V.nFound = found;
