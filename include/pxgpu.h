@@ -129,9 +129,9 @@ typedef struct {
   uint32_t* outerHitOff;
   uint32_t* outerHitCells;  /* cells by outer hit in fill order, as u32 pairs (cell, inner hit | outer hit << 16) */
   int32_t* neighborSlot; uint8_t* neighborCount; uint32_t* neighborPool;  /* 36 per slot */
-  int32_t* trackSlot; uint16_t* trackCount; uint16_t* trackPool;          /* 48 per slot */
-  uint32_t* multOff; uint16_t* multBins;   /* tuple multiplicity */
-  uint32_t *hitToTupleOff; uint16_t* hitToTupleBins;
+  int32_t* trackSlot; uint16_t* trackCount; uint32_t* trackPool;          /* 48 tuple indices per slot */
+  uint32_t* multOff; uint32_t* multBins;   /* tuple multiplicity (tuple indices) */
+  uint32_t *hitToTupleOff; uint32_t* hitToTupleBins;
   uint32_t counts[16];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,
                             nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors, nStart */
 } pxg_ca_internals;
@@ -154,6 +154,12 @@ int pxg_stream_create_profile(pxg_context* ctx, uint32_t max_events, uint32_t ma
                               pxg_stream** out);
 /* Device bytes a stream's pool of max_events events and max_words raw words takes. */
 size_t pxg_pool_bytes(uint32_t max_events, uint32_t max_words, int profile);
+/* Capacities of a profile's ntuplet containers (the values of CAConstants.h:18-39 for
+ * PXG_PROFILE_REFERENCE).  Returns 0, or PXG_ERR_ARG for an unknown profile. */
+typedef struct {
+  uint32_t max_doublets, max_tuples, max_tuples_mult, max_hit_to_tuple, max_cell_lists, max_cells_per_hit;
+} pxg_capacities;
+int pxg_profile_capacities(int profile, pxg_capacities* out);
 void pxg_stream_destroy(pxg_stream* s);
 int pxg_sync(pxg_stream* s);
 

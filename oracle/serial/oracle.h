@@ -50,7 +50,6 @@ namespace orc {
 
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
   // run-time ones are in Event::caps (pxformats.h CACapacities)
-  constexpr int kMaxCellsPerHit = 128;
   constexpr double kBField = 0.0114256972711507;  // CAHitNtupletCUDA.cc:31
 
   template <class T, int N>
@@ -60,6 +59,20 @@ namespace orc {
     int push(T v) {
       if (size < N) {
         data[size] = v;
+        return size++;
+      }
+      return -1;
+    }
+  };
+
+  // VecArray with a run-time capacity (OuterHitOfCell, CAConstants.h:61, whose
+  // capacity maxCellsPerHit is a capacity of the profile here)
+  struct CappedList {
+    int size = 0;
+    std::vector<uint32_t> data;
+    int push(uint32_t v, uint32_t cap) {
+      if (uint32_t(size) < cap) {
+        data.push_back(v);
         return size++;
       }
       return -1;
@@ -80,18 +93,18 @@ namespace orc {
     uint32_t status = 0;
     uint32_t nCells = 0;
     std::vector<Cell> cells;
-    std::vector<Vec<uint32_t, kMaxCellsPerHit>> outerHitOfCell;
+    std::vector<CappedList> outerHitOfCell;
     std::vector<Vec<uint32_t, 36>> neighborStore;
-    std::vector<Vec<uint16_t, 48>> trackStore;
+    std::vector<Vec<TupleIndex, 48>> trackStore;
     uint32_t apcM = 0, apcN = 0, nTuples = 0;
     std::vector<uint32_t> tupleOff;   // [32769]
     std::vector<uint16_t> tupleBins;  // hit indices
     std::vector<uint8_t> quality;
     std::vector<uint32_t> multOff;  // [9]
-    std::vector<uint16_t> multBins;
+    std::vector<TupleIndex> multBins;
     std::vector<uint16_t> detIndices;
     std::vector<uint32_t> hitToTupleOff;
-    std::vector<uint16_t> hitToTupleBins;
+    std::vector<TupleIndex> hitToTupleBins;
     std::vector<std::array<float, 5>> state;
     std::vector<std::array<float, 15>> cov;
     std::vector<float> pt, eta, chi2;
@@ -194,7 +207,7 @@ namespace orc {
   // a23-a25 (orc_stageD.cc)
   void findVertices(Event& ev);
   uint32_t clusterTracks(int algo, uint32_t nt, const float* zt, const float* ezt2, int32_t* iv, int32_t* nn);
-  void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
+  void clusterAndFit(VertexState& V, const std::vector<TupleIndex>& itrk, const std::vector<float>& zt,
                      const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo = 0);
   int verticesDirect(uint32_t nt, const float* zt, const float* ezt2, const float* ptt2, int16_t* iv, float* zv,
                      float* wv, float* chi2, float* ptv2, int32_t* ndof, uint16_t* sortInd, int algo = 0);

@@ -51,7 +51,7 @@ namespace {
     std::vector<int16_t> lp;
     std::vector<uint16_t> used;
     std::vector<uint32_t> nbOff{0}, nbVal, trOff{0};
-    std::vector<uint16_t> trVal;
+    std::vector<uint32_t> trVal;
     for (const Cell& c : C.cells) {
       inner.push_back(c.innerHit);
       outer.push_back(c.outerHit);
@@ -75,7 +75,7 @@ namespace {
     exportVec(e, "cellNeighborOff", nbOff, 4);
     exportVec(e, "cellNeighbors", nbVal, 4);
     exportVec(e, "cellTrackOff", trOff, 4);
-    exportVec(e, "cellTracks", trVal, 2);
+    exportVec(e, "cellTracks", trVal, 4);
     std::vector<uint32_t> ohOff{0}, ohVal;
     for (uint32_t h = 0; h < ev.nHits && h < C.outerHitOfCell.size(); ++h) {
       const auto& v = C.outerHitOfCell[h];
@@ -93,11 +93,11 @@ namespace {
     std::vector<uint32_t> stats(C.stats.begin(), C.stats.end());
     exportVec(e, "stats", stats, 4);
     exportVec(e, "multOff", C.multOff, 4);
-    exportVec(e, "multBins", C.multBins, 2);
+    exportVec(e, "multBins", C.multBins, 4);
     std::vector<uint16_t> det(C.detIndices.begin(), C.detIndices.begin() + bins.size());
     exportVec(e, "detIndices", det, 2);
     exportVec(e, "hitToTupleOff", C.hitToTupleOff, 4);
-    exportVec(e, "hitToTupleBins", C.hitToTupleBins, 2);
+    exportVec(e, "hitToTupleBins", C.hitToTupleBins, 4);
     std::vector<float> st, cv;
     for (auto& a : C.state)
       st.insert(st.end(), a.begin(), a.end());

@@ -195,7 +195,7 @@ namespace orc {
           c.neighbors = 0;
           c.tracks = 0;
           C.cells.push_back(c);
-          if (C.outerHitOfCell[oi].push(ind) < 0)
+          if (C.outerHitOfCell[oi].push(ind, ev.caps.maxCellsPerHit) < 0)
             C.status |= kCaOuterHitOverflow;
         }
       }
@@ -255,7 +255,7 @@ namespace orc {
   // a14 fishbone: gpuFishbone.h:21-90 (checkTrack: the late pass, over the cells in tuples)
   void fishbone(Event& ev, bool checkTrack) {
     auto& C = ev.ca;
-    constexpr int kMax = kMaxCellsPerHit;
+    constexpr int kMax = kCaMaxCellsPerHitCap;
     float x[kMax], y[kMax], z[kMax], n[kMax];
     uint16_t d[kMax];
     uint32_t cc[kMax];
@@ -411,7 +411,7 @@ namespace orc {
         continue;
       uint32_t w = --C.multOff[nh];
       if (w < ev.caps.maxTuplesMult)
-        C.multBins[w] = uint16_t(t);
+        C.multBins[w] = t;
     }
     // The reference writes past the container's capacity when it overflows;
     // here the entries beyond it are dropped and the offsets clamped, so that
@@ -466,9 +466,9 @@ namespace orc {
       if (tr.size < 2)
         continue;
       float mc = 10000.f;
-      uint16_t im = 60000;
+      TupleIndex im = kNoTuple;
       for (int k = 0; k < tr.size; ++k) {
-        const uint16_t it = tr.data[k];
+        const TupleIndex it = tr.data[k];
         float score = std::abs(C.state[it][1]);
         if (C.quality[it] == kQualityLoose && score < mc) {
           mc = score;
@@ -476,7 +476,7 @@ namespace orc {
         }
       }
       for (int k = 0; k < tr.size; ++k) {
-        const uint16_t it = tr.data[k];
+        const TupleIndex it = tr.data[k];
         if (C.quality[it] != kQualityBad && it != im)
           C.quality[it] = kQualityDup;
       }
@@ -509,7 +509,7 @@ namespace orc {
       for (uint32_t k = C.tupleOff[t]; k < C.tupleOff[t + 1]; ++k) {
         uint32_t w = --C.hitToTupleOff[C.tupleBins[k]];
         if (w < ev.caps.maxHitToTuple)
-          C.hitToTupleBins[w] = uint16_t(t);
+          C.hitToTupleBins[w] = t;
       }
     }
     for (auto& o : C.hitToTupleOff)
@@ -527,16 +527,16 @@ namespace orc {
       if (maxNh > 3)
         continue;
       float mc = 10000.f;
-      uint16_t im = 60000;
+      TupleIndex im = kNoTuple;
       for (uint32_t k = b; k < e; ++k) {
-        const uint16_t it = C.hitToTupleBins[k];
+        const TupleIndex it = C.hitToTupleBins[k];
         if (C.quality[it] != kQualityBad && std::abs(C.state[it][1]) < mc) {
           mc = std::abs(C.state[it][1]);
           im = it;
         }
       }
       for (uint32_t k = b; k < e; ++k) {
-        const uint16_t it = C.hitToTupleBins[k];
+        const TupleIndex it = C.hitToTupleBins[k];
         if (C.quality[it] != kQualityBad && it != im)
           C.quality[it] = kQualityDup;
       }

@@ -22,7 +22,7 @@ namespace orc {
     constexpr float ptMin = 0.5f;
 
     // loadTracks (gpuVertexFinderImpl.h:10-43)
-    std::vector<uint16_t> itrk;
+    std::vector<TupleIndex> itrk;
     std::vector<float> zt, ezt2, ptt2;
     for (uint32_t idx = 0; idx < ev.caps.maxTuples; ++idx) {
       const uint32_t nHits = C.tupleOff[idx + 1] - C.tupleOff[idx];
@@ -36,7 +36,7 @@ namespace orc {
       const float pt = C.pt[idx];
       if (pt < ptMin)
         continue;
-      itrk.push_back(uint16_t(idx));
+      itrk.push_back(idx);
       zt.push_back(C.state[idx][4]);
       ezt2.push_back(C.cov[idx][14]);
       ptt2.push_back(pt * pt);
@@ -248,7 +248,7 @@ namespace orc {
   }
 
   // density clustering, fit, split, refit and sort over loaded tracks
-  void clusterAndFit(VertexState& V, const std::vector<uint16_t>& itrk, const std::vector<float>& zt,
+  void clusterAndFit(VertexState& V, const std::vector<TupleIndex>& itrk, const std::vector<float>& zt,
                      const std::vector<float>& ezt2, const std::vector<float>& ptt2, int algo) {
     V.zv.assign(kMaxVertices, 0);
     V.wv.assign(kMaxVertices, 0);
@@ -400,9 +400,9 @@ namespace orc {
     VertexState V;
     V.idv.assign(nt, -1);
     V.ndof.assign(std::max<uint32_t>(nt, kMaxVertices), 0);
-    std::vector<uint16_t> itrk(nt);
+    std::vector<TupleIndex> itrk(nt);
     for (uint32_t i = 0; i < nt; ++i)
-      itrk[i] = uint16_t(i);
+      itrk[i] = i;
     clusterAndFit(V, itrk, std::vector<float>(zt, zt + nt), std::vector<float>(ezt2, ezt2 + nt),
                   std::vector<float>(ptt2, ptt2 + nt), algo);
     for (uint32_t i = 0; i < nt; ++i)

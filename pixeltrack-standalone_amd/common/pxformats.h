@@ -54,15 +54,20 @@ namespace pxf {
   // Capacities of the ntuplet containers.  The reference profile is the
   // serial program's (CAConstants.h:21-64, CAHitNtupletGeneratorOnGPU.cc:48,
   // PixelTrackHeterogeneous.h:62); the high-occupancy profile scales them for
-  // PU200-like events.  Tuple indices are 16 bit and 60000 is the "none"
-  // marker of the duplicate remover, so maxTuples stays below it.
+  // PU200-like events (about 1.2M cells, 122k tuples and up to 160 cells
+  // sharing an outer hit with this geometry).  Tuple indices are stored as 32
+  // bit (the reference's tindex_type is 16 bit, CAConstants.h:46, which caps
+  // it at 65535 tuples; the values are the same below that).
   struct CACapacities {
-    uint32_t maxDoublets;    // cells
-    uint32_t maxTuples;      // tuple container bins (TrackSoA stride)
-    uint32_t maxTuplesMult;  // tuple multiplicity entries
-    uint32_t maxHitToTuple;  // hit -> tuple entries
-    uint32_t maxCellLists;   // cell neighbour lists / cell track lists (each store)
+    uint32_t maxDoublets;      // cells
+    uint32_t maxTuples;        // tuple container bins (TrackSoA stride)
+    uint32_t maxTuplesMult;    // tuple multiplicity entries
+    uint32_t maxHitToTuple;    // hit -> tuple entries
+    uint32_t maxCellLists;     // cell neighbour lists / cell track lists (each store)
+    uint32_t maxCellsPerHit;   // cells kept per outer hit (CAConstants.h maxCellsPerHit)
   };
+  using TupleIndex = uint32_t;
+  constexpr TupleIndex kNoTuple = 0xffffffffu;  // "none" of the duplicate removers
   // Per-event status bits of the ntuplet stage (a set bit names a container
   // that reached its capacity; what was stored is still well defined).
   enum : uint32_t {
@@ -79,12 +84,12 @@ namespace pxf {
   };
   constexpr uint32_t kCaMaxNeighbors = 36;      // CAConstants.h CellNeighbors
   constexpr uint32_t kCaMaxCellTracks = 48;     // CAConstants.h CellTracks
-  constexpr uint32_t kCaMaxCellsPerHit = 128;   // CAConstants.h maxCellsPerHit
+  constexpr uint32_t kCaMaxCellsPerHitCap = 1024;  // largest maxCellsPerHit of any profile
   constexpr uint32_t kMaxVertices = 1024;       // ZVertexSoA.h MAXVTX
   enum : uint8_t { kQualityBad = 0, kQualityDup = 1, kQualityLoose = 2 };
 
-  constexpr CACapacities kCapsReference{458752, 32 * 1024, 24 * 1024, 4 * 24 * 1024, 64 * 1024};
-  constexpr CACapacities kCapsHighOccupancy{4 * 458752, 56 * 1024, 56 * 1024, 4 * 56 * 1024, 256 * 1024};
+  constexpr CACapacities kCapsReference{458752, 32 * 1024, 24 * 1024, 4 * 24 * 1024, 64 * 1024, 128};
+  constexpr CACapacities kCapsHighOccupancy{4 * 458752, 192 * 1024, 192 * 1024, 4 * 192 * 1024, 256 * 1024, 256};
 
   // ---- conditions ------------------------------------------------------------
   struct alignas(128) SiPixelFedCablingMapGPU {

@@ -354,7 +354,7 @@ namespace pxg {
       }
     }
 
-    // ---- outer-hit cell lists (GPUCACell::OuterHitOfCell, first 128 cells ascending) ----
+    // ---- outer-hit cell lists (GPUCACell::OuterHitOfCell, first maxCellsPerHit cells ascending) ----
     __global__ void __launch_bounds__(kEventBlock) ohScanKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.x;
       __shared__ uint32_t lds[kEventBlock / 64];
@@ -411,21 +411,22 @@ namespace pxg {
     }
 
     // rank of cell c in the outer-hit list of h (ascending cell order); only
-    // needed when the list is longer than the 128 cells the serial container keeps
-    __device__ __forceinline__ bool listedInOuterHit(const uint2* cells, uint32_t b, uint32_t n, uint32_t c) {
-      if (n <= kCaMaxCellsPerHit)
+    // needed when the list is longer than the `cap` cells the serial container keeps
+    __device__ __forceinline__ bool listedInOuterHit(const uint2* cells, uint32_t b, uint32_t n, uint32_t c,
+                                                     uint32_t cap) {
+      if (n <= cap)
         return true;
       uint32_t rank = 0;
       for (uint32_t k = 0; k < n; ++k)
         rank += cells[b + k].x < c;
-      return rank < kCaMaxCellsPerHit;
+      return rank < cap;
     }
 
     // ---- a13 kernel_connect: CAHitNtupletGeneratorKernelsImpl.h:191-262 ----------------
     // Gathered per outer cell: the neighbours of cell `oc` are the cells `me`
     // whose inner hit is oc's outer hit and which pass the cuts, ascending in
     // `me` (the serial program's append order); `oc` must be among the first
-    // 128 cells of its outer hit's list (GPUCACell::OuterHitOfCell).
+    // maxCellsPerHit cells of its outer hit's list (GPUCACell::OuterHitOfCell).
     //
     // Thread per cell in outer-hit list order, so the lanes of a wave form a few
     // groups sharing an outer hit and hence a candidate list.  The wave first
@@ -493,9 +494,9 @@ namespace pxg {
           hi = rec.y & 0xffff;
           hm = rec.y >> 16;
           const uint32_t lb = off[hm], ln = off[hm + 1] - lb;
-          if (ln > kCaMaxCellsPerHit)
+          if (ln > ca.caps.maxCellsPerHit)
             status |= kCaOuterHitOverflow;
-          live = listedInOuterHit(ca.ohCells + cb, lb, ln, oc);
+          live = listedInOuterHit(ca.ohCells + cb, lb, ln, oc, ca.caps.maxCellsPerHit);
           p1 = H.pos[hi];
           p2 = H.pos[hm];
           thetaCut = H.detInd[hm] < 1184 ? kCaThetaCutBarrel : kCaThetaCutForward;
@@ -635,7 +636,7 @@ namespace pxg {
     }
 
     // ---- a14 fishbone: gpuFishbone.h:21-90 --------------------------------------------
-    // The serial loop visits the used cells of each outer-hit list (first 128,
+    // The serial loop visits the used cells of each outer-hit list (first maxCellsPerHit,
     // ascending) in rows: row ci kills the later aligned cells until the first
     // aligned later cell that is closer, which kills ci and ends the row.  The
     // rows do not read the kills, so each row runs on its own (one thread per
@@ -716,17 +717,18 @@ namespace pxg {
                                     : entry(pos);
           const uint32_t ci = me.c;
           if (me.det >> 16) {
-            // with more than 128 cells only the 128 lowest indices take part
+            // with more than maxCellsPerHit cells only that many lowest indices take part
+            const uint32_t cap = ca.caps.maxCellsPerHit;
             uint32_t limit = 0xffffffffu;
             bool listed = true;
-            if (n > kCaMaxCellsPerHit) {
-              listed = listedInOuterHit(cells, b, n, ci);
+            if (n > cap) {
+              listed = listedInOuterHit(cells, b, n, ci, cap);
               for (uint32_t k = 0; listed && k < n; ++k) {
                 const uint32_t c = cells[b + k].x;
                 uint32_t rank = 0;
                 for (uint32_t m = 0; m < n; ++m)
                   rank += cells[b + m].x < c;
-                if (rank == kCaMaxCellsPerHit)
+                if (rank == cap)
                   limit = c;
               }
             }
@@ -1044,7 +1046,7 @@ namespace pxg {
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
-      uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
       for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
         for (int k = 0; k < 4; ++k) {
           const uint32_t c = ca.tupleCells[4 * tb + 4 * size_t(t) + k];
@@ -1055,7 +1057,7 @@ namespace pxg {
             continue;  // no list: the store is full
           const uint32_t pos = atomicIncHalf(ca.trCount + cb, c);
           if (pos < kCaMaxCellTracks)
-            pool[size_t(slot) * kCaMaxCellTracks + pos] = uint16_t(t);
+            pool[size_t(slot) * kCaMaxCellTracks + pos] = t;
         }
       }
     }
@@ -1068,7 +1070,7 @@ namespace pxg {
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
-      uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
       const uint32_t* trCells = ca.dfsT + cb;
       const uint32_t* trTotal = ca.dfsH + cb;
@@ -1076,7 +1078,7 @@ namespace pxg {
       uint32_t status = 0;
       for (uint32_t slot = gridStart(); slot < nLists; slot += gridStride()) {
         const uint32_t c = trCells[slot], total = trTotal[slot];
-        uint16_t* list = pool + size_t(slot) * kCaMaxCellTracks;
+        TupleIndex* list = pool + size_t(slot) * kCaMaxCellTracks;
         uint32_t n = total;
         if (total > kCaMaxCellTracks) {
           // keep the 48 smallest tuple indices: rebuild from the tuples
@@ -1088,13 +1090,13 @@ namespace pxg {
               if (x == 0xffffffffu)
                 break;
               if (x == c) {
-                list[n++] = uint16_t(t);
+                list[n++] = t;
                 break;
               }
             }
         } else {
           for (uint32_t k = 1; k < n; ++k) {
-            const uint16_t v = list[k];
+            const TupleIndex v = list[k];
             uint32_t m = k;
             while (m > 0 && list[m - 1] > v) {
               list[m] = list[m - 1];
@@ -1163,7 +1165,7 @@ namespace pxg {
           atomicOr(&ca.cnt[e].status, kCaMultiplicityOverflow);
       }
       __syncthreads();
-      uint16_t* mb = ca.multBins + size_t(e) * cap;
+      TupleIndex* mb = ca.multBins + size_t(e) * cap;
       for (uint32_t base = 0; base < nT; base += kEventBlock) {
         const uint32_t t = base + threadIdx.x;
         const int b = t < nT ? binOf(t) : -1;
@@ -1182,7 +1184,7 @@ namespace pxg {
             rank += waveCnt[b][w];
           const uint32_t pos = binEnd[b] - 1 - rank;
           if (pos < cap)
-            mb[pos] = uint16_t(t);
+            mb[pos] = t;
         }
         __syncthreads();
         if (threadIdx.x < 8 && threadIdx.x >= 3) {

@@ -58,14 +58,14 @@ namespace pxg {
       const uint32_t e = logicalBlock().y;
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
-      const uint16_t* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      const TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
       const uint32_t* trCells = ca.dfsT + cb;
       const uint32_t nLists = min(ca.cnt[e].trLists, ca.caps.maxCellLists - 1);
       for (uint32_t slot = gridStart(); slot < nLists; slot += gridStride()) {
         const uint32_t c = trCells[slot];
         if (!(ca.cellFlags[cb + c] & 2))
           continue;
-        const uint16_t* list = pool + size_t(slot) * kCaMaxCellTracks;
+        const TupleIndex* list = pool + size_t(slot) * kCaMaxCellTracks;
         const uint32_t n = ca.trCount[cb + c];
         for (uint32_t k = 0; k < n; ++k)
           ca.quality[tb + list[k]] = kQualityBad;
@@ -81,9 +81,9 @@ namespace pxg {
 
     // the predecessors of cell c: lower cells with two or more tuples in its tuples
     template <typename F>
-    __device__ __forceinline__ void forPreds(const CAPool& ca, size_t cb, size_t tb, const uint16_t* pool, uint32_t c,
+    __device__ __forceinline__ void forPreds(const CAPool& ca, size_t cb, size_t tb, const TupleIndex* pool, uint32_t c,
                                              F&& f) {
-      const uint16_t* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
+      const TupleIndex* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
       const uint32_t n = ca.trCount[cb + c];
       for (uint32_t a = 0; a < n; ++a) {
         const uint32_t* cells = ca.tupleCells + 4 * tb + 4 * size_t(list[a]);
@@ -98,14 +98,14 @@ namespace pxg {
     }
 
     // keep the loose tuple of cell c with the smallest |tip|, the others are duplicates
-    __device__ __forceinline__ void dupProcess(const CAPool& ca, size_t cb, const uint16_t* pool, uint8_t* q,
+    __device__ __forceinline__ void dupProcess(const CAPool& ca, size_t cb, const TupleIndex* pool, uint8_t* q,
                                                const float* tip, uint32_t c) {
-      const uint16_t* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
+      const TupleIndex* list = pool + size_t(ca.trSlot[cb + c]) * kCaMaxCellTracks;
       const uint32_t n = ca.trCount[cb + c];
       float mc = 10000.f;
-      uint16_t im = 60000;
+      TupleIndex im = kNoTuple;
       for (uint32_t a = 0; a < n; ++a) {
-        const uint16_t it = list[a];
+        const TupleIndex it = list[a];
         const float score = fabsf(tip[it]);
         if (q[it] == kQualityLoose && score < mc) {
           mc = score;
@@ -113,7 +113,7 @@ namespace pxg {
         }
       }
       for (uint32_t a = 0; a < n; ++a) {
-        const uint16_t it = list[a];
+        const TupleIndex it = list[a];
         if (q[it] != kQualityBad && it != im)
           q[it] = kQualityDup;
       }
@@ -128,7 +128,7 @@ namespace pxg {
       const size_t tb = ca.tupleBase(e);
       const size_t lb = size_t(e) * ca.caps.maxCellLists;
       const uint32_t* multi = ca.multiCells + cb;
-      const uint16_t* pool = ca.trPool + lb * kCaMaxCellTracks;
+      const TupleIndex* pool = ca.trPool + lb * kCaMaxCellTracks;
       uint32_t* preds = ca.dupPreds + lb * kDupPredCap;
       for (uint32_t k = gridStart(); k < nMulti; k += gridStride()) {
         uint32_t m = 0;
@@ -173,7 +173,7 @@ namespace pxg {
       const size_t lb = size_t(e) * ca.caps.maxCellLists;
       const uint32_t M = ca.caps.maxTuples;
       const uint32_t* multi = ca.multiCells + cb;
-      const uint16_t* pool = ca.trPool + lb * kCaMaxCellTracks;
+      const TupleIndex* pool = ca.trPool + lb * kCaMaxCellTracks;
       const uint32_t* preds = ca.dupPreds + lb * kDupPredCap;
       const uint8_t* count = ca.dupCount + lb;
       uint8_t* q = ca.quality + tb;
@@ -370,7 +370,7 @@ namespace pxg {
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
       uint32_t* cur = ca.h2tCount + size_t(e) * kHitStride;
       const uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
-      uint16_t* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
+      TupleIndex* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
       for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
         if (ca.quality[tb + t] != kQualityLoose)
           continue;
@@ -378,7 +378,7 @@ namespace pxg {
           const uint32_t h = ca.tupleBins[5 * tb + k];
           const uint32_t pos = hoff[h] + atomicAdd(&cur[h], 1u);
           if (pos < hoff[h + 1])
-            bins[pos] = uint16_t(t);
+            bins[pos] = t;
         }
       }
     }
@@ -392,13 +392,13 @@ namespace pxg {
       const uint32_t M = ca.caps.maxTuples;
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
       const uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
-      uint16_t* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
+      TupleIndex* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
       uint8_t* q = ca.quality + tb;
       const float* tip = ca.state + 5 * tb + 1 * size_t(M);
       for (uint32_t h = gridStart(); h < nHits; h += gridStride()) {
         const uint32_t b = hoff[h], n = hoff[h + 1] - b;
         for (uint32_t k = 1; k < n; ++k) {
-          const uint16_t v = bins[b + k];
+          const TupleIndex v = bins[b + k];
           uint32_t m = k;
           while (m > 0 && bins[b + m - 1] < v) {
             bins[b + m] = bins[b + m - 1];
@@ -410,27 +410,27 @@ namespace pxg {
           continue;
         uint32_t maxNh = 0;
         for (uint32_t k = 0; k < n; ++k) {
-          const uint16_t it = bins[b + k];
+          const TupleIndex it = bins[b + k];
           maxNh = max(maxNh, off[it + 1] - off[it]);
         }
         for (uint32_t k = 0; k < n; ++k) {
-          const uint16_t it = bins[b + k];
+          const TupleIndex it = bins[b + k];
           if (off[it + 1] - off[it] != maxNh)
             q[it] = kQualityDup;
         }
         if (maxNh > 3)
           continue;
         float mc = 10000.f;
-        uint16_t im = 60000;
+        TupleIndex im = kNoTuple;
         for (uint32_t k = 0; k < n; ++k) {
-          const uint16_t it = bins[b + k];
+          const TupleIndex it = bins[b + k];
           if (q[it] != kQualityBad && fabsf(tip[it]) < mc) {
             mc = fabsf(tip[it]);
             im = it;
           }
         }
         for (uint32_t k = 0; k < n; ++k) {
-          const uint16_t it = bins[b + k];
+          const TupleIndex it = bins[b + k];
           if (q[it] != kQualityBad && it != im)
             q[it] = kQualityDup;
         }

@@ -499,7 +499,7 @@ namespace pxg {
     __global__ void __launch_bounds__(256) fitKernel(DevConditions cond, Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const uint32_t* mo = ca.multOff + size_t(e) * 16;
-      const uint16_t* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
+      const TupleIndex* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
       const uint32_t b3 = mo[3], b4 = mo[4], end = mo[6];  // sizes 3, then 4 and 5 (fitted on 4 hits)
       const double B = double(float(0.0114256972711507));  // CAHitNtupletCUDA.cc:31 (float field)
       for (uint32_t k = b3 + logicalBlock().x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x) {

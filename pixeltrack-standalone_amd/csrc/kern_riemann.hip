@@ -1186,7 +1186,7 @@ namespace pxg {
     __global__ void __launch_bounds__(256) riemannKernel(DevConditions cond, Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const uint32_t* mo = ca.multOff + size_t(e) * 16;
-      const uint16_t* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
+      const TupleIndex* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
       // sizes 3 (N = 3), then 4 and 5 on their first 4 hits (N = 4)
       const uint32_t begin = N == 3 ? mo[3] : mo[4], end = N == 3 ? mo[4] : mo[6];
       const double B = double(float(0.0114256972711507));  // CAHitNtupletCUDA.cc:31 (float field)

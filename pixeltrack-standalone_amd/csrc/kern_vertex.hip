@@ -17,8 +17,6 @@ namespace pxg {
     constexpr int kVB = kEventBlock;
     constexpr int kUnassigned = 9999;
     constexpr uint32_t kLdsTracks = 2048;
-    // loadTracks gives each thread at most 64 tuples
-    static_assert(kCapsReference.maxTuples <= 64 * kVB && kCapsHighOccupancy.maxTuples <= 64 * kVB);
 
     struct VtxLds {
       uint32_t counts[kMaxVertices];
@@ -107,7 +105,7 @@ namespace pxg {
       int16_t* idv = ca.idv + tb;
       int32_t* const nnOut = ca.ndof + tb;
       int32_t* nn = nnOut;
-      uint16_t* itrk = ca.vItrk + tb;
+      TupleIndex* itrk = ca.vItrk + tb;
       float* zt = ca.vZt + tb;
       float* ezt2 = ca.vEzt2 + tb;
       float* ptt2 = ca.vPtt2 + tb;
@@ -123,33 +121,38 @@ namespace pxg {
       constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f, ptMin = 0.5f;
 
       // ---- loadTracks: loose tuples with 4+ hits and pt >= 0.5 GeV, in tuple order.
-      // Each thread takes a contiguous run of tuples (at most 64: maxTuples <= 65536),
-      // one scan over the threads orders the taken ones.
+      // The tuples go in chunks of 64 * kVB; in a chunk each thread takes a contiguous
+      // run of at most 64 tuples and one scan over the threads orders the taken ones.
       {
-        const uint32_t R = (nT + kVB - 1) / kVB;
-        const uint32_t t0 = min(threadIdx.x * R, nT), t1 = min(t0 + R, nT);
-        uint64_t takeMask = 0;
-        uint32_t cnt = 0;
-        for (uint32_t t = t0; t < t1; ++t) {
-          const uint32_t nh = toff[t + 1] - toff[t];
-          idv[t] = nh == 0 ? 0 : -1;  // only the last stored tuple can be empty
-          const bool take = nh >= 4 && q[t] == kQualityLoose && !(ca.pt[tb + t] < ptMin);
-          takeMask |= uint64_t(take) << (t - t0);
-          cnt += take;
-        }
-        uint32_t tot;
-        uint32_t k = wgExclusiveScan<kVB>(cnt, tot, L.scan);
-        for (uint64_t m = takeMask; m; m &= m - 1) {
-          const uint32_t t = t0 + __ffsll((unsigned long long)m) - 1;
-          const float pt = ca.pt[tb + t];
-          itrk[k] = uint16_t(t);
-          zt[k] = ca.state[5 * tb + 4 * size_t(M) + t];
-          ezt2[k] = ca.cov[15 * tb + 14 * size_t(M) + t];
-          ptt2[k] = pt * pt;
-          ++k;
+        uint32_t taken = 0;  // tracks of the earlier chunks
+        for (uint32_t c0 = 0; c0 < nT; c0 += 64 * kVB) {
+          const uint32_t n = min(nT - c0, 64u * kVB);
+          const uint32_t R = (n + kVB - 1) / kVB;
+          const uint32_t t0 = c0 + min(threadIdx.x * R, n), t1 = min(t0 + R, c0 + n);
+          uint64_t takeMask = 0;
+          uint32_t cnt = 0;
+          for (uint32_t t = t0; t < t1; ++t) {
+            const uint32_t nh = toff[t + 1] - toff[t];
+            idv[t] = nh == 0 ? 0 : -1;  // only the last stored tuple can be empty
+            const bool take = nh >= 4 && q[t] == kQualityLoose && !(ca.pt[tb + t] < ptMin);
+            takeMask |= uint64_t(take) << (t - t0);
+            cnt += take;
+          }
+          uint32_t tot;
+          uint32_t k = taken + wgExclusiveScan<kVB>(cnt, tot, L.scan);
+          for (uint64_t m = takeMask; m; m &= m - 1) {
+            const uint32_t t = t0 + __ffsll((unsigned long long)m) - 1;
+            const float pt = ca.pt[tb + t];
+            itrk[k] = t;
+            zt[k] = ca.state[5 * tb + 4 * size_t(M) + t];
+            ezt2[k] = ca.cov[15 * tb + 14 * size_t(M) + t];
+            ptt2[k] = pt * pt;
+            ++k;
+          }
+          taken += tot;
         }
         if (threadIdx.x == 0)
-          L.nt = tot;
+          L.nt = taken;
         __syncthreads();
       }
       const uint32_t nt = L.nt;

@@ -38,13 +38,6 @@ using pxg::detail::fail;
 
 namespace {
 
-#define PXG_HIP(call)                                                                          \
-  do {                                                                                         \
-    hipError_t err_ = (call);                                                                  \
-    if (err_ != hipSuccess)                                                                    \
-      return fail(PXG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(err_), __FILE__, __LINE__); \
-  } while (0)
-
   template <typename T>
   hipError_t devAlloc(T*& p, size_t n, std::vector<void*>& owned) {
     void* q = nullptr;
@@ -832,6 +825,14 @@ int pxg_find_vertices(pxg_stream* s, const pxg_tracks* t, pxg_vertices* v) {
 
 size_t pxg_pool_bytes(uint32_t maxEvents, uint32_t maxWords, int profile) {
   return poolBytes(maxEvents, maxWords, profile == PXG_PROFILE_HIGH_OCCUPANCY ? kCapsHighOccupancy : kCapsReference);
+}
+
+int pxg_profile_capacities(int profile, pxg_capacities* out) {
+  if (!out || (profile != PXG_PROFILE_REFERENCE && profile != PXG_PROFILE_HIGH_OCCUPANCY))
+    return fail(PXG_ERR_ARG, "pxg_profile_capacities: unknown profile or null output");
+  const CACapacities& c = profile == PXG_PROFILE_HIGH_OCCUPANCY ? kCapsHighOccupancy : kCapsReference;
+  *out = {c.maxDoublets, c.maxTuples, c.maxTuplesMult, c.maxHitToTuple, c.maxCellLists, c.maxCellsPerHit};
+  return 0;
 }
 
 // ---- tuning ---------------------------------------------------------------------------

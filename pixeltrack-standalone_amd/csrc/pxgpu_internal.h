@@ -184,8 +184,8 @@ namespace pxg {
     uint64_t* startMask;     // [maxDoublets / 64 + 64] start cells of the ntuplet walk
     uint32_t *dfsT, *dfsH;   // [maxDoublets] tuples / hits per start cell, then offsets
     int32_t* trSlot;         // [maxDoublets] track list slot (-1 none, -2 store full)
-    uint16_t* trCount;       // [maxDoublets] tracks of the cell (< maxTuples < 65536), then stored (<= 48)
-    uint16_t* trPool;        // [maxCellLists * 48]
+    uint16_t* trCount;       // [maxDoublets] tracks of the cell (about 20 at PU200), then stored (<= 48)
+    TupleIndex* trPool;      // [maxCellLists * 48]
     uint32_t* multiCells;    // [maxDoublets] cells with two or more tracks (any order)
     uint32_t* dupPreds;      // [maxCellLists * 8] per multiCells entry: its lower cells sharing a tuple (first 8)
     uint8_t* dupCount;       // [maxCellLists] their number, 255 if more than 8
@@ -194,26 +194,25 @@ namespace pxg {
     uint32_t* tupleCells;    // [4 * maxTuples] cells of a tuple (0xffffffff past the end)
     uint8_t* quality;        // [maxTuples]
     uint32_t* multOff;       // [16]
-    uint16_t* multBins;      // [maxTuplesMult]
+    TupleIndex* multBins;    // [maxTuplesMult]
     uint16_t* detIndices;    // [5 * maxTuples]
     float* state;            // [5][maxTuples]: phi, tip, q/pt, cot(theta), zip
     float* cov;              // [15][maxTuples]: packed upper triangle, row by row
     float *pt, *eta, *chi2;  // [maxTuples]
     uint32_t* h2tCount;      // [kHitStride]
     uint32_t* h2tOff;        // [kHitStride]
-    uint16_t* h2tBins;       // [maxHitToTuple]
+    TupleIndex* h2tBins;     // [maxHitToTuple]
     // vertices
     int16_t* idv;            // [maxTuples]
     int32_t* ndof;           // [maxTuples]: per-track neighbour counts, then per-vertex ndof
     float *zv, *wv, *vchi2, *ptv2;  // [kMaxVertices]
     uint16_t* sortInd;       // [kMaxVertices]
-    uint16_t* vItrk;         // [maxTuples] tuple index of each loaded track
+    TupleIndex* vItrk;       // [maxTuples] tuple index of each loaded track
     float *vZt, *vEzt2, *vPtt2;      // [maxTuples]
     int32_t *vIv, *vTmp;     // [maxTuples]
     float *vZs, *vEs;        // [maxTuples]: z and its error of the tracks in z-bin order
     int32_t* vNnS;           // [maxTuples]: neighbour counts in z-bin order
     uint8_t* vIzt;           // [maxTuples]
-    uint16_t* vBins;         // [maxTuples]: tracks by z bin
     uint32_t* vTrk;          // [maxTuples]: tracks by vertex, ascending within a vertex
     uint32_t* vOff;          // [kMaxVertices + 64]
 

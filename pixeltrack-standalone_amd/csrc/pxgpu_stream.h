@@ -76,3 +76,12 @@ namespace pxg::detail {
   // destroys the stream's captured stage sequences
   void dropGraphs(pxg_stream* s);
 }  // namespace pxg::detail
+
+// returns PXG_ERR_HIP (with the message for pxg_last_error) from the enclosing function
+#define PXG_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t err_ = (call);                                                                \
+    if (err_ != hipSuccess)                                                                  \
+      return pxg::detail::fail(PXG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(err_), __FILE__, \
+                               __LINE__);                                                    \
+  } while (0)

@@ -103,6 +103,7 @@ def _gpu():
         lib.pxg_event_ca_internals.argtypes = [vp, u32, C.POINTER(CAInternals)]
         lib.pxg_pool_bytes.restype = C.c_size_t
         lib.pxg_pool_bytes.argtypes = [u32, u32, C.c_int]
+        lib.pxg_profile_capacities.argtypes = [C.c_int, C.POINTER(u32 * 6)]
         lib.pxg_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
         lib.pxg_server_destroy.argtypes = [vp]
         lib.pxg_server_submit.argtypes = [vp, vp, vp, u32, C.POINTER(BeamSpot), C.POINTER(C.c_uint64)]
@@ -218,6 +219,7 @@ class Stream:
         self._h = vp()
         _check(self.lib.pxg_stream_create_profile(ctx._h, max_events, max_words, profile, C.byref(self._h)),
                "pxg_stream_create_profile")
+        self.caps = capacities(profile)
         self._inputs = []
 
     def close(self):
@@ -302,8 +304,9 @@ class Stream:
         ohoff = self._copy(ci.outerHitOff, np.uint32, n_hits + 1)
         ohc = self._copy(ci.outerHitCells, np.uint32, 2 * int(ohoff[-1]) if n_hits else 0)[0::2]
         # the device keeps each hit's cells in fill order; the serial container holds the
-        # first 128 in ascending order
-        lists = [np.sort(ohc[ohoff[h]:ohoff[h + 1]])[:128] for h in range(n_hits)]
+        # first maxCellsPerHit in ascending order
+        cap = self.caps["max_cells_per_hit"]
+        lists = [np.sort(ohc[ohoff[h]:ohoff[h + 1]])[:cap] for h in range(n_hits)]
         out["outerHitOff"] = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.uint32)
         out["outerHitCells"] = np.concatenate(lists).astype(np.uint32) if lists else np.zeros(0, np.uint32)
 
@@ -319,12 +322,12 @@ class Stream:
 
         out["cellNeighborOff"], out["cellNeighbors"] = csr(ci.neighborSlot, ci.neighborCount, ci.neighborPool, 36,
                                                            np.uint32, np.uint8)
-        out["cellTrackOff"], out["cellTracks"] = csr(ci.trackSlot, ci.trackCount, ci.trackPool, 48, np.uint16,
+        out["cellTrackOff"], out["cellTracks"] = csr(ci.trackSlot, ci.trackCount, ci.trackPool, 48, np.uint32,
                                                      np.uint16)
         out["multOff"] = self._copy(ci.multOff, np.uint32, 9)
-        out["multBins"] = self._copy(ci.multBins, np.uint16, int(out["multOff"][8]))
+        out["multBins"] = self._copy(ci.multBins, np.uint32, int(out["multOff"][8]))
         out["hitToTupleOff"] = self._copy(ci.hitToTupleOff, np.uint32, n_hits + 1)
-        out["hitToTupleBins"] = self._copy(ci.hitToTupleBins, np.uint16, int(out["hitToTupleOff"][-1]) if n_hits else 0)
+        out["hitToTupleBins"] = self._copy(ci.hitToTupleBins, np.uint32, int(out["hitToTupleOff"][-1]) if n_hits else 0)
         out.update(cnt)
         return out
 
@@ -570,6 +573,17 @@ class Server:
         w, b, e = u32(), C.c_uint64(), C.c_uint64()
         _check(self.lib.pxg_server_info(self._h, C.byref(w), C.byref(b), C.byref(e)), "pxg_server_info")
         return {"max_batch_words": w.value, "batches": b.value, "events": e.value}
+
+
+CAPACITY_FIELDS = ("max_doublets", "max_tuples", "max_tuples_mult", "max_hit_to_tuple", "max_cell_lists",
+                   "max_cells_per_hit")
+
+
+def capacities(profile=PROFILE_REFERENCE):
+    """The ntuplet container capacities of a profile (pxg_profile_capacities)."""
+    out = (u32 * 6)()
+    _check(_gpu().pxg_profile_capacities(profile, C.byref(out)), "pxg_profile_capacities")
+    return dict(zip(CAPACITY_FIELDS, list(out)))
 
 
 def pool_bytes(max_events, max_words, profile=PROFILE_REFERENCE):

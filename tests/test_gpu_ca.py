@@ -141,14 +141,21 @@ def test_batched_matches_plain(detector, oracle, gpu_ctx):
 
 
 def test_high_occupancy_profile(detector, conditions, gpu_ctx):
+    """PU200 events under the high-occupancy capacities: nothing overflows (status 0),
+    more than 65535 tuples (indices past the reference's 16-bit tindex_type) and outer
+    hits with more than the reference's 128 cells."""
     import pyoracle
     from pxtrack import gpu
     orc = pyoracle.Oracle(conditions, profile=1)
     s = gpu_ctx.stream(max_events=1, max_words=300000, profile=gpu.PROFILE_HIGH_OCCUPANCY)
-    raw, _ = event(detector, "pu200", 1)
-    o = orc.process(raw, upto=5)
-    print(f"pu200 hits={o['nHits']} cells={o['nCells']} tuples={o['nTuples']} status={o['caStatus']:#x}", flush=True)
-    compare_ca(o, _gpu_event(gpu_ctx, s, raw), "pu200/high-occupancy")
+    for idx in (1, 4):
+        raw, _ = event(detector, "pu200", idx)
+        o = orc.process(raw, upto=5)
+        print(f"pu200[{idx}] hits={o['nHits']} cells={o['nCells']} tuples={o['nTuples']} "
+              f"status={o['caStatus']:#x}", flush=True)
+        assert o["caStatus"] == 0 and o["nTuples"] > 65535
+        assert np.diff(o["outerHitOff"]).max() > 128
+        compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"pu200[{idx}]/high-occupancy")
 
 
 def test_reference_capacities_overflow(detector, oracle, gpu_ctx):

@@ -243,6 +243,68 @@ def test_cell_track_lists_and_containers(small_events):
             assert list(hb[hoff[h]:hoff[h + 1]]) == per_hit.get(h, [])[::-1], (name, h)
 
 
+def test_high_occupancy_profile_holds_pu200(detector, conditions):
+    """PU200 under the high-occupancy capacities: no container overflows, tuple
+    indices go past 65535 (the reference's 16-bit tindex_type, CAConstants.h:46),
+    and the containers indexed by tuple hold them whole.  Checked with array
+    operations against the tuples: the outer-hit lists are each hit's cells in
+    ascending order (more than the reference's 128 for some hits, at most 256);
+    every cell's track list is the ascending list of the tuples through it (first
+    48); the multiplicity bins hold tuples of their size in descending order; the
+    hit -> tuple lists hold tuples through that hit in descending order."""
+    import pyoracle
+    orc = pyoracle.Oracle(conditions, profile=1)
+    o = orc.process(event(detector, "pu200", 1)[0], upto=5)
+    nt, nc, nh = o["nTuples"], o["nCells"], o["nHits"]
+    assert o["caStatus"] == 0 and nt > 65535 and nc > 4 * 458752 // 2
+    inner, outer = o["cellInner"].astype(np.int64), o["cellOuter"].astype(np.int64)
+    # outer-hit lists: stable sort of the cells by outer hit
+    want = np.argsort(outer, kind="stable")
+    lens = np.bincount(outer, minlength=nh)
+    assert lens.max() > 128 and lens.max() <= 256
+    assert np.array_equal(np.diff(o["outerHitOff"]), lens[:nh])
+    assert np.array_equal(o["outerHitCells"], want)
+    # cells of every tuple from consecutive hit pairs
+    off = o["tupleOff"][:nt + 1].astype(np.int64)
+    bins = o["tupleBins"].astype(np.int64)
+    size = np.diff(off)
+    key = inner * 65536 + outer
+    order = np.argsort(key)
+    t_of = np.repeat(np.arange(nt), size - 1)
+    first = np.repeat(off[:-1], size - 1)
+    k = np.arange(t_of.size) - np.repeat(np.cumsum(size - 1) - (size - 1), size - 1)
+    a, b2 = bins[first + k], bins[first + k + 1]
+    pos = np.searchsorted(key[order], a * 65536 + b2)
+    cell = order[pos]
+    assert np.array_equal(key[cell], a * 65536 + b2)
+    srt = np.lexsort((t_of, cell))
+    cell_s, t_s = cell[srt], t_of[srt]
+    cnt = np.bincount(cell_s, minlength=nc)
+    rank = np.arange(cell_s.size) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    keep = rank < 48
+    assert np.array_equal(np.diff(o["cellTrackOff"]), np.minimum(cnt, 48))
+    assert np.array_equal(o["cellTracks"], t_s[keep])
+    assert o["cellTracks"].max() > 65535
+    # multiplicity: bins by size, descending tuple index within a size
+    moff, mb = o["multOff"], o["multBins"]
+    for n in range(3, 8):
+        got = mb[moff[n]:moff[n + 1]].astype(np.int64)
+        assert (size[got] == n).all() and (np.diff(got) < 0).all(), n
+    assert mb[:moff[8]].max() > 65535
+    # hit -> tuple: each listed tuple goes through the hit, descending within a hit
+    hoff, hb = o["hitToTupleOff"].astype(np.int64), o["hitToTupleBins"].astype(np.int64)
+    hit = np.repeat(np.arange(nh), np.diff(hoff[:nh + 1]))
+    ent = hb[:hoff[nh]]
+    through = np.zeros(ent.size, bool)
+    for j in range(5):
+        sel = size[ent] > j
+        through[sel] |= bins[off[ent[sel]] + j] == hit[sel]
+    assert through.all()
+    same = hit[1:] == hit[:-1]
+    assert (ent[1:][same] < ent[:-1][same]).all()
+    assert ent.max() > 65535
+
+
 def q_before_fit_ok(o, t):
     """A tuple is in the multiplicity container unless the early duplicate remover marked it."""
     return t not in early_dups(o)
