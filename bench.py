@@ -89,6 +89,17 @@ def row_bytes(row, c):
     return ROWS[row][1](c)
 
 
+def kernel_symbol(name):
+    """Short name of a profiled kernel: 'void pxg::(anonymous namespace)::fitKernel<3>(...)'
+    gives 'fitKernel<3>'."""
+    name = name.replace("(anonymous namespace)", "anon").replace("void ", "")
+    return name.split("(")[0].split("::")[-1].strip()
+
+
+def kernel_label(sym):
+    return KERNEL_LABEL.get(sym.split("<")[0])
+
+
 class Roctx:
     """ROCTx ranges around the phases (no-ops when the library is absent)."""
 
@@ -601,7 +612,7 @@ def summarize_trace(d):
     order = ["timed", "profiled", "isolated", "warmup", "autotune", "parity", "plain", "serving", "other"]
     stats = {}
     for k in ks:
-        sym = k["Kernel_Name"].split("(")[0].split("<")[0].replace("void ", "").split("::")[-1].strip()
+        sym = kernel_symbol(k["Kernel_Name"])
         t0, t1 = int(k["Start_Timestamp"]), int(k["End_Timestamp"])
         ph = next((p for a, b, p in phases if a <= t0 <= b), "other")
         s = stats.setdefault(sym, {})
@@ -616,7 +627,7 @@ def summarize_trace(d):
             return f"{t / c:.2f}" if c else ""
         c, t = s.get("timed", (0, 0.0))
         share = f"{100 * t / tot_timed:.2f} %" if tot_timed and c else ""
-        print(f"| {sym} | {KERNEL_LABEL.get(sym, '')} | {c} | {avg('timed')} | {share} | {avg('profiled')} | "
+        print(f"| {sym} | {kernel_label(sym) or ''} | {c} | {avg('timed')} | {share} | {avg('profiled')} | "
               f"{avg('isolated')} |")
     if phases:
         print()
@@ -635,7 +646,7 @@ def _pmc_per_kernel(d, counter):
     for r in _read_csv(f[0]):
         if r.get("Counter_Name") != counter:
             continue
-        sym = r["Kernel_Name"].split("(")[0].split("<")[0].replace("void ", "").split("::")[-1].strip()
+        sym = kernel_symbol(r["Kernel_Name"])
         c, v, m = out.get(sym, (0, 0.0, 0.0))
         x = float(r["Counter_Value"])
         out[sym] = (c + 1, v + x, max(m, x))
@@ -656,14 +667,14 @@ def summarize_pmc(fetch_dir, write_dir, events, copy_mib):
     fpu, wpu = nbytes / cf_, nbytes / cw_
     labels = {}
     for sym, (c, v, _) in fetch.items():
-        lab = KERNEL_LABEL.get(sym)
+        lab = kernel_label(sym)
         if not lab:
             continue
         a = labels.setdefault(lab, [0.0, 0.0, 0])
         a[0] += v * fpu / events
         a[2] += c
     for sym, (c, v, _) in write.items():
-        lab = KERNEL_LABEL.get(sym)
+        lab = kernel_label(sym)
         if lab:
             labels.setdefault(lab, [0.0, 0.0, 0])[1] += v * wpu / events
     out = {"what": "HBM bytes per event of each profiler label: FETCH_SIZE and WRITE_SIZE summed over its "

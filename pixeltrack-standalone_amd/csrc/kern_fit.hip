@@ -4,6 +4,12 @@
 // (triplets with N = 3, quadruplets and pentuplets on their first 4 hits).
 // The small symmetric positive-definite systems ((N + 1) x (N + 1) at most)
 // are solved through an LDL^T factorisation held in registers.
+//
+// Triplets and quadruplets run as two kernels, each with its own register
+// footprint, and the double-precision atan2, sin, cos and log are called, not
+// inlined: inlined, the unrolled loops interleaved N of them at once and the
+// fit needed about 370 registers per lane (one wave per SIMD).  Called, the
+// triplet fit takes 146 and the quadruplet fit 168 (three waves per SIMD).
 #include <hip/hip_runtime.h>
 
 #include "pxgpu_internal.h"
@@ -12,6 +18,12 @@ namespace pxg {
 
   namespace {
     __device__ __forceinline__ double sq(double a) { return a * a; }
+
+    // the math library's double functions behind a call each (same results)
+    __device__ __noinline__ double atan2Call(double y, double x) { return atan2(y, x); }
+    __device__ __noinline__ double cosCall(double x) { return cos(x); }
+    __device__ __noinline__ double sinCall(double x) { return sin(x); }
+    __device__ __noinline__ double logCall(double x) { return log(x); }
 
     template <int n>
     __device__ void invertSPD(const double* a, double* inv) {
@@ -69,7 +81,7 @@ namespace pxg {
       pt2 *= pt2;
       constexpr double XXI_0 = 0.06 / 16.;
       constexpr double fact = 0.7 * (13.6 / 1000.) * (13.6 / 1000.);
-      return fact / (pt2 * (1. + sq(slope))) * (fabs(length) * XXI_0) * sq(1. + 0.038 * log(fabs(length) * XXI_0));
+      return fact / (pt2 * (1. + sq(slope))) * (fabs(length) * XXI_0) * sq(1. + 0.038 * logCall(fabs(length) * XXI_0));
     }
 
     struct Rot2 {
@@ -135,11 +147,11 @@ namespace pxg {
     // BrokenLine.h:85-108
     __device__ void translateKarimaki(CircleFit& c, double x0, double y0) {
       const double p0 = c.par[0], p1 = c.par[1], p2 = c.par[2];
-      const double DP = x0 * cos(p0) + y0 * sin(p0);
-      const double DO = x0 * sin(p0) - y0 * cos(p0) + p1;
+      const double DP = x0 * cosCall(p0) + y0 * sinCall(p0);
+      const double DO = x0 * sinCall(p0) - y0 * cosCall(p0) + p1;
       const double uu = 1 + p2 * p1;
-      const double C = -p2 * y0 + uu * cos(p0);
-      const double BB = p2 * x0 + uu * sin(p0);
+      const double C = -p2 * y0 + uu * cosCall(p0);
+      const double BB = p2 * x0 + uu * sinCall(p0);
       const double A = 2. * DO + p2 * (sq(DO) + sq(DP));
       const double U = sqrt(1. + p2 * A);
       const double xi = 1. / (sq(BB) + sq(C));
@@ -149,7 +161,7 @@ namespace pxg {
       const double zeta = sq(DO) + sq(DP);
       const double J[9] = {xi * uu * v, -xi * sq(p2) * DP, xi * DP, 2. * mu * uu * DP, 2. * mu * v,
                            mu * zeta - lambda * A, 0, 0, 1.};
-      c.par[0] = atan2(BB, C);
+      c.par[0] = atan2Call(BB, C);
       c.par[1] = A / (1 + U);
       sandwich3(J, c.cov);
     }
@@ -209,7 +221,7 @@ namespace pxg {
       r[2] = sqrt(a2 * b2 * c2) / (2. * fabs(cross2(bx, by, ax, ay)));
       const double dx = h[0][0] - r[0], dy = h[1][0] - r[1];
       const double ex = h[0][n - 1] - r[0], ey = h[1][n - 1] - r[1];
-      r[3] = r[2] * atan2(cross2(dx, dy, ex, ey), dx * ex + dy * ey) / (h[2][n - 1] - h[2][0]);
+      r[3] = r[2] * atan2Call(cross2(dx, dy, ex, ey), dx * ex + dy * ey) / (h[2][n - 1] - h[2][0]);
     }
 
     // BrokenLine.h:119-162
@@ -233,7 +245,7 @@ namespace pxg {
       for (int i = 0; i < n; ++i) {
         dx = d.radii[0][i];
         dy = d.radii[1][i];
-        d.s[i] = d.q * ff[2] * atan2(cross2(dx, dy, ex, ey), dx * ex + dy * ey);
+        d.s[i] = d.q * ff[2] * atan2Call(cross2(dx, dy, ex, ey), dx * ex + dy * ey);
       }
 #pragma unroll
       for (int i = 0; i < n; ++i) {
@@ -397,7 +409,7 @@ namespace pxg {
       const double ey = h[1][1] + (-d.Z[1] + u[1]) * d.radii[1][1];
       const double emdx = ex - dx, emdy = ey - dy;
       const double tmp1 = emdx * emdx + emdy * emdy;
-      c.par[0] = atan2(emdy, emdx);
+      c.par[0] = atan2Call(emdy, emdx);
       c.par[1] = -c.q * (ff[2] - sqrt(sq(ff[2]) - 0.25 * tmp1));
       c.par[2] = c.q * (1. / ff[2] + u[n]);
       const int qHalf = c.q / 2;  // integer division as in the reference: 0 for q = +-1
@@ -496,25 +508,23 @@ namespace pxg {
       ca.chi2[tb + tkid] = float((Ci.chi2 + L.chi2) / (2 * N - 5));
     }
 
-    __global__ void __launch_bounds__(256) fitKernel(DevConditions cond, Pool p, CAPool ca) {
+    // the fits of the tuples with N hits: sizes 3, then 4 and 5 (fitted on 4 hits)
+    template <int N>
+    __global__ void __launch_bounds__(256, 3) fitKernel(DevConditions cond, Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const uint32_t* mo = ca.multOff + size_t(e) * 16;
       const TupleIndex* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
-      const uint32_t b3 = mo[3], b4 = mo[4], end = mo[6];  // sizes 3, then 4 and 5 (fitted on 4 hits)
+      const uint32_t begin = N == 3 ? mo[3] : mo[4], end = N == 3 ? mo[4] : mo[6];
       const double B = double(float(0.0114256972711507));  // CAHitNtupletCUDA.cc:31 (float field)
-      for (uint32_t k = b3 + logicalBlock().x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x) {
-        const uint32_t tkid = mb[k];
-        if (k < b4)
-          fitTuple<3>(cond, p, ca, e, tkid, B);
-        else
-          fitTuple<4>(cond, p, ca, e, tkid, B);
-      }
+      for (uint32_t k = begin + logicalBlock().x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x)
+        fitTuple<N>(cond, p, ca, e, mb[k], B);
     }
   }  // namespace
 
   void launchFit(const CALaunch& l) {
     l.t->b("fit");
-    fitKernel<<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
+    fitKernel<3><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
+    fitKernel<4><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
     l.t->e();
   }
 
