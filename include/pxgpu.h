@@ -184,13 +184,17 @@ int pxg_rechits(pxg_stream* s, const pxg_digis* digis, const pxg_clusters* clust
 
 /* Rec-hits -> ntuplets with broken-line fits and quality (CAHitNtupletGeneratorOnGPU::makeTuples,
  * CAHitNtupletGeneratorOnGPU.cc:96-126; CAHitNtupletCUDA::produce, CAHitNtupletCUDA.cc:38-47); the
- * hits must be the products of the last pxg_rechits on the same stream. */
-int pxg_ntuplets(pxg_stream* s, const pxg_hits* hits, pxg_tracks* tracks);
+ * hits must be the products of the last pxg_rechits on the same stream.  bfield is the field of the
+ * helix fits in 1/GeV that produce passes to makeTuples (0.0114256972711507, CAHitNtupletCUDA.cc:31);
+ * it stays the stream's setting (pxg_set_bfield) for later batched runs. */
+int pxg_ntuplets(pxg_stream* s, const pxg_hits* hits, float bfield, pxg_tracks* tracks);
 
 /* Tracks -> vertices (PixelVertexProducerCUDA::produce, PixelVertexProducerCUDA.cc:26-40;
  * gpuVertexFinder::Producer::make, gpuVertexFinderImpl.h:84-111); the tracks must be the
- * products of the last pxg_ntuplets on the same stream. */
-int pxg_find_vertices(pxg_stream* s, const pxg_tracks* tracks, pxg_vertices* vertices);
+ * products of the last pxg_ntuplets on the same stream.  ptMin is the producer's track pt cut
+ * (m_ptMin = 0.5 GeV, PixelVertexProducerCUDA.cc:37, applied in loadTracks,
+ * gpuVertexFinderImpl.h:33); it stays the stream's setting (pxg_set_vertex_ptmin). */
+int pxg_find_vertices(pxg_stream* s, const pxg_tracks* tracks, float ptMin, pxg_vertices* vertices);
 
 /* ---- resident / batched path --------------------------------------------------------- */
 
@@ -296,6 +300,13 @@ int pxg_set_fishbone(pxg_stream* s, int early, int late);
  * (CAHitNtupletGeneratorOnGPU.cc:131-135 and :168-172, RiemannFitOnGPU.cc:3-113; default off,
  * CAHitNtupletGeneratorOnGPU.cc:52): nonzero fits with the Riemann fit, zero with the broken line. */
 int pxg_set_riemann_fit(pxg_stream* s, int riemann);
+/* Field of the stream's helix fits in 1/GeV, as CAHitNtupletCUDA::produce passes it to makeTuples
+ * (CAHitNtupletCUDA.cc:31, default 0.0114256972711507; HelixFitOnGPU keeps it as a float,
+ * HelixFitOnGPU.h:43).  Positive and finite, else PXG_ERR_ARG. */
+int pxg_set_bfield(pxg_stream* s, float bfield);
+/* Track pt cut of the stream's vertex finder (PixelVertexProducerCUDA's m_ptMin,
+ * PixelVertexProducerCUDA.cc:37, default 0.5 GeV).  Not negative and finite, else PXG_ERR_ARG. */
+int pxg_set_vertex_ptmin(pxg_stream* s, float ptMin);
 
 /* The ntuplet generator's statistics: the reference's doStats switch (default off,
  * CAHitNtupletGeneratorOnGPU.cc:58) and its Counters (CAHitNtupletGeneratorKernels.h:14-26),

@@ -51,6 +51,10 @@ def _load(path, prefix):
             lib.orc_rmath.argtypes = [C.c_int, C.c_double, C.c_double]
             lib.orc_conditions_set_riemann.restype = C.c_int
             lib.orc_conditions_set_riemann.argtypes = [vp, C.c_int]
+            lib.orc_conditions_set_bfield.restype = C.c_int
+            lib.orc_conditions_set_bfield.argtypes = [vp, C.c_float]
+            lib.orc_conditions_set_vertex_ptmin.restype = C.c_int
+            lib.orc_conditions_set_vertex_ptmin.argtypes = [vp, C.c_float]
             lib.orc_vertices.restype = C.c_int
             lib.orc_vertices.argtypes = [C.c_uint32] + [vp] * 10
             lib.orc_vertices_algo.restype = C.c_int
@@ -162,8 +166,12 @@ class Oracle(_Chain):
     VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2
 
     def __init__(self, conditions, profile=0, vertex_algo=0, early_fishbone=True, late_fishbone=False,
-                 riemann_fit=False):
+                 riemann_fit=False, bfield=None, vertex_ptmin=None):
         super().__init__(conditions)
+        if bfield is not None and self.lib.orc_conditions_set_bfield(self._h, bfield) != 0:
+            raise ValueError(f"bad field {bfield}")
+        if vertex_ptmin is not None and self.lib.orc_conditions_set_vertex_ptmin(self._h, vertex_ptmin) != 0:
+            raise ValueError(f"bad vertex pt cut {vertex_ptmin}")
         self.lib.orc_conditions_set_fishbone(self._h, int(early_fishbone), int(late_fishbone))
         self.lib.orc_conditions_set_riemann(self._h, int(riemann_fit))
         if self.lib.orc_conditions_set_profile(self._h, profile) != 0:

@@ -46,11 +46,12 @@ namespace orc {
     int vertexAlgo = 0;  // 0 density, 1 DBSCAN, 2 iterative (gpuVertexFinder.h:46-60)
     bool earlyFishbone = true, lateFishbone = false;  // CAHitNtupletGeneratorOnGPU.cc:55-56
     bool riemannFit = false;                          // CAHitNtupletGeneratorOnGPU.cc:52 useRiemannFit
+    float bField = 0.0114256972711507f;               // CAHitNtupletCUDA.cc:31, a float in makeTuples
+    float vertexPtMin = 0.5f;                         // PixelVertexProducerCUDA.cc:37 m_ptMin
   };
 
   // ---- stage C/D fixed capacities (CAConstants.h, ZVertexSoA.h); the
   // run-time ones are in Event::caps (pxformats.h CACapacities)
-  constexpr double kBField = 0.0114256972711507;  // CAHitNtupletCUDA.cc:31
 
   template <class T, int N>
   struct Vec {
@@ -152,6 +153,7 @@ namespace orc {
     // a11-a22: CA ntuplets and fits; a23-a25: vertices
     CACapacities caps = kCapsReference;
     int vertexAlgo = 0;
+    float vertexPtMin = 0.5f;
     bool earlyFishbone = true, lateFishbone = false;
     CAState ca;
     VertexState vtx;

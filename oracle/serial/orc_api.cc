@@ -156,6 +156,22 @@ int orc_conditions_set_riemann(orc_conditions* h, int riemann) {
   return 0;
 }
 
+// field of the helix fits in 1/GeV (CAHitNtupletCUDA.cc:31) and the vertexer's track pt cut
+// (PixelVertexProducerCUDA.cc:37)
+int orc_conditions_set_bfield(orc_conditions* h, float bfield) {
+  if (!(bfield > 0))
+    return -1;
+  h->c.bField = bfield;
+  return 0;
+}
+
+int orc_conditions_set_vertex_ptmin(orc_conditions* h, float ptMin) {
+  if (!(ptMin >= 0))
+    return -1;
+  h->c.vertexPtMin = ptMin;
+  return 0;
+}
+
 // capacity profile of the ntuplet containers: 0 reference, 1 high occupancy
 // vertex clustering algorithm of the chain: 0 density (the reference's default), 1 DBSCAN, 2 iterative
 int orc_conditions_set_vertex_algo(orc_conditions* h, int algo) {

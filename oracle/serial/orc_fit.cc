@@ -482,7 +482,7 @@ namespace orc {
     C.chi2.assign(ev.caps.maxTuples, 0);
     if (ev.nHits == 0)
       return;
-    const double B = double(float(kBField));
+    const double B = double(c.bField);
     for (uint32_t nh = 3; nh <= 5; ++nh) {
       for (uint32_t k = C.multOff[nh]; k < C.multOff[nh + 1]; ++k) {
         const uint32_t tkid = C.multBins[k];

@@ -1019,7 +1019,7 @@ namespace orc {
     C.chi2.assign(ev.caps.maxTuples, 0);
     if (ev.nHits == 0)
       return;
-    const double B = double(float(kBField));
+    const double B = double(c.bField);
     auto one = [&](auto tag, uint32_t tkid) {
       constexpr int N = decltype(tag)::value;
       double h[3][N];

@@ -291,6 +291,7 @@ namespace orc {
   bool process(const Conditions& c, const uint8_t* raw, size_t len, int upto, Event& ev) {
     ev.caps = c.caps;
     ev.vertexAlgo = c.vertexAlgo;
+    ev.vertexPtMin = c.vertexPtMin;
     ev.earlyFishbone = c.earlyFishbone;
     ev.lateFishbone = c.lateFishbone;
     if (!unpackFeds(c, raw, len, ev))

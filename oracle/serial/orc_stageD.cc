@@ -19,7 +19,7 @@ namespace orc {
     V = VertexState{};
     V.idv.assign(ev.caps.maxTuples, 0);
     V.ndof.assign(ev.caps.maxTuples, 0);
-    constexpr float ptMin = 0.5f;
+    const float ptMin = ev.vertexPtMin;
 
     // loadTracks (gpuVertexFinderImpl.h:10-43)
     std::vector<TupleIndex> itrk;

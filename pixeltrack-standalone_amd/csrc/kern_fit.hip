@@ -510,12 +510,11 @@ namespace pxg {
 
     // the fits of the tuples with N hits: sizes 3, then 4 and 5 (fitted on 4 hits)
     template <int N>
-    __global__ void __launch_bounds__(256, 3) fitKernel(DevConditions cond, Pool p, CAPool ca) {
+    __global__ void __launch_bounds__(256, 3) fitKernel(DevConditions cond, Pool p, CAPool ca, double B) {
       const uint32_t e = logicalBlock().y;
       const uint32_t* mo = ca.multOff + size_t(e) * 16;
       const TupleIndex* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
       const uint32_t begin = N == 3 ? mo[3] : mo[4], end = N == 3 ? mo[4] : mo[6];
-      const double B = double(float(0.0114256972711507));  // CAHitNtupletCUDA.cc:31 (float field)
       for (uint32_t k = begin + logicalBlock().x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x)
         fitTuple<N>(cond, p, ca, e, mb[k], B);
     }
@@ -523,8 +522,8 @@ namespace pxg {
 
   void launchFit(const CALaunch& l) {
     l.t->b("fit");
-    fitKernel<3><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
-    fitKernel<4><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
+    fitKernel<3><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca, l.bField);
+    fitKernel<4><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca, l.bField);
     l.t->e();
   }
 

@@ -1183,13 +1183,12 @@ namespace pxg {
     // triplets and quadruplets take different code paths: one launch each, so that
     // a wave runs one of them
     template <int N>
-    __global__ void __launch_bounds__(256) riemannKernel(DevConditions cond, Pool p, CAPool ca) {
+    __global__ void __launch_bounds__(256) riemannKernel(DevConditions cond, Pool p, CAPool ca, double B) {
       const uint32_t e = logicalBlock().y;
       const uint32_t* mo = ca.multOff + size_t(e) * 16;
       const TupleIndex* mb = ca.multBins + size_t(e) * ca.caps.maxTuplesMult;
       // sizes 3 (N = 3), then 4 and 5 on their first 4 hits (N = 4)
       const uint32_t begin = N == 3 ? mo[3] : mo[4], end = N == 3 ? mo[4] : mo[6];
-      const double B = double(float(0.0114256972711507));  // CAHitNtupletCUDA.cc:31 (float field)
       for (uint32_t k = begin + logicalBlock().x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x)
         riemannTuple<N>(cond, p, ca, e, mb[k], B);
     }
@@ -1197,10 +1196,10 @@ namespace pxg {
 
   void launchRiemannFit(const CALaunch& l) {
     l.t->b("riemannFit3");
-    riemannKernel<3><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
+    riemannKernel<3><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca, l.bField);
     l.t->e();
     l.t->b("riemannFit4");
-    riemannKernel<4><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca);
+    riemannKernel<4><<<dim3(l.blocks, l.nEvents), 256, 0, l.s>>>(*l.cond, *l.p, *l.ca, l.bField);
     l.t->e();
   }
 

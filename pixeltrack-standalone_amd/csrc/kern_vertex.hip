@@ -94,7 +94,7 @@ namespace pxg {
       }
     }
 
-    __global__ void __launch_bounds__(kVB) vertexKernel(CAPool ca, int algo) {
+    __global__ void __launch_bounds__(kVB) vertexKernel(CAPool ca, int algo, float ptMin) {
       const uint32_t e = blockIdx.x;
       __shared__ VtxLds L;
       const size_t tb = ca.tupleBase(e);
@@ -118,7 +118,7 @@ namespace pxg {
       __shared__ float zv[kMaxVertices], wv[kMaxVertices], chi2[kMaxVertices], ptv2[kMaxVertices];
       uint16_t* sortInd = ca.sortInd + size_t(e) * kMaxVertices;
       constexpr int minT = 2;
-      constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f, ptMin = 0.5f;
+      constexpr float eps = 0.07f, errmax = 0.01f, chi2max = 9.f;
 
       // ---- loadTracks: loose tuples with 4+ hits and pt >= 0.5 GeV, in tuple order.
       // The tuples go in chunks of 64 * kVB; in a chunk each thread takes a contiguous
@@ -674,7 +674,7 @@ namespace pxg {
 
   void launchVertices(const CALaunch& l) {
     l.t->b("vertices");
-    vertexKernel<<<l.nEvents, kVB, 0, l.s>>>(*l.ca, l.vertexAlgo);
+    vertexKernel<<<l.nEvents, kVB, 0, l.s>>>(*l.ca, l.vertexAlgo, l.vertexPtMin);
     l.t->e();
   }
 

@@ -345,7 +345,7 @@ namespace pxg::detail {
       timer = {timerBegin, timerEnd, s};
     const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
                      s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone,
-                     s->riemannFit, s->doStats ? s->counters : nullptr};
+                     s->riemannFit, s->doStats ? s->counters : nullptr, double(s->bField), s->vertexPtMin};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -793,12 +793,15 @@ int pxg_rechits(pxg_stream* s, const pxg_digis* d, const pxg_clusters* c, const 
   return rc;
 }
 
-int pxg_ntuplets(pxg_stream* s, const pxg_hits* h, pxg_tracks* t) {
+int pxg_ntuplets(pxg_stream* s, const pxg_hits* h, float bfield, pxg_tracks* t) {
   if (!s || !h || !t)
     return fail(PXG_ERR_ARG, "pxg_ntuplets: null argument");
   if (s->nLoaded < 1 || h->xg != s->pool.xg)
     return fail(PXG_ERR_ARG, "pxg_ntuplets: hits are not the last products of this stream");
-  int rc = runStages(s, 0, 1, 4, 4);
+  int rc = pxg_set_bfield(s, bfield);
+  if (rc)
+    return rc;
+  rc = runStages(s, 0, 1, 4, 4);
   if (rc)
     return rc;
   rc = pxg_fetch_counts(s, 0, 1);
@@ -808,12 +811,15 @@ int pxg_ntuplets(pxg_stream* s, const pxg_hits* h, pxg_tracks* t) {
   return rc;
 }
 
-int pxg_find_vertices(pxg_stream* s, const pxg_tracks* t, pxg_vertices* v) {
+int pxg_find_vertices(pxg_stream* s, const pxg_tracks* t, float ptMin, pxg_vertices* v) {
   if (!s || !t || !v)
     return fail(PXG_ERR_ARG, "pxg_find_vertices: null argument");
   if (s->nLoaded < 1 || t->hitOff != s->ca.tupleOff)
     return fail(PXG_ERR_ARG, "pxg_find_vertices: tracks are not the last products of this stream");
-  int rc = runStages(s, 0, 1, 5, 5);
+  int rc = pxg_set_vertex_ptmin(s, ptMin);
+  if (rc)
+    return rc;
+  rc = runStages(s, 0, 1, 5, 5);
   if (rc)
     return rc;
   rc = pxg_fetch_counts(s, 0, 1);
@@ -854,6 +860,26 @@ int pxg_set_vertex_algorithm(pxg_stream* s, int algo) {
                 PXG_VERTEX_ITERATIVE == kVertexIterative);
   s->vertexAlgo = algo;
   dropGraphs(s);
+  return 0;
+}
+
+int pxg_set_bfield(pxg_stream* s, float bfield) {
+  if (!s || !(bfield > 0.f) || !(bfield < 1e30f))
+    return fail(PXG_ERR_ARG, "pxg_set_bfield: null stream or field not positive and finite");
+  if (bfield != s->bField) {
+    s->bField = bfield;
+    dropGraphs(s);
+  }
+  return 0;
+}
+
+int pxg_set_vertex_ptmin(pxg_stream* s, float ptMin) {
+  if (!s || !(ptMin >= 0.f) || !(ptMin < 1e30f))
+    return fail(PXG_ERR_ARG, "pxg_set_vertex_ptmin: null stream or pt cut negative or not finite");
+  if (ptMin != s->vertexPtMin) {
+    s->vertexPtMin = ptMin;
+    dropGraphs(s);
+  }
   return 0;
 }
 

@@ -252,6 +252,8 @@ namespace pxg {
     bool lateFishbone;      // fishbone on the cells in tuples after it, and the cleaner after classify
     bool riemannFit;        // Riemann fit in place of the broken-line fit
     unsigned long long* counters;  // doStats counters (11), null when off
+    double bField;          // the fits' field in 1/GeV (a float widened, as makeTuples passes it)
+    float vertexPtMin;      // tracks below this pt are left out of the vertex finder
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);

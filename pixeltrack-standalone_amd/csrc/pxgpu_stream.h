@@ -45,6 +45,8 @@ struct pxg_stream {
   bool lateFishbone = false;
   bool riemannFit = false;     // the reference's default: broken-line fit
   bool doStats = false;        // the reference's default: no counters
+  float bField = 0.0114256972711507f;  // CAHitNtupletCUDA.cc:31
+  float vertexPtMin = 0.5f;            // PixelVertexProducerCUDA.cc:37
   unsigned long long* counters = nullptr;  // device, PXG_N_COUNTERS
   // pxg_run's stage sequences captured as HIP graphs, by (first, count, upto); their
   // grid sizes come from the loaded events and the settings, so loading events or

@@ -235,9 +235,10 @@ public:
       : hits_(reg.consumes<TrackingRecHit2DCPU>()), tracks_(reg.produces<PixelTrackHeterogeneous>()) {}
 
   void produce(edm::Event& ev, edm::EventSetup const&) override {
+    const float bf = 0.0114256972711507f;  // 1/fieldInGeV, CAHitNtupletCUDA.cc:31
     auto const& h = ev.get(hits_);
     pxg_tracks t;
-    const int st = pxg_ntuplets(h.stream, &h.view, &t);
+    const int st = pxg_ntuplets(h.stream, &h.view, bf, &t);
     check(st, "pxg_ntuplets");
     ev.emplace(tracks_, PixelTrackHeterogeneous{h.stream, t, st});
   }
@@ -257,11 +258,12 @@ public:
   void produce(edm::Event& ev, edm::EventSetup const&) override {
     auto const& t = ev.get(tracks_);
     pxg_vertices v;
-    check(pxg_find_vertices(t.stream, &t.view, &v), "pxg_find_vertices");
+    check(pxg_find_vertices(t.stream, &t.view, ptMin_, &v), "pxg_find_vertices");
     ev.emplace(vertices_, ZVertexHeterogeneous{t.stream, v});
   }
 
 private:
+  const float ptMin_ = 0.5f;  // m_ptMin, PixelVertexProducerCUDA.cc:37
   edm::EDGetTokenT<PixelTrackHeterogeneous> tracks_;
   edm::EDPutTokenT<ZVertexHeterogeneous> vertices_;
 };

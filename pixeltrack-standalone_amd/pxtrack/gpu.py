@@ -77,6 +77,8 @@ class ServerResult(C.Structure):
 CA_COUNTS = ["nCells", "nDoublets", "apcM", "apcN", "nTuples", "caStatus", "nbLists", "trLists", "nMulti", "nMult",
              "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors", "nStart"]
 PROFILE_REFERENCE, PROFILE_HIGH_OCCUPANCY = 0, 1
+BFIELD = 0.0114256972711507  # 1/GeV, CAHitNtupletCUDA.cc:31
+VERTEX_PTMIN = 0.5           # GeV, PixelVertexProducerCUDA.cc:37
 VERTEX_DENSITY, VERTEX_DBSCAN, VERTEX_ITERATIVE = 0, 1, 2  # pxg_set_vertex_algorithm
 
 
@@ -97,8 +99,8 @@ def _gpu():
         lib.pxg_device_name.argtypes = [C.c_char_p, C.c_size_t]
         lib.pxg_stream_create.argtypes = [vp, u32, u32, C.POINTER(vp)]
         lib.pxg_stream_create_profile.argtypes = [vp, u32, u32, C.c_int, C.POINTER(vp)]
-        lib.pxg_ntuplets.argtypes = [vp, C.POINTER(Hits), C.POINTER(Tracks)]
-        lib.pxg_find_vertices.argtypes = [vp, C.POINTER(Tracks), C.POINTER(Vertices)]
+        lib.pxg_ntuplets.argtypes = [vp, C.POINTER(Hits), C.c_float, C.POINTER(Tracks)]
+        lib.pxg_find_vertices.argtypes = [vp, C.POINTER(Tracks), C.c_float, C.POINTER(Vertices)]
         lib.pxg_event_tracks.argtypes = [vp, u32, C.POINTER(Tracks), C.POINTER(Vertices)]
         lib.pxg_event_ca_internals.argtypes = [vp, u32, C.POINTER(CAInternals)]
         lib.pxg_pool_bytes.restype = C.c_size_t
@@ -115,6 +117,8 @@ def _gpu():
         lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
         lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
         lib.pxg_set_riemann_fit.argtypes = [vp, C.c_int]
+        lib.pxg_set_bfield.argtypes = [vp, C.c_float]
+        lib.pxg_set_vertex_ptmin.argtypes = [vp, C.c_float]
         lib.pxg_set_stats.argtypes = [vp, C.c_int]
         lib.pxg_set_graphs.argtypes = [vp, C.c_int]
         lib.pxg_get_counters.argtypes = [vp, C.POINTER(C.c_uint64)]
@@ -220,6 +224,7 @@ class Stream:
         _check(self.lib.pxg_stream_create_profile(ctx._h, max_events, max_words, profile, C.byref(self._h)),
                "pxg_stream_create_profile")
         self.caps = capacities(profile)
+        self.bfield, self.vertex_ptmin = BFIELD, VERTEX_PTMIN
         self._inputs = []
 
     def close(self):
@@ -356,12 +361,13 @@ class Stream:
 
     def ntuplets(self, h):
         t = Tracks()
-        st = _check(self.lib.pxg_ntuplets(self._h, C.byref(h), C.byref(t)), "pxg_ntuplets")
+        st = _check(self.lib.pxg_ntuplets(self._h, C.byref(h), self.bfield, C.byref(t)), "pxg_ntuplets")
         return st, t
 
     def find_vertices(self, t):
         v = Vertices()
-        st = _check(self.lib.pxg_find_vertices(self._h, C.byref(t), C.byref(v)), "pxg_find_vertices")
+        st = _check(self.lib.pxg_find_vertices(self._h, C.byref(t), self.vertex_ptmin, C.byref(v)),
+                    "pxg_find_vertices")
         return st, v
 
     def process(self, words, fed_ids, upto=3, internals=False, beamspot=None):
@@ -470,6 +476,18 @@ class Stream:
     def set_riemann_fit(self, on=True):
         """Riemann fit in place of the broken-line fit (default off)."""
         _check(self.lib.pxg_set_riemann_fit(self._h, int(on)), "pxg_set_riemann_fit")
+
+    def set_bfield(self, bfield=None):
+        """Field of the helix fits in 1/GeV (default: CAHitNtupletCUDA's 0.0114256972711507)."""
+        b = BFIELD if bfield is None else bfield
+        _check(self.lib.pxg_set_bfield(self._h, b), "pxg_set_bfield")
+        self.bfield = b
+
+    def set_vertex_ptmin(self, pt=None):
+        """Track pt cut of the vertex finder in GeV (default: PixelVertexProducerCUDA's 0.5)."""
+        p = VERTEX_PTMIN if pt is None else pt
+        _check(self.lib.pxg_set_vertex_ptmin(self._h, p), "pxg_set_vertex_ptmin")
+        self.vertex_ptmin = p
 
     def tuning(self):
         a, b = u32(), u32()

@@ -203,6 +203,43 @@ def test_riemann_fit(detector, conditions, gpu_ctx):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"riemann pu50[{idx}]")
 
 
+def test_bfield_and_vertex_ptmin(detector, conditions, gpu_ctx):
+    """The field makeTuples takes (CAHitNtupletCUDA.cc:31) and the vertex finder's pt cut
+    (PixelVertexProducerCUDA.cc:37) as parameters: a weaker field and a higher cut on both
+    sides change the products the same way, on the plain path and on a batched launch
+    (the stream settings), for the broken-line and the Riemann fit."""
+    import pyoracle
+    from pxtrack import gpu
+    bf, ptmin = 0.0095, 0.8
+    raws = [event(detector, "pu50", idx)[0] for idx in (1, 4)]
+    default = pyoracle.Oracle(conditions).process(raws[0], upto=5)
+    for riemann in (False, True):
+        orc = pyoracle.Oracle(conditions, bfield=bf, vertex_ptmin=ptmin, riemann_fit=riemann)
+        plain = gpu_ctx.stream(max_events=1, max_words=300000)
+        plain.set_riemann_fit(riemann)
+        plain.bfield, plain.vertex_ptmin = bf, ptmin  # passed by pxg_ntuplets / pxg_find_vertices
+        batch = gpu_ctx.stream(max_events=2, max_words=600000)
+        batch.set_riemann_fit(riemann)
+        batch.set_bfield(bf)
+        batch.set_vertex_ptmin(ptmin)
+        batch.load([gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in raws])
+        batch.run(0, 2, upto=5)
+        batch.fetch_counts(0, 2)
+        for i, raw in enumerate(raws):
+            o = orc.process(raw, upto=5)
+            if i == 0 and not riemann:
+                n = o["nTuples"]
+                assert not np.array_equal(o["pt"][:n], default["pt"][:n])
+                assert o["vtxTracks"] < default["vtxTracks"]
+            print(f"bfield {bf}, ptMin {ptmin}, riemann {riemann}: event {i} tracks to the vertexer "
+                  f"{o['vtxTracks']}, vertices {o['nVertices']}", flush=True)
+            compare_ca(o, _gpu_event(gpu_ctx, plain, raw), f"bf/ptmin plain[{i}] riemann={riemann}")
+            compare_ca(o, batch.event_products(i, upto=5), f"bf/ptmin batch[{i}] riemann={riemann}",
+                       internals=False)
+        plain.close()
+        batch.close()
+
+
 @pytest.mark.parametrize("early,late", [(False, True), (True, True), (False, False)])
 def test_fishbone_switches(detector, conditions, gpu_ctx, early, late):
     """The reference's earlyFishbone / lateFishbone switches

@@ -305,6 +305,38 @@ def test_high_occupancy_profile_holds_pu200(detector, conditions):
     assert ent.max() > 65535
 
 
+def test_vertex_ptmin_selects_the_tracks(detector, conditions):
+    """loadTracks (gpuVertexFinderImpl.h:10-43) with the producer's pt cut as a parameter:
+    the vertex finder gets exactly the loose tuples with 4 or more hits and pt >= ptMin."""
+    import pyoracle
+    raw = event(detector, "pu50", 2)[0]
+    counts = []
+    for ptmin in (0.0, 0.5, 1.0, 2.0):
+        o = pyoracle.Oracle(conditions, vertex_ptmin=ptmin).process(raw, upto=5)
+        n = o["nTuples"]
+        size = np.diff(o["tupleOff"][:n + 1].astype(np.int64))
+        take = (size >= 4) & (o["quality"][:n] == 2) & ~(o["pt"][:n] < np.float32(ptmin))
+        assert o["vtxTracks"] == take.sum(), ptmin
+        assert (o["idv"][:n][~take & (size > 0)] == -1).all(), ptmin
+        counts.append(int(take.sum()))
+    assert counts[0] >= counts[1] > counts[2] > counts[3] > 0
+
+
+def test_bfield_scales_pt(detector, conditions):
+    """The fits' field enters pt = B / |curvature| (TrajectoryStateSoA copyFromCircle); the
+    curvature itself moves only through the scattering weights, so pt scales with B to well
+    within a percent for most tracks."""
+    import pyoracle
+    raw = event(detector, "pu50", 2)[0]
+    a = pyoracle.Oracle(conditions).process(raw, upto=4)
+    b = pyoracle.Oracle(conditions, bfield=0.0114256972711507 * 1.25).process(raw, upto=4)
+    n = a["nTuples"]
+    assert b["nTuples"] == n
+    ok = (a["quality"][:n] == 2) & (b["quality"][:n] == 2)
+    r = b["pt"][:n][ok] / a["pt"][:n][ok]
+    assert abs(np.median(r) - 1.25) < 2e-3 and np.mean(np.abs(r - 1.25) < 0.0125) > 0.9
+
+
 def q_before_fit_ok(o, t):
     """A tuple is in the multiplicity container unless the early duplicate remover marked it."""
     return t not in early_dups(o)
