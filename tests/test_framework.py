@@ -140,84 +140,49 @@ def test_gpu_job_count_validator(dataset, oracle_products, tmp_path):
     assert "Average absolute vertex difference" not in r.stdout  # only printed when all pass
 
 
-def _histo(values, nbins, lo, hi):
-    """Fixed-range histogram with under/overflow bins in float32 arithmetic, the
-    rule of the reference's SimpleAtomicHisto::fill."""
-    lo32, hi32 = np.float32(lo), np.float32(hi)
-    out = np.zeros(nbins + 2, dtype=np.int64)
-    for v in np.asarray(values, dtype=np.float32):
-        if v < lo32:
-            out[0] += 1
-        elif v >= hi32:
-            out[-1] += 1
-        else:
-            i = int(np.float32(np.float32(v - lo32) / np.float32(hi32 - lo32)) * np.float32(nbins))
-            out[min(i, nbins - 1) + 1] += 1
-    return out
-
-
 @pytest.mark.gpu
-def test_gpu_job_histograms(dataset, oracle_products, tmp_path):
+def test_gpu_job_histograms(detector, dataset, oracle_products, gpu_ctx, tmp_path):
+    """HistoValidator histograms of three runs of the same events, compared bin by bin
+    with the histogram-ratio tool (pxtrack.histos, plot-histograms.py:97-196): the oracle
+    ("serial", filled in Python), the job driver's plain per-event path ("mi355x", written
+    by its HistoValidator) and the batched path ("batched", one launch over all events).
+    Exact products give equal histograms; the fitted quantities, equal only within the
+    fits' tolerance, may move an entry across a bin edge."""
+    from pxtrack import gen, gpu, histos
     r = subprocess.run([EXE, "--data", dataset, "--numberOfThreads", "2", "--histogram"], capture_output=True,
                        text=True, timeout=300, cwd=str(tmp_path))
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    got = {}
-    for line in open(tmp_path / "histograms_mi355x.txt"):
-        name, n, lo, hi, *counts = line.split()
-        assert int(n) == len(counts)
-        got[name] = (float(lo), float(hi), np.array(counts, dtype=np.int64))
-    assert len(got) == 33
-    exp = {}
-    def add(name, values, nbins, lo, hi):
-        h = _histo(values, nbins, lo, hi)
-        exp[name] = exp.get(name, 0) + h
+    got = histos.read(str(tmp_path / "histograms_mi355x.txt"))
+    assert set(got) == set(histos.BINNING) and len(got) == 33
+    serial = histos.Histograms()
     for o in oracle_products:
-        nt = o["nTuples"]
-        loose = (o["quality"][:nt] >= 2)
-        sizes = np.diff(o["tupleOff"][:nt + 1].astype(np.int64))
-        st = o["state"][:5 * nt].reshape(-1, 5)
-        nv = o["nVertices"]
-        add("digi_n", [o["nDigis"]], 100, 0, 1e5)
-        add("digi_adc", o["adc"][:o["nDigis"]], 250, 0, 5e4)
-        add("module_n", [o["nModules"]], 100, 1500, 2000)
-        add("cluster_n", [o["nClusters"]], 200, 5000, 25000)
-        add("cluster_per_module_n", o["clusInModule"][:o["nModules"]], 110, 0, 110)
-        nh = o["nHits"]
-        add("hit_n", [nh], 200, 5000, 25000)
-        for key, name, nb, lo, hi in (("xl", "hit_lx", 200, -1, 1), ("yl", "hit_ly", 800, -4, 4),
-                                      ("xerr", "hit_lex", 100, 0, 5e-5), ("yerr", "hit_ley", 100, 0, 1e-4),
-                                      ("xg", "hit_gx", 200, -20, 20), ("yg", "hit_gy", 200, -20, 20),
-                                      ("zg", "hit_gz", 600, -60, 60), ("rg", "hit_gr", 200, 0, 20),
-                                      ("charge", "hit_charge", 400, 0, 4e6), ("xsize", "hit_sizex", 800, 0, 800),
-                                      ("ysize", "hit_sizey", 800, 0, 800)):
-            add(name, o[key][:nh], nb, lo, hi)
-        add("track_n", [int(loose.sum())], 150, 0, 15000)
-        add("track_nhits", sizes[loose], 3, 3, 6)
-        add("track_quality", o["quality"][:nt][loose], 6, 0, 6)
-        add("track_chi2", o["chi2"][:nt][loose], 100, 0, 40)
-        add("track_pt", o["pt"][:nt][loose], 400, 0, 400)
-        add("track_eta", o["eta"][:nt][loose], 100, -3, 3)
-        add("track_phi", st[loose, 0], 100, -3.15, 3.15)
-        add("track_tip", st[loose, 1], 100, -1, 1)
-        add("track_tip_zoom", st[loose, 1], 100, -0.05, 0.05)
-        add("track_zip", st[loose, 4], 100, -15, 15)
-        add("track_zip_zoom", st[loose, 4], 100, -0.1, 0.1)
-        add("vertex_n", [nv], 60, 0, 60)
-        add("vertex_z", o["zv"][:nv], 100, -15, 15)
-        add("vertex_chi2", o["chi2v"][:nv], 100, 0, 40)
-        add("vertex_ndof", o["ndof"][:nv], 170, 0, 170)
-        add("vertex_pt2", o["ptv2"][:nv], 100, 0, 4000)
-    # floats the device computes within 1e-5 of the oracle (fits, vertex fits) may sit on
-    # the other side of a bin edge: same totals, at most a few entries moved
-    approx = {"track_chi2", "track_pt", "track_eta", "track_phi", "track_tip", "track_tip_zoom", "track_zip",
-              "track_zip_zoom", "vertex_z", "vertex_chi2", "vertex_pt2"}
-    for name, h in exp.items():
-        g = got[name][2]
-        assert len(g) == len(h), name
-        if name in approx:
-            assert g.sum() == h.sum(), name
-            assert np.abs(g - h).sum() <= 4, (name, np.nonzero(g - h))
+        serial.fill(o)
+    serial.write(str(tmp_path / "histograms_serial.txt"))
+    raws = [detector.event(gen.preset("pu50", **CFG), i)[0] for i in range(N_EVENTS)]
+    inputs = [gpu.unpack_raw(raw, gpu_ctx.fed_ids) for raw in raws]
+    s = gpu_ctx.stream(max_events=N_EVENTS, max_words=int(sum(len(w) for w, _ in inputs)))
+    s.load(inputs)
+    s.run(0, N_EVENTS, upto=5)
+    s.fetch_counts(0, N_EVENTS)
+    batched = histos.Histograms()
+    for e in range(N_EVENTS):
+        batched.fill(s.event_products(e, upto=5))
+    s.close()
+    batched.write(str(tmp_path / "histograms_batched.txt"))
+    for name, (lo, hi, counts) in got.items():
+        nb, blo, bhi = histos.BINNING[name]
+        assert len(counts) == nb + 2 and lo == pytest.approx(blo, rel=1e-5) and hi == pytest.approx(bhi, rel=1e-5)
+        exp = serial.counts[name]
+        if name in histos.APPROX:
+            assert counts.sum() == exp.sum(), name
+            assert np.abs(counts - exp).sum() <= 4, (name, np.nonzero(counts - exp))
         else:
-            assert np.array_equal(g, h), (name, np.nonzero(g - h))
-    assert set(exp) == set(got)
+            assert np.array_equal(counts, exp), (name, np.nonzero(counts - exp))
+    files = [str(tmp_path / f"histograms_{k}.txt") for k in ("serial", "mi355x", "batched")]
+    plots = tmp_path / "plots"
+    rc = histos.main(["--den", "serial", *files, "--approx-tolerance", "2e-3", "--plot", str(plots)])
+    assert rc == 0
+    assert len(list(plots.glob("*.png"))) == 33
+    # batched and plain runs give identical products, hence identical histograms
+    assert histos.main(["--den", "mi355x", files[1], files[2]]) == 0
