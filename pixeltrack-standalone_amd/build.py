@@ -3,6 +3,7 @@
   libpxgen.so   synthetic event source (g++)
   libpxhost.so  host FED loop and the EDM-style framework with its plugins (g++)
   libpxgpu.so   HIP kernels + C ABI (hipcc --offload-arch=gfx950)
+  tests/native/libpxprim_selftest.so   self-test kernels of the device primitives (tests only)
 
 Every call compiles (no staleness check): the build is the "does it build" check.
 """
@@ -51,6 +52,13 @@ def build_gpu():
     _run([HIPCC] + HIP_FLAGS + srcs + ["-o", os.path.join(LIB, "libpxgpu.so")])
 
 
+def build_selftest():
+    """Self-test kernels of the shared device primitives (tests/native), for the GPU tests."""
+    src = os.path.join(REPO, "tests", "native", "primitives_selftest.hip")
+    if os.path.exists(src):
+        _run([HIPCC] + HIP_FLAGS + [src, "-o", os.path.join(REPO, "tests", "native", "libpxprim_selftest.so")])
+
+
 def build_framework():
     """The job driver with the reference's module interface, linked against the C ABI."""
     srcs = sorted(glob.glob(os.path.join(PKG, "framework", "*.cc")))
@@ -65,6 +73,7 @@ def build_all():
     build_host()
     build_gpu()
     build_framework()
+    build_selftest()
 
 
 if __name__ == "__main__":

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "primitives.h"
 #include "pxgpu_internal.h"
 
 namespace pxg {
@@ -76,59 +77,6 @@ namespace pxg {
     const float y = c * y2 + beta;
     const float dca0 = sqrtf(x * x + y * y) - 1.f;
     return fabsf(dca0) < cut * fabsf(c);
-  }
-
-  // exclusive scan over a workgroup of NT threads (wave64); returns the
-  // thread's exclusive prefix and the workgroup total.  lds: NT / 64 words.
-  template <int NT>
-  __device__ __forceinline__ uint32_t wgExclusiveScan(uint32_t v, uint32_t& total, uint32_t* lds) {
-    static_assert(NT % 64 == 0 && NT <= 4096, "workgroup size");
-    constexpr int kWaves = NT / 64;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t t = __shfl_up(incl, o);
-      if (lane >= o)
-        incl += t;
-    }
-    if (lane == 63)
-      lds[wid] = incl;
-    __syncthreads();
-    if (wid == 0) {
-      uint32_t w = lane < kWaves ? lds[lane] : 0;
-#pragma unroll
-      for (int o = 1; o < kWaves; o <<= 1) {
-        const uint32_t t = __shfl_up(w, o);
-        if (lane >= o)
-          w += t;
-      }
-      if (lane < kWaves)
-        lds[lane] = w;
-    }
-    __syncthreads();
-    const uint32_t before = wid ? lds[wid - 1] : 0;
-    total = lds[kWaves - 1];
-    __syncthreads();
-    return before + incl - v;
-  }
-
-  // in-place exclusive scan of a[0, n) by one workgroup of NT threads; a[n] = total.
-  template <int NT>
-  __device__ uint32_t wgScanArray(uint32_t* a, uint32_t n, uint32_t* lds) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += NT) {
-      const uint32_t i = base + threadIdx.x;
-      const uint32_t v = i < n ? a[i] : 0;
-      uint32_t tot;
-      const uint32_t ex = wgExclusiveScan<NT>(v, tot, lds);
-      if (i < n)
-        a[i] = carry + ex;
-      carry += tot;
-    }
-    if (threadIdx.x == 0)
-      a[n] = carry;
-    return carry;
   }
 
   // layer starts and the cumulative inner-hit counts of the pairs of event e

@@ -51,41 +51,12 @@ namespace pxg {
     __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
     __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
-    __device__ __forceinline__ uint32_t laneId() { return threadIdx.x & 63; }
-    __device__ __forceinline__ uint64_t lanesBelow() { return (1ull << laneId()) - 1; }
-    // Counter allocation with one atomic per wave: the active lanes asking for
-    // a slot get consecutive values (lane order).  Lanes that diverged away do
-    // not take part, so it may be called in divergent code.
     // increments the 16-bit counter i of a 4-byte aligned array through the 32-bit word
     // holding it and returns its old value (no carry: the counts stay below 65536)
     __device__ __forceinline__ uint32_t atomicIncHalf(uint16_t* base, uint32_t i) {
       uint32_t* w = reinterpret_cast<uint32_t*>(base) + (i >> 1);
       const uint32_t sh = (i & 1) * 16;
       return (atomicAdd(w, 1u << sh) >> sh) & 0xffffu;
-    }
-
-    __device__ __forceinline__ uint32_t waveAlloc(uint32_t* counter, bool need) {
-      const uint64_t m = __ballot(need);
-      if (!m)
-        return 0;
-      const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-      uint32_t base = 0;
-      if (laneId() == leader)
-        base = atomicAdd(counter, uint32_t(__popcll(m)));
-      base = __shfl(base, leader);
-      return base + __popcll(m & lanesBelow());
-    }
-
-    // sum over the active lanes, added to *counter once per wave
-    __device__ __forceinline__ void waveAdd(uint32_t* counter, uint32_t v) {
-      const uint64_t m = __ballot(v != 0);
-      if (!m)
-        return;
-      uint32_t sum = v;
-      for (int o = 32; o > 0; o >>= 1)
-        sum += __shfl_xor(sum, o);
-      if (laneId() == __ffsll((unsigned long long)m) - 1)
-        atomicAdd(counter, sum);
     }
 
     // wave index within the event's workgroups, uniform over the wave
@@ -443,22 +414,6 @@ namespace pxg {
     constexpr uint32_t kStage = 256;  // staged candidate cells per wave
     constexpr uint32_t kSegs = 64;    // staged candidate ranges per wave
 
-    __device__ __forceinline__ uint32_t waveExclusiveSum(uint32_t v) {
-      uint32_t x = v;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (laneId() >= uint32_t(o))
-          x += y;
-      }
-      return x - v;
-    }
-
-    __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
-      for (int o = 32; o > 0; o >>= 1)
-        v = max(v, uint32_t(__shfl_xor(v, o)));
-      return v;
-    }
-
     __global__ void __launch_bounds__(kCaBlock, 7) connectKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
@@ -525,8 +480,8 @@ namespace pxg {
         // groups: consecutive lanes with the same outer hit; the first lane leads
         const uint32_t hmPrev = __shfl_up(hm, 1);
         const bool leader = valid && (lane == 0 || hmPrev != hm);
-        const uint32_t segOff = waveExclusiveSum(leader ? nseg : 0);
-        const uint32_t candOff = waveExclusiveSum(leader ? total : 0);
+        const uint32_t segOff = waveExclusiveScan(leader ? nseg : 0);
+        const uint32_t candOff = waveExclusiveScan(leader ? total : 0);
         const bool fits = leader && segOff + nseg <= kSegs && candOff + total <= kStage;
         if (fits) {
           uint32_t sg = segOff, base = candOff;

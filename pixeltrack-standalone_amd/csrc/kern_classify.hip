@@ -322,18 +322,24 @@ namespace pxg {
       }
     }
 
-    // a22 hit -> tuple association of the loose tuples
+    // a22 hit -> tuple association of the loose tuples (HitToTuple, CAConstants.h:65): a
+    // one-to-many container over the event's hits, counted, scanned and filled
+    __device__ __forceinline__ OneToMany hitToTuple(const CAPool& ca, uint32_t e, uint32_t nHits) {
+      return {ca.h2tCount + size_t(e) * kHitStride, ca.h2tOff + size_t(e) * kHitStride, nHits,
+              ca.caps.maxHitToTuple};
+    }
+
     __global__ void __launch_bounds__(kCaBlock) hitToTupleCountKernel(CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
-      uint32_t* cnt = ca.h2tCount + size_t(e) * kHitStride;
+      const OneToMany h2t = hitToTuple(ca, e, 0);
       for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
         if (ca.quality[tb + t] != kQualityLoose)
           continue;
         for (uint32_t k = off[t]; k < off[t + 1]; ++k)
-          atomicAdd(&cnt[ca.tupleBins[5 * tb + k]], 1u);
+          h2t.countEntry(ca.tupleBins[5 * tb + k]);
       }
     }
 
@@ -341,26 +347,10 @@ namespace pxg {
       const uint32_t e = blockIdx.x;
       __shared__ uint32_t lds[kEventBlock / 64];
       const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
-      uint32_t* cnt = ca.h2tCount + size_t(e) * kHitStride;
-      uint32_t* off = ca.h2tOff + size_t(e) * kHitStride;
-      const uint32_t cap = ca.caps.maxHitToTuple;
-      uint32_t carry = 0;
-      for (uint32_t base = 0; base < nHits; base += kEventBlock) {
-        const uint32_t h = base + threadIdx.x;
-        const uint32_t v = h < nHits ? cnt[h] : 0;
-        uint32_t tot;
-        const uint32_t ex = wgExclusiveScan<kEventBlock>(v, tot, lds);
-        if (h < nHits) {
-          off[h] = min(carry + ex, cap);
-          cnt[h] = 0;
-        }
-        carry += tot;
-      }
-      if (threadIdx.x == 0) {
-        off[nHits] = min(carry, cap);
-        if (carry > cap)
-          atomicOr(&ca.cnt[e].status, kCaHitToTupleOverflow);
-      }
+      const OneToMany h2t = hitToTuple(ca, e, nHits);
+      const uint32_t total = h2t.scan<kEventBlock>(lds);
+      if (threadIdx.x == 0 && total > h2t.capacity)
+        atomicOr(&ca.cnt[e].status, kCaHitToTupleOverflow);
     }
 
     __global__ void __launch_bounds__(kCaBlock) hitToTupleFillKernel(CAPool ca) {
@@ -368,18 +358,13 @@ namespace pxg {
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
       const uint32_t* off = ca.tupleOff + ca.offBase(e);
-      uint32_t* cur = ca.h2tCount + size_t(e) * kHitStride;
-      const uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
+      const OneToMany h2t = hitToTuple(ca, e, 0);
       TupleIndex* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
       for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
         if (ca.quality[tb + t] != kQualityLoose)
           continue;
-        for (uint32_t k = off[t]; k < off[t + 1]; ++k) {
-          const uint32_t h = ca.tupleBins[5 * tb + k];
-          const uint32_t pos = hoff[h] + atomicAdd(&cur[h], 1u);
-          if (pos < hoff[h + 1])
-            bins[pos] = t;
-        }
+        for (uint32_t k = off[t]; k < off[t + 1]; ++k)
+          h2t.fill(ca.tupleBins[5 * tb + k], TupleIndex(t), bins);
       }
     }
 

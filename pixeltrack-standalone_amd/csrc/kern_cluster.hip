@@ -24,6 +24,7 @@
 //   clusStartKernel     one workgroup per event: the clamped prefix over the
 //                       2000 cluster counts (SiPixelRawToClusterGPUKernel.cc:
 //                       464-506).
+#include "primitives.h"
 #include "pxgpu_internal.h"
 
 namespace pxg {
@@ -34,31 +35,6 @@ namespace pxg {
     constexpr uint32_t kNoPixel = 0xffffffffu;
     constexpr int kCols = kColsPerModule;  // 416
     constexpr int kBigWorkgroups = 768;  // persistent workgroups for the large module runs (3 per CU by LDS)
-
-    // exclusive workgroup scan of one 32-bit value per thread; returns the total
-    __device__ __forceinline__ uint32_t scanExclusive(uint32_t v, uint32_t& excl, uint32_t* wsum) {
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      uint32_t incl = v;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o);
-        if (lane >= o)
-          incl += t;
-      }
-      if (lane == 63)
-        wsum[wave] = incl;
-      __syncthreads();
-      uint32_t before = 0, total = 0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        uint32_t s = wsum[w];
-        before += (w < wave) ? s : 0;
-        total += s;
-      }
-      excl = before + incl - v;
-      __syncthreads();
-      return total;
-    }
 
     __device__ __forceinline__ uint32_t findRoot(volatile uint32_t* parent, uint32_t x) {
       uint32_t p = parent[x];
@@ -150,14 +126,7 @@ namespace pxg {
         v[j] = c <= kCols ? colStart[c] : 0;
         s += v[j];
       }
-      uint32_t incl = s;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o);
-        if (lane >= o)
-          incl += t;
-      }
-      uint32_t run = incl - s;
+      uint32_t run = waveExclusiveScan(s);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
         const int c = 1 + lane * 7 + j;
@@ -213,8 +182,8 @@ namespace pxg {
     for (uint32_t c0 = 0; c0 < n; c0 += kBlock) {
       const uint32_t i = c0 + threadIdx.x;
       const bool root = i < n && sxy[i] != kNoPixel && parent[i] == i;
-      uint32_t excl;
-      const uint32_t tot = scanExclusive(root ? 1u : 0u, excl, wsum);
+      uint32_t tot;
+      const uint32_t excl = wgExclusiveScan<kBlock>(root ? 1u : 0u, tot, wsum);
       if (root)
         work[i] = nclus + excl;  // rank of the root
       nclus += tot;
@@ -249,8 +218,8 @@ namespace pxg {
     for (uint32_t c0 = 0; c0 < ncut; c0 += kBlock) {
       const uint32_t c = c0 + threadIdx.x;
       const uint32_t ok = (c < ncut && charge[c] > cut) ? 1u : 0u;
-      uint32_t excl;
-      const uint32_t tot = scanExclusive(ok, excl, wsum);
+      uint32_t tot;
+      const uint32_t excl = wgExclusiveScan<kBlock>(ok, tot, wsum);
       if (c < ncut)
         newId[c] = ok ? (kept + excl) : 9999u;
       kept += tot;
@@ -453,7 +422,6 @@ namespace pxg {
   __global__ void __launch_bounds__(1024) clusStartKernel(Pool p) {
     __shared__ uint32_t wsum[16];
     const uint32_t e = blockIdx.x;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t* cim = p.clusInModule + size_t(e) * kMaxModuleRanges;
     uint32_t* cms = p.clusModuleStart + size_t(e) * kMaxModuleRanges;
     uint32_t run = 0;
@@ -461,25 +429,10 @@ namespace pxg {
       const uint32_t m = c0 + threadIdx.x;
       uint32_t v = m < kMaxNumModules ? cim[m] : 0;
       v = v < kMaxClustersPerModule ? v : kMaxClustersPerModule;
-      uint32_t incl = v;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o);
-        if (int(lane) >= o)
-          incl += t;
-      }
-      if (lane == 63)
-        wsum[wave] = incl;
-      __syncthreads();
-      uint32_t before = 0, tot = 0;
-      for (uint32_t w = 0; w < 16; ++w) {
-        const uint32_t sw = wsum[w];
-        before += w < wave ? sw : 0;
-        tot += sw;
-      }
-      __syncthreads();
+      uint32_t tot;
+      const uint32_t excl = wgExclusiveScan<1024>(v, tot, wsum);
       if (m < kMaxNumModules) {
-        const uint32_t sum = run + before + incl;
+        const uint32_t sum = run + excl + v;
         cms[m + 1] = sum < kMaxNumClusters ? sum : kMaxNumClusters;
       }
       run += tot;

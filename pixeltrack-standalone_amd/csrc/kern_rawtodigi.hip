@@ -17,6 +17,7 @@
 // gpuCalibPixel.h:22-66 (calibration), gpuClustering.h:19-37 (module starts:
 // a valid digi starts a module when the previous valid digi of the event has a
 // different module id).
+#include "primitives.h"
 #include "pxgpu_internal.h"
 
 namespace pxg {
@@ -281,44 +282,10 @@ namespace pxg {
   // ---------------------------------------------------------------------------
   namespace {
     constexpr int kScanBlock = 1024;
-
-    // exclusive scan of a 64-bit value over the workgroup; returns the total
-    __device__ __forceinline__ uint64_t blockExclusiveScan(uint64_t v, uint64_t& excl, uint64_t* lds) {
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      uint64_t incl = v;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint64_t t = __shfl_up(incl, o);
-        if (lane >= o)
-          incl += t;
-      }
-      if (lane == 63)
-        lds[wave] = incl;
-      __syncthreads();
-      if (wave == 0) {
-        uint64_t s = lane < (kScanBlock / 64) ? lds[lane] : 0;
-        uint64_t si = s;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-          uint64_t t = __shfl_up(si, o);
-          if (lane >= o)
-            si += t;
-        }
-        if (lane < (kScanBlock / 64))
-          lds[16 + lane] = si - s;
-        if (lane == 15)
-          lds[32] = si;
-      }
-      __syncthreads();
-      excl = lds[16 + wave] + incl - v;
-      uint64_t total = lds[32];
-      __syncthreads();
-      return total;
-    }
   }  // namespace
 
   __global__ void __launch_bounds__(kScanBlock) moduleStartKernel(Pool p) {
-    __shared__ uint64_t lds[40];
+    __shared__ uint64_t lds[kScanBlock / 64];
     const uint32_t e = blockIdx.x;
     const uint32_t wordOffset = p.events[e].wordOffset;
     const uint32_t nWords = p.events[e].nWords;
@@ -344,8 +311,8 @@ namespace pxg {
         em = p.boundaryMask[2 * (wordOffset / 64 + k) + 1];
       }
       const uint64_t packed = uint64_t(__popcll(fm)) | (uint64_t(__popcll(em)) << 32);
-      uint64_t excl;
-      const uint64_t total = blockExclusiveScan(packed, excl, lds);
+      uint64_t total;
+      const uint64_t excl = wgExclusiveScan<kScanBlock>(packed, total, lds);
       uint32_t r = baseMod + uint32_t(excl & 0xffffffffu);
       while (fm) {
         const int b = __ffsll((unsigned long long)fm) - 1;

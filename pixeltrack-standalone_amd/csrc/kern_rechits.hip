@@ -26,6 +26,7 @@
 // CondFormats/pixelCPEforGPU.h:106-340, DataFormats/SOARotation.h:55-94,
 // DataFormats/approx_atan2.h:230-263, PixelRecHits.cc:15-48,
 // CUDACore/HistoContainer.h:18-73.
+#include "primitives.h"
 #include "pxgpu_internal.h"
 
 namespace pxg {
@@ -319,21 +320,8 @@ namespace pxg {
       const uint32_t b0 = 2 * threadIdx.x;
       const uint32_t v0 = cnt[b0], v1 = cnt[b0 + 1];
       const uint32_t s = v0 + v1;
-      uint32_t incl = s;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o);
-        if (int(lane) >= o)
-          incl += t;
-      }
-      if (lane == 63)
-        wsum[layer] = incl;
-      __syncthreads();
-      uint32_t before = 0;
-      for (uint32_t w = 0; w < layer; ++w)
-        before += wsum[w];
-      const uint32_t excl = before + incl - s;
-      __syncthreads();
+      uint32_t tot;
+      const uint32_t excl = wgExclusiveScan<64 * kPhiWaves>(s, tot, wsum);
       cnt[b0] = excl + v0;       // inclusive: end of bin b0
       cnt[b0 + 1] = excl + s;    // end of bin b0 + 1
       // bin 1280 has no entries: its end is the total
