@@ -177,6 +177,25 @@ int pxg_raw_to_digi(pxg_stream* s, const uint32_t* words, const uint8_t* fedIds,
 int pxg_raw_to_cluster(pxg_stream* s, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords,
                        pxg_digis* digis, pxg_clusters* clusters, pxg_errors* errors);
 
+/* No counterpart in the reference: the same products as pxg_raw_to_cluster for one event,
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc).  It allocates nothing: the inputs and outputs
+ * are the caller's device arrays, and the kernels' scratch (event descriptor and counters,
+ * module-boundary and error masks, error records at their word index, the clusteriser's
+ * work list: PXG_NOALLOC_SCRATCH_BYTES) is a static device array that each GPU holds once
+ * until the process ends.  words (n_words u32) and fed_ids (n_words / 2 u8) are device
+ * arrays; digis->xx, yy, adc, moduleInd, clus, pdigi, rawIdArr hold n_words entries each,
+ * clusters->moduleStart, clusInModule, moduleId, clusModuleStart 2001 each, errors->data
+ * n_words PixelErrorCompact records; the counts (nModules, nDigis, nClusters, size) are
+ * written back.  hip_stream is the hipStream_t to run on (0: the null stream); the call
+ * returns after that stream has finished.  n_words above PXG_NOALLOC_MAX_WORDS is rejected
+ * with PXG_ERR_CAPACITY.  Calls that overlap on one GPU share its scratch array and race:
+ * serialise them.  Returns the status of pxg_raw_to_cluster. */
+#define PXG_NOALLOC_MAX_WORDS 300000u /* the words of one event at most (150 FEDs x 2000) */
+#define PXG_NOALLOC_SCRATCH_BYTES 3684352u
+int pxg_raw_to_cluster_noalloc(pxg_context* ctx, void* hip_stream, const uint32_t* words, const uint8_t* fed_ids,
+                               uint32_t n_words, pxg_digis* digis, pxg_clusters* clusters, pxg_errors* errors);
+
 /* Digis + clusters -> rec-hits (PixelRecHitGPUKernel::makeHits, PixelRecHits.cc:33-53); the
  * inputs must be the products of the last pxg_raw_to_cluster on the same stream. */
 int pxg_rechits(pxg_stream* s, const pxg_digis* digis, const pxg_clusters* clusters, const pxg_beamspot* bs,

@@ -521,6 +521,11 @@ int pxg_init(const pxg_conditions* c, pxg_context** out) {
                c->gain_data_bytes,
                static_cast<const CPECommonParams*>(com),      static_cast<const CPEDetParams*>(det),
                static_cast<const CPELayerGeometry*>(lay),     static_cast<const AverageGeometry*>(avg)};
+  const int rc = noallocSetup(ctx->device);
+  if (rc) {
+    pxg_finalize(ctx);
+    return rc;
+  }
   *out = ctx;
   return 0;
 }

@@ -77,6 +77,9 @@ namespace pxg::detail {
   size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps);
   // destroys the stream's captured stage sequences
   void dropGraphs(pxg_stream* s);
+  // looks up the device's address of the no-alloc entry point's scratch array (once per
+  // device; the device must be current)
+  int noallocSetup(int device);
 }  // namespace pxg::detail
 
 // returns PXG_ERR_HIP (with the message for pxg_last_error) from the enclosing function

@@ -106,6 +106,8 @@ def _gpu():
         lib.pxg_pool_bytes.restype = C.c_size_t
         lib.pxg_pool_bytes.argtypes = [u32, u32, C.c_int]
         lib.pxg_profile_capacities.argtypes = [C.c_int, C.POINTER(u32 * 6)]
+        lib.pxg_raw_to_cluster_noalloc.argtypes = [vp, vp, vp, vp, u32, C.POINTER(Digis), C.POINTER(Clusters),
+                                                   C.POINTER(Errors)]
         lib.pxg_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
         lib.pxg_server_destroy.argtypes = [vp]
         lib.pxg_server_submit.argtypes = [vp, vp, vp, u32, C.POINTER(BeamSpot), C.POINTER(C.c_uint64)]
@@ -172,6 +174,11 @@ def _check(rc, what):
     if rc < 0:
         raise RuntimeError(f"{what}: {_gpu().pxg_last_error().decode()}")
     return rc
+
+
+def last_error():
+    """The message of the library's last failed call (pxg_last_error)."""
+    return _gpu().pxg_last_error().decode()
 
 
 def device_name():
