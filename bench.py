@@ -127,9 +127,11 @@ def gen_events(detector, cfg, n, threads, first=0):
         return list(ex.map(lambda i: detector.event(cfg, i)[0], range(first, first + n)))
 
 
-def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
+def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None, pinned=None):
     """Submits `inputs` with Poisson arrivals at `rate` events/s (all at once if rate is None)
-    to a serving loop and returns throughput and latency percentiles (ms)."""
+    to a serving loop and returns throughput and latency percentiles (ms).  With `pinned`
+    (the same events in mapped pinned buffers) the events are handed over without a host
+    copy (pxg_server_submit_pinned), else copied at submission (pxg_server_submit)."""
     from pxtrack import gpu
     sv = gpu.Server(ctx, **server_kw)
     rng = np.random.default_rng(seed)
@@ -145,7 +147,7 @@ def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
                 break
             if d > 2e-4:
                 time.sleep(d - 1e-4)
-        ids[sv.submit(w, f)] = i
+        ids[sv.submit_pinned(pinned[i]) if pinned is not None else sv.submit(w, f)] = i
     res = []
     deadline = time.time() + 120
     while len(res) < len(ids) and time.time() < deadline:
@@ -156,7 +158,8 @@ def serve_at(ctx, inputs, rate, server_kw, seed=1, classes=None):
         return {"rate": rate, "error": f"{len(res)} of {len(ids)} completed"}
     lat = np.array([(r["t_done_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])
     span = (max(r["t_done_ns"] for r in res) - min(r["t_submit_ns"] for r in res)) * 1e-9
-    out = {"rate": rate, "events": len(res), "events_per_s": round(len(res) / span, 1),
+    out = {"rate": rate, "submit": "pinned" if pinned is not None else "copy", "events": len(res),
+           "events_per_s": round(len(res) / span, 1),
            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
            "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 1),
@@ -183,11 +186,20 @@ def serving_sweep(ctx, det, cond, args, tuned):
     batched_kw = dict(max_batch=args.batch, n_streams=3, hbm_budget=budget, upto=args.upto, transfer=False, **shape)
     plain_kw = dict(max_batch=1, n_streams=1, upto=args.upto, transfer=False, **shape)
     rates = [1000, 3000, 10000, 30000, None]
+    # the events as a front end holding them in mapped pinned buffers hands them over: the
+    # device reads them while it loads each batch (PCIe-inclusive, no host copy)
+    pev = [gpu.PinnedEvent(w, f) for w, f in ev]
     out = {"workload": f"{len(ev)} events, pileup Poisson(60) clipped to [10, 200], Poisson arrivals",
            "hbm_budget_gb": args.serve_budget_gb, "batched": [], "plain": []}
     for r in rates:
-        out["batched"].append(serve_at(ctx, ev, r, batched_kw))
+        out["batched"].append(serve_at(ctx, ev, r, batched_kw, pinned=pev))
         print(f"[bench] serving batched rate {r}: {out['batched'][-1]}", file=sys.stderr, flush=True)
+    # the same at saturation with a host copy of every event at submission
+    out["batched_copy_submit"] = serve_at(ctx, ev, None, batched_kw)
+    print(f"[bench] serving batched, copied at submission: {out['batched_copy_submit']}", file=sys.stderr,
+          flush=True)
+    for pe in pev:
+        pe.close()
     for r in rates[:2] + [None]:
         sub = ev[: min(len(ev), 300)]
         out["plain"].append(serve_at(ctx, sub, r, plain_kw))
@@ -200,10 +212,13 @@ def serving_sweep(ctx, det, cond, args, tuned):
           [(gpu.unpack_raw(r, cond["fed_ids"]), "pu200") for r in r200]
     order = np.random.default_rng(7).permutation(len(mix))
     mix = [mix[i] for i in order]
+    pmix = [gpu.PinnedEvent(*m[0]) for m in mix]
     mix_kw = dict(max_batch=args.batch // 2, n_streams=3, hbm_budget=budget, upto=args.upto,
                   profile=gpu.PROFILE_HIGH_OCCUPANCY, transfer=False, **shape)
-    out["mix_pu50_pu200"] = [serve_at(ctx, [m[0] for m in mix], r, mix_kw, classes=[m[1] for m in mix])
-                             for r in (2000, None)]
+    out["mix_pu50_pu200"] = [serve_at(ctx, [m[0] for m in mix], r, mix_kw, classes=[m[1] for m in mix],
+                                      pinned=pmix) for r in (2000, None)]
+    for pe in pmix:
+        pe.close()
     print(f"[bench] serving mix: {out['mix_pu50_pu200']}", file=sys.stderr, flush=True)
     return out
 

@@ -287,8 +287,19 @@ typedef struct {
 
 int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server** out);
 void pxg_server_destroy(pxg_server* sv); /* finishes the events already submitted */
+/* Copies the caller's words and FED ids into one of the server's pinned chunks and queues the
+ * event (the caller may reuse its buffers on return; waits for a free chunk when all hold
+ * queued events). */
 int pxg_server_submit(pxg_server* sv, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords,
                       const pxg_beamspot* bs, uint64_t* id);
+/* Queues the event without a host copy: words and fedIds must be mapped pinned host memory
+ * (pxg_host_alloc), which the device reads while it loads the batch; the caller leaves them
+ * unchanged until the event is polled.  PXG_ERR_ARG for other memory. */
+int pxg_server_submit_pinned(pxg_server* sv, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords,
+                             const pxg_beamspot* bs, uint64_t* id);
+/* Mapped pinned host memory (readable by the device) for pxg_server_submit_pinned. */
+int pxg_host_alloc(size_t bytes, void** out);
+void pxg_host_free(void* p);
 /* Completed events (up to max), waiting up to wait_ms for the first; returns the number. */
 int pxg_server_poll(pxg_server* sv, pxg_server_result* out, int max, int wait_ms);
 /* Packed products of a polled event, valid until pxg_server_release. */

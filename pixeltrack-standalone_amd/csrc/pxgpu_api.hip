@@ -429,41 +429,91 @@ void pxg::detail::dropGraphs(pxg_stream* s) {
   s->graphs.clear();
 }
 
+namespace {
+  // Validates a batch and writes its descriptors (256-word aligned offsets) to the
+  // stream's pinned descriptor array.
+  int describeEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs,
+                     uint32_t nBeamspots) {
+    if (!s || !raws || !bs || n == 0 || (nBeamspots != 1 && nBeamspots != n))
+      return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
+    dropGraphs(s);
+    if (n > s->pool.maxEvents)
+      return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
+    uint32_t off = 0, blocks = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+      if (raws[e].nWords % 2)
+        return fail(PXG_ERR_ARG, "event %u: odd number of words %u", e, raws[e].nWords);
+      if (raws[e].nWords > kMaxFedWords)
+        return fail(PXG_ERR_CAPACITY, "event %u: %u words > %u", e, raws[e].nWords, kMaxFedWords);
+      if (uint64_t(off) + raws[e].nWords > s->wordCapacity)
+        return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u words exceed the pool", off + raws[e].nWords);
+      EventDesc& d = s->hostEvents[e];
+      d.wordOffset = off;
+      d.nWords = raws[e].nWords;
+      d.blockOffset = blocks;
+      d.pad = 0;
+      std::memcpy(&d.bs, bs + (nBeamspots == 1 ? 0 : e), sizeof(BeamSpotPOD));
+      blocks += (raws[e].nWords + kDecodeBlock - 1) / kDecodeBlock;
+      off = alignUp(off + raws[e].nWords, kWordAlign);
+    }
+    return 0;
+  }
+
+  void markLoaded(pxg_stream* s, uint32_t n) {
+    s->nLoaded = n;
+    std::fill(s->stagesRun.begin(), s->stagesRun.end(), uint8_t(0));
+  }
+
+  // event blockIdx.y of the table: its words and FED ids from host memory into the pool
+  __global__ void __launch_bounds__(256) gatherKernel(const GatherSrc* table, uint32_t* words, uint8_t* fedIds) {
+    const GatherSrc g = table[blockIdx.y];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nWords; i += stride)
+      words[g.wordOffset + i] = g.words[i];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nWords / 2; i += stride)
+      fedIds[g.wordOffset / 2 + i] = g.fedIds[i];
+  }
+}  // namespace
+
 int pxg::detail::loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs,
                              uint32_t nBeamspots, bool sync) {
-  if (!s || !raws || !bs || n == 0 || (nBeamspots != 1 && nBeamspots != n))
-    return fail(PXG_ERR_ARG, "pxg_load_events: bad arguments");
-  dropGraphs(s);
-  if (n > s->pool.maxEvents)
-    return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u events > capacity %u", n, s->pool.maxEvents);
-  uint32_t off = 0, blocks = 0;
+  int rc = describeEvents(s, n, raws, bs, nBeamspots);
+  if (rc)
+    return rc;
   for (uint32_t e = 0; e < n; ++e) {
-    if (raws[e].nWords % 2)
-      return fail(PXG_ERR_ARG, "event %u: odd number of words %u", e, raws[e].nWords);
-    if (raws[e].nWords > kMaxFedWords)
-      return fail(PXG_ERR_CAPACITY, "event %u: %u words > %u", e, raws[e].nWords, kMaxFedWords);
-    if (uint64_t(off) + raws[e].nWords > s->wordCapacity)
-      return fail(PXG_ERR_CAPACITY, "pxg_load_events: %u words exceed the pool", off + raws[e].nWords);
-    EventDesc& d = s->hostEvents[e];
-    d.wordOffset = off;
-    d.nWords = raws[e].nWords;
-    d.blockOffset = blocks;
-    d.pad = 0;
-    std::memcpy(&d.bs, bs + (nBeamspots == 1 ? 0 : e), sizeof(BeamSpotPOD));
+    const EventDesc& d = s->hostEvents[e];
     if (raws[e].nWords) {
-      PXG_HIP(hipMemcpyAsync(s->pool.words + off, raws[e].words, size_t(raws[e].nWords) * 4, hipMemcpyHostToDevice,
-                             s->stream));
-      PXG_HIP(hipMemcpyAsync(s->pool.fedIds + off / 2, raws[e].fedIds, raws[e].nWords / 2, hipMemcpyHostToDevice,
-                             s->stream));
+      PXG_HIP(hipMemcpyAsync(s->pool.words + d.wordOffset, raws[e].words, size_t(raws[e].nWords) * 4,
+                             hipMemcpyHostToDevice, s->stream));
+      PXG_HIP(hipMemcpyAsync(s->pool.fedIds + d.wordOffset / 2, raws[e].fedIds, raws[e].nWords / 2,
+                             hipMemcpyHostToDevice, s->stream));
     }
-    blocks += (raws[e].nWords + kDecodeBlock - 1) / kDecodeBlock;
-    off = alignUp(off + raws[e].nWords, kWordAlign);
   }
   PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, n * sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
   if (sync)
     PXG_HIP(hipStreamSynchronize(s->stream));
-  s->nLoaded = n;
-  std::fill(s->stagesRun.begin(), s->stagesRun.end(), uint8_t(0));
+  markLoaded(s, n);
+  return 0;
+}
+
+int pxg::detail::loadEventsGather(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs,
+                                   GatherSrc* table, const GatherSrc* tableDev) {
+  int rc = describeEvents(s, n, raws, bs, n);
+  if (rc)
+    return rc;
+  uint32_t maxWords = 0;
+  for (uint32_t e = 0; e < n; ++e) {
+    table[e] = {raws[e].words, raws[e].fedIds, raws[e].nWords, s->hostEvents[e].wordOffset};
+    maxWords = std::max(maxWords, raws[e].nWords);
+  }
+  PXG_HIP(hipMemcpyAsync(s->pool.events, s->hostEvents, n * sizeof(EventDesc), hipMemcpyHostToDevice, s->stream));
+  // about 16 words per thread for the largest event
+  const uint32_t blocksPerEvent = std::max(1u, std::min(256u, (maxWords + 4095) / 4096));
+  gatherKernel<<<dim3(blocksPerEvent, n), 256, 0, s->stream>>>(tableDev, s->pool.words, s->pool.fedIds);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(PXG_ERR_HIP, "launch of the event gather failed: %s", hipGetErrorString(e));
+  markLoaded(s, n);
   return 0;
 }
 

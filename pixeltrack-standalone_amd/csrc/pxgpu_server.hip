@@ -5,8 +5,11 @@
 // pxg_stream with a device pool for one batch.  Whenever a slot is free and
 // requests wait, it admits the oldest waiting requests into a batch while the
 // batch's event count and raw words fit the slot's pool (the pool sizes come
-// from the HBM budget), copies them in, and enqueues the whole chain as one
-// launch per stage.  Slots run concurrently on their own HIP streams, so a
+// from the HBM budget), and enqueues the load and the whole chain as one launch
+// per stage.  The load is one kernel that reads every request's words from
+// mapped pinned host memory: the library's chunks (pxg_server_submit copies the
+// caller's words there) or the caller's own pinned buffers
+// (pxg_server_submit_pinned, no host copy at all).  Slots run concurrently on their own HIP streams, so a
 // batch formed while others are on the device overlaps with them.  When a
 // batch's counters are back, the per-event products are packed on the device
 // into one buffer and copied to the host in one transfer; the requests are
@@ -111,14 +114,22 @@ namespace {
     }
   }
 
-  // Pinned host chunks holding one event's words and FED ids, so that the
-  // device copies read them directly (sized for the largest event).
+  // Mapped pinned host chunks holding one event's words and FED ids, which the
+  // load kernel reads directly (sized for the largest event).
   constexpr size_t kChunkWords = kMaxFedWords;
   constexpr size_t kChunkBytes = kChunkWords * 4 + kChunkWords / 2 + 256;
+  constexpr unsigned kMappedPinned = hipHostMallocMapped | hipHostMallocPortable;
+
+  struct Chunk {
+    uint8_t* host = nullptr;       // words at host, FED ids at host + 4 * kChunkWords
+    const uint8_t* dev = nullptr;  // the same memory as the device addresses it
+  };
 
   struct Request {
     uint64_t id;
-    uint8_t* chunk = nullptr;  // words at chunk, FED ids at chunk + 4 * kChunkWords
+    Chunk chunk;                         // the library's chunk, or none for caller buffers
+    const uint32_t* words = nullptr;     // device addresses of the words and FED ids
+    const uint8_t* fedIds = nullptr;
     uint32_t nWords = 0;
     pxg_beamspot bs;
     uint32_t counts[4] = {0, 0, 0, 0};  // words, hits, tuples, vertices
@@ -141,13 +152,15 @@ namespace {
     size_t packCapacity = 0;
     uint32_t packBytes = 0;
     hipEvent_t evRun = nullptr, evPack = nullptr;
+    pxg::detail::GatherSrc* gather = nullptr;           // mapped pinned, one entry per event
+    const pxg::detail::GatherSrc* gatherDev = nullptr;  // its device address
   };
 }  // namespace
 
 struct pxg_server {
   pxg_server_config cfg{};
   pxg_context* ctx = nullptr;
-  std::vector<uint8_t*> freeChunks, allChunks;  // guarded by mu
+  std::vector<Chunk> freeChunks, allChunks;  // guarded by mu
   uint32_t maxBatchWords = 0;
   std::vector<Slot> slots;
   std::mutex mu;
@@ -175,12 +188,12 @@ int pxg_server::launch(Slot& sl) {
   const uint64_t t = nowNs();
   for (uint32_t e = 0; e < n; ++e) {
     Request& r = *sl.batch[e];
-    views[e] = {reinterpret_cast<const uint32_t*>(r.chunk), r.chunk + 4 * kChunkWords, r.nWords};
+    views[e] = {r.words, r.fedIds, r.nWords};
     beamspots[e] = r.bs;
     r.tLaunch = t;
     r.batchSize = n;
   }
-  int rc = pxg::detail::loadEvents(sl.s, n, views.data(), beamspots.data(), n, false);
+  int rc = pxg::detail::loadEventsGather(sl.s, n, views.data(), beamspots.data(), sl.gather, sl.gatherDev);
   if (!rc)
     rc = pxg::detail::runStages(sl.s, 0, n, 1, cfg.upto);
   if (rc)
@@ -235,8 +248,11 @@ void pxg_server::finish(Slot& sl) {
     r->counts[1] = s->hostCounts[e].nClusters;
     r->counts[2] = cfg.upto >= 4 ? s->hostCA[e].nTuples : 0;
     r->counts[3] = cfg.upto >= 5 ? s->hostCA[e].nVertices : 0;
-    freeChunks.push_back(r->chunk);
-    r->chunk = nullptr;
+    if (r->chunk.host)
+      freeChunks.push_back(r->chunk);
+    r->chunk = {};
+    r->words = nullptr;
+    r->fedIds = nullptr;
     // counters of the stages the server runs only (the others hold an earlier batch's)
     const uint32_t caBits = cfg.upto >= 4 ? s->hostCA[e].status << PXG_STATUS_CA_SHIFT : 0u;
     const uint32_t vBits = cfg.upto >= 5 && s->hostCA[e].vstatus ? uint32_t(PXG_STATUS_VERTEX_OVERFLOW) : 0u;
@@ -352,6 +368,11 @@ int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server
         e = hipEventCreateWithFlags(&sl.evRun, hipEventDisableTiming);
       if (!e)
         e = hipEventCreateWithFlags(&sl.evPack, hipEventDisableTiming);
+      if (!e)
+        e = hipHostMalloc(reinterpret_cast<void**>(&sl.gather), B * sizeof(pxg::detail::GatherSrc), kMappedPinned);
+      if (!e)
+        e = hipHostGetDevicePointer(reinterpret_cast<void**>(const_cast<pxg::detail::GatherSrc**>(&sl.gatherDev)),
+                                    sl.gather, 0);
     }
     if (rc || e) {
       pxg_server_destroy(sv);
@@ -361,11 +382,16 @@ int pxg_server_create(pxg_context* ctx, const pxg_server_config* cfg, pxg_server
   // pinned chunks for the events of all slots, a batch's worth waiting, and a margin
   const size_t nChunks = size_t(cfg->n_streams + 1) * cfg->max_batch + 256;
   for (size_t k = 0; k < nChunks; ++k) {
-    uint8_t* c = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void**>(&c), kChunkBytes, hipHostMallocDefault) != hipSuccess) {
+    Chunk c;
+    void* dev = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&c.host), kChunkBytes, kMappedPinned) != hipSuccess ||
+        hipHostGetDevicePointer(&dev, c.host, 0) != hipSuccess) {
+      if (c.host)
+        (void)hipHostFree(c.host);
       pxg_server_destroy(sv);
       return fail(PXG_ERR_HIP, "pxg_server_create: pinned host memory for %zu events unavailable", nChunks);
     }
+    c.dev = static_cast<const uint8_t*>(dev);
     sv->allChunks.push_back(c);
     sv->freeChunks.push_back(c);
   }
@@ -399,10 +425,12 @@ void pxg_server_destroy(pxg_server* sv) {
       (void)hipHostFree(sl.hLayout);
     if (sl.hPack)
       (void)hipHostFree(sl.hPack);
+    if (sl.gather)
+      (void)hipHostFree(sl.gather);
     pxg_stream_destroy(sl.s);
   }
-  for (uint8_t* c : sv->allChunks)
-    (void)hipHostFree(c);
+  for (const Chunk& c : sv->allChunks)
+    (void)hipHostFree(c.host);
   delete sv;
 }
 
@@ -438,8 +466,10 @@ int pxg_server_submit(pxg_server* sv, const uint32_t* words, const uint8_t* fedI
     r->chunk = sv->freeChunks.back();
     sv->freeChunks.pop_back();
   }
-  std::memcpy(r->chunk, words, size_t(nWords) * 4);
-  std::memcpy(r->chunk + 4 * kChunkWords, fedIds, nWords / 2);
+  std::memcpy(r->chunk.host, words, size_t(nWords) * 4);
+  std::memcpy(r->chunk.host + 4 * kChunkWords, fedIds, nWords / 2);
+  r->words = reinterpret_cast<const uint32_t*>(r->chunk.dev);
+  r->fedIds = r->chunk.dev + 4 * kChunkWords;
   r->nWords = nWords;
   r->bs = *bs;
   std::lock_guard<std::mutex> g(sv->mu);
@@ -448,6 +478,52 @@ int pxg_server_submit(pxg_server* sv, const uint32_t* words, const uint8_t* fedI
   sv->waiting.push_back(std::move(r));
   sv->cv.notify_all();
   return 0;
+}
+
+int pxg_server_submit_pinned(pxg_server* sv, const uint32_t* words, const uint8_t* fedIds, uint32_t nWords,
+                             const pxg_beamspot* bs, uint64_t* id) {
+  if (!sv || !bs || !id || (nWords && (!words || !fedIds)) || nWords % 2)
+    return fail(PXG_ERR_ARG, "pxg_server_submit_pinned: bad arguments");
+  if (nWords > kChunkWords || nWords > sv->maxBatchWords)
+    return fail(PXG_ERR_CAPACITY, "pxg_server_submit_pinned: %u words exceed the capacity", nWords);
+  if (sv->error)
+    return fail(sv->error, "pxg_server_submit_pinned: server stopped: %s", sv->errorText.c_str());
+  auto r = std::make_unique<Request>();
+  r->tSubmit = nowNs();
+  if (nWords) {
+    void *dw = nullptr, *df = nullptr;
+    if (hipHostGetDevicePointer(&dw, const_cast<uint32_t*>(words), 0) != hipSuccess ||
+        hipHostGetDevicePointer(&df, const_cast<uint8_t*>(fedIds), 0) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PXG_ERR_ARG, "pxg_server_submit_pinned: the words or FED ids are not mapped pinned memory "
+                               "(pxg_host_alloc)");
+    }
+    r->words = static_cast<const uint32_t*>(dw);
+    r->fedIds = static_cast<const uint8_t*>(df);
+  }
+  r->nWords = nWords;
+  r->bs = *bs;
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->id = sv->nextId++;
+  *id = r->id;
+  sv->waiting.push_back(std::move(r));
+  sv->cv.notify_all();
+  return 0;
+}
+
+int pxg_host_alloc(size_t bytes, void** out) {
+  if (!out || bytes == 0)
+    return fail(PXG_ERR_ARG, "pxg_host_alloc: bad arguments");
+  *out = nullptr;
+  const hipError_t e = hipHostMalloc(out, bytes, kMappedPinned);
+  if (e != hipSuccess)
+    return fail(PXG_ERR_HIP, "pxg_host_alloc(%zu): %s", bytes, hipGetErrorString(e));
+  return 0;
+}
+
+void pxg_host_free(void* p) {
+  if (p)
+    (void)hipHostFree(p);
 }
 
 int pxg_server_poll(pxg_server* sv, pxg_server_result* out, int max, int wait_ms) {

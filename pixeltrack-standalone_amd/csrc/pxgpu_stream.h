@@ -73,6 +73,20 @@ namespace pxg::detail {
   // nBeamspots entries: 1 (every event gets bs[0]) or n (event e gets bs[e]).
   int loadEvents(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs, uint32_t nBeamspots,
                  bool sync);
+  // one event's raw words for loadEventsGather, in host memory the device can read
+  // (mapped pinned memory, addresses as the device sees them)
+  struct GatherSrc {
+    const uint32_t* words;
+    const uint8_t* fedIds;
+    uint32_t nWords, wordOffset;
+  };
+  // like loadEvents, but one kernel reads every event's words and FED ids from
+  // device-accessible host memory (raws hold device addresses), instead of two
+  // copies per event; table: n entries of mapped pinned memory, tableDev its
+  // device address.  Asynchronous: the sources must stay unchanged until the
+  // stream has passed the load.
+  int loadEventsGather(pxg_stream* s, uint32_t n, const pxg_raw_view* raws, const pxg_beamspot* bs,
+                       GatherSrc* table, const GatherSrc* tableDev);
   // bytes of device memory a pool of maxEvents events and maxWords words takes
   size_t poolBytes(uint32_t maxEvents, uint32_t maxWords, const CACapacities& caps);
   // destroys the stream's captured stage sequences

@@ -111,6 +111,9 @@ def _gpu():
         lib.pxg_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
         lib.pxg_server_destroy.argtypes = [vp]
         lib.pxg_server_submit.argtypes = [vp, vp, vp, u32, C.POINTER(BeamSpot), C.POINTER(C.c_uint64)]
+        lib.pxg_server_submit_pinned.argtypes = [vp, vp, vp, u32, C.POINTER(BeamSpot), C.POINTER(C.c_uint64)]
+        lib.pxg_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+        lib.pxg_host_free.argtypes = [vp]
         lib.pxg_server_poll.argtypes = [vp, C.POINTER(ServerResult), C.c_int, C.c_int]
         lib.pxg_server_products.argtypes = [vp, C.c_uint64, C.POINTER(vp), C.POINTER(u32)]
         lib.pxg_server_release.argtypes = [vp, C.c_uint64]
@@ -578,6 +581,15 @@ class Server:
                                           C.byref(rid)), "pxg_server_submit")
         return rid.value
 
+    def submit_pinned(self, event, beamspot=None):
+        """Queues a PinnedEvent without a host copy; it must stay unchanged until polled."""
+        rid = C.c_uint64()
+        bs = beamspot if beamspot is not None else self.ctx.beamspot
+        _check(self.lib.pxg_server_submit_pinned(self._h, event.words.ctypes.data, event.fed_ids.ctypes.data,
+                                                 event.words.size, C.byref(bs), C.byref(rid)),
+               "pxg_server_submit_pinned")
+        return rid.value
+
     def poll(self, max_results=1024, wait_ms=0):
         out = (ServerResult * max_results)()
         n = _check(self.lib.pxg_server_poll(self._h, out, max_results, wait_ms), "pxg_server_poll")
@@ -598,6 +610,35 @@ class Server:
         w, b, e = u32(), C.c_uint64(), C.c_uint64()
         _check(self.lib.pxg_server_info(self._h, C.byref(w), C.byref(b), C.byref(e)), "pxg_server_info")
         return {"max_batch_words": w.value, "batches": b.value, "events": e.value}
+
+
+class PinnedEvent:
+    """One event's words and FED ids in mapped pinned host memory (pxg_host_alloc), for
+    Server.submit_pinned: the device reads them directly while it loads the batch."""
+
+    def __init__(self, words, fed_ids):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        fed_ids = np.ascontiguousarray(fed_ids, dtype=np.uint8)
+        self._lib = _gpu()
+        self._ptr = vp()
+        nb = max(1, 4 * words.size + fed_ids.size + 16)
+        _check(self._lib.pxg_host_alloc(nb, C.byref(self._ptr)), "pxg_host_alloc")
+        raw = (C.c_uint8 * nb).from_address(self._ptr.value)
+        buf = np.frombuffer(raw, dtype=np.uint8)
+        self.words = buf[:4 * words.size].view(np.uint32)
+        off = (4 * words.size + 15) // 16 * 16
+        self.fed_ids = buf[off:off + fed_ids.size]
+        self.words[:] = words
+        self.fed_ids[:] = fed_ids
+
+    def close(self):
+        if getattr(self, "_ptr", None) and self._ptr.value:
+            self.words = self.fed_ids = None
+            self._lib.pxg_host_free(self._ptr)
+            self._ptr = vp()
+
+    def __del__(self):
+        self.close()
 
 
 CAPACITY_FIELDS = ("max_doublets", "max_tuples", "max_tuples_mult", "max_hit_to_tuple", "max_cell_lists",

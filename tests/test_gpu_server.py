@@ -55,6 +55,37 @@ def test_server_mix_equals_alone(detector, gpu_ctx, gpu_stream):
     sv.close()
 
 
+def test_server_pinned_submission(detector, gpu_ctx, gpu_stream):
+    """Events handed over in the caller's mapped pinned buffers (pxg_server_submit_pinned:
+    the device reads them, no host copy) mixed with copied ones in the same batches: every
+    result equals the plain path's; memory that is not mapped pinned is refused."""
+    from pxtrack import gpu
+    raws = [event(detector, "pu20_80", 90 + i)[0] for i in range(16)]
+    inputs = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in raws]
+    pinned = [gpu.PinnedEvent(*x) for x in inputs]
+    sv = gpu.Server(gpu_ctx, max_batch=8, n_streams=2)
+    ids = {}
+    for i, x in enumerate(inputs):
+        rid = sv.submit_pinned(pinned[i]) if i % 3 else sv.submit(*x)
+        ids[rid] = i
+    results = []
+    deadline = time.time() + 60
+    while len(results) < len(ids) and time.time() < deadline:
+        results += sv.poll(wait_ms=100)
+    assert len(results) == len(ids)
+    assert max(r["batch_size"] for r in results) > 1
+    for r in results:
+        i = ids[r["id"]]
+        _same(_plain(gpu_stream, *inputs[i]), sv.products(r["id"]), f"pinned event {i}")
+    class Fake:  # pageable numpy memory
+        words, fed_ids = inputs[0][0].copy(), inputs[0][1].copy()
+    with pytest.raises(RuntimeError, match="mapped pinned"):
+        sv.submit_pinned(Fake())
+    sv.close()
+    for pe in pinned:
+        pe.close()
+
+
 def test_server_budget_limits_batch_words(detector, gpu_ctx, gpu_stream):
     """A small HBM budget shrinks the slots' word capacity; events still complete, alone or batched."""
     from pxtrack import gpu
