@@ -251,6 +251,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tuning", metavar="LANES,GRID",
+                    help="launch shape (doublet lanes per slot, workgroups per launch) instead of the autotuner's")
     ap.add_argument("--streams", type=int, default=3, help="HIP streams the batches are spread over")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
     ap.add_argument("--serve-events", type=int, default=600)
@@ -272,6 +274,10 @@ def main():
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="summarise a rocprofv3 --kernel-trace --marker-trace csv output directory of this "
                          "script per phase (markdown on stdout) and exit")
+    ap.add_argument("--summarize-sq", metavar="DIR",
+                    help="summarise a rocprofv3 --pmc csv run of this script with SQ_WAVE_CYCLES, "
+                         "SQ_LDS_BANK_CONFLICT, SQ_ACTIVE_INST_VALU, SQ_ACTIVE_INST_LDS and GRBM_GUI_ACTIVE into "
+                         "occupancy, LDS bank-conflict and VALU-busy percentages per kernel (markdown) and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc csv output directories (FETCH_SIZE, WRITE_SIZE) of an "
                          "--only-timed --calibrate-copy run of this script into HBM bytes per event and kernel "
@@ -279,6 +285,9 @@ def main():
     args = ap.parse_args()
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
+        return
+    if args.summarize_sq:
+        summarize_sq(args.summarize_sq)
         return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.events * args.steps, args.calibrate_copy)
@@ -349,7 +358,12 @@ def main():
 
     # ---- launch shape chosen by the autotuner on the first batch
     tuned = None
-    if not args.no_tune:
+    if args.tuning:
+        lanes, grid = (int(x) for x in args.tuning.split(","))
+        for st in streams:
+            st.set_tuning(lanes, grid)
+        tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": None}
+    elif not args.no_tune:
         rtx.push("autotune")
         lanes, grid, tms = stream.autotune(0, min(B, bounds[1]), args.upto)
         rtx.pop()
@@ -666,6 +680,38 @@ def _pmc_per_kernel(d, counter):
         x = float(r["Counter_Value"])
         out[sym] = (c + 1, v + x, max(m, x))
     return out
+
+
+def summarize_sq(d, cu_num=256, waves_per_cu=32):
+    """Per kernel, from the dispatches of a --pmc run (rocprofv3's derived-metric formulas):
+    occupancy % = 400 SQ_WAVE_CYCLES / GRBM_GUI_ACTIVE / CUs / max waves per CU (SQ_WAVE_CYCLES
+    counts in quad cycles), LDS bank-conflict % = 100 SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE /
+    CUs, VALU busy % = 100 SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE, LDS busy % likewise with
+    SQ_ACTIVE_INST_LDS; each summed over the kernel's dispatches."""
+    f = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    if not f:
+        sys.exit(f"no counter_collection.csv under {d}")
+    acc = {}
+    for r in _read_csv(f[0]):
+        sym = kernel_symbol(r["Kernel_Name"])
+        a = acc.setdefault(sym, {"dispatches": set()})
+        a["dispatches"].add(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        name = r["Counter_Name"]
+        a[name] = a.get(name, 0.0) + float(r["Counter_Value"])
+    print("| kernel | label | dispatches | occupancy % | LDS bank conflict % | VALU busy % | LDS busy % |")
+    print("|---|---|---|---|---|---|---|")
+    rows = []
+    for sym, a in acc.items():
+        gui = a.get("GRBM_GUI_ACTIVE", 0.0)
+        if not gui:
+            continue
+        occ = 400 * a.get("SQ_WAVE_CYCLES", 0.0) / gui / cu_num / waves_per_cu
+        bank = 100 * a.get("SQ_LDS_BANK_CONFLICT", 0.0) / gui / cu_num
+        valu = 100 * a.get("SQ_ACTIVE_INST_VALU", 0.0) / cu_num / gui
+        lds = 100 * a.get("SQ_ACTIVE_INST_LDS", 0.0) / cu_num / gui
+        rows.append((gui, sym, len(a["dispatches"]), occ, bank, valu, lds))
+    for gui, sym, n, occ, bank, valu, lds in sorted(rows, reverse=True):
+        print(f"| {sym} | {kernel_label(sym) or ''} | {n} | {occ:.1f} | {bank:.2f} | {valu:.1f} | {lds:.1f} |")
 
 
 def summarize_pmc(fetch_dir, write_dir, events, copy_mib):
