@@ -105,7 +105,10 @@ class Roctx:
 
     def __init__(self):
         self.lib = None
-        for name in ("libroctx64.so", "/opt/rocm/lib/libroctx64.so"):
+        # rocprofv3 --marker-trace intercepts the rocprofiler-sdk ROCTx library (the older
+        # roctracer one is not seen by it)
+        for name in ("librocprofiler-sdk-roctx.so", "/opt/rocm/lib/librocprofiler-sdk-roctx.so", "libroctx64.so",
+                     "/opt/rocm/lib/libroctx64.so"):
             try:
                 self.lib = ctypes.CDLL(name)
                 self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
@@ -682,12 +685,15 @@ def _pmc_per_kernel(d, counter):
     return out
 
 
-def summarize_sq(d, cu_num=256, waves_per_cu=32):
+def summarize_sq(d, cu_num=256, waves_per_cu=32, xcds=8):
     """Per kernel, from the dispatches of a --pmc run (rocprofv3's derived-metric formulas):
     occupancy % = 400 SQ_WAVE_CYCLES / GRBM_GUI_ACTIVE / CUs / max waves per CU (SQ_WAVE_CYCLES
     counts in quad cycles), LDS bank-conflict % = 100 SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE /
     CUs, VALU busy % = 100 SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE, LDS busy % likewise with
-    SQ_ACTIVE_INST_LDS; each summed over the kernel's dispatches."""
+    SQ_ACTIVE_INST_LDS; each summed over the kernel's dispatches.  On this MI355X the SQ counters
+    of a dispatch cover one of the 8 XCDs against the whole GPU's GRBM_GUI_ACTIVE: a copy kernel
+    of 2048 waves (at most 8 waves per CU, 25 % occupancy) reads 2.8 % by the plain formula,
+    so every ratio is multiplied by the XCD count."""
     f = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
     if not f:
         sys.exit(f"no counter_collection.csv under {d}")
@@ -705,10 +711,10 @@ def summarize_sq(d, cu_num=256, waves_per_cu=32):
         gui = a.get("GRBM_GUI_ACTIVE", 0.0)
         if not gui:
             continue
-        occ = 400 * a.get("SQ_WAVE_CYCLES", 0.0) / gui / cu_num / waves_per_cu
-        bank = 100 * a.get("SQ_LDS_BANK_CONFLICT", 0.0) / gui / cu_num
-        valu = 100 * a.get("SQ_ACTIVE_INST_VALU", 0.0) / cu_num / gui
-        lds = 100 * a.get("SQ_ACTIVE_INST_LDS", 0.0) / cu_num / gui
+        occ = 400 * xcds * a.get("SQ_WAVE_CYCLES", 0.0) / gui / cu_num / waves_per_cu
+        bank = 100 * xcds * a.get("SQ_LDS_BANK_CONFLICT", 0.0) / gui / cu_num
+        valu = 100 * xcds * a.get("SQ_ACTIVE_INST_VALU", 0.0) / cu_num / gui
+        lds = 100 * xcds * a.get("SQ_ACTIVE_INST_LDS", 0.0) / cu_num / gui
         rows.append((gui, sym, len(a["dispatches"]), occ, bank, valu, lds))
     for gui, sym, n, occ, bank, valu, lds in sorted(rows, reverse=True):
         print(f"| {sym} | {kernel_label(sym) or ''} | {n} | {occ:.1f} | {bank:.2f} | {valu:.1f} | {lds:.1f} |")
