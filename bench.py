@@ -254,8 +254,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tuning", metavar="LANES,GRID",
-                    help="launch shape (doublet lanes per slot, workgroups per launch) instead of the autotuner's")
+    ap.add_argument("--tuning", metavar="LANES,GRID[,PER_EVENT,STAGE,WINDOW]",
+                    help="launch shape (doublet lanes per slot, workgroups per launch, at most per event, connect's "
+                         "staged candidates, fishbone window) instead of the autotuner's")
     ap.add_argument("--streams", type=int, default=3, help="HIP streams the batches are spread over")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
     ap.add_argument("--serve-events", type=int, default=600)
@@ -362,17 +363,19 @@ def main():
     # ---- launch shape chosen by the autotuner on the first batch
     tuned = None
     if args.tuning:
-        lanes, grid = (int(x) for x in args.tuning.split(","))
+        v = [int(x) for x in args.tuning.split(",")]
+        keys = ("doublet_lanes", "grid_target", "max_blocks_per_event", "connect_stage", "fishbone_window")
+        shape = dict(zip(keys, v))
         for st in streams:
-            st.set_tuning(lanes, grid)
-        tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": None}
+            st.set_launch_shape(shape)
+        tuned = dict(stream.launch_shape(), ms_per_batch=None)
     elif not args.no_tune:
         rtx.push("autotune")
-        lanes, grid, tms = stream.autotune(0, min(B, bounds[1]), args.upto)
+        shape, tms = stream.autotune_shape(0, min(B, bounds[1]), args.upto)
         rtx.pop()
         for st in streams[1:]:
-            st.set_tuning(lanes, grid)
-        tuned = {"doublet_lanes": lanes, "grid_target": grid, "ms_per_batch": round(tms, 4)}
+            st.set_launch_shape(shape)
+        tuned = dict(shape, ms_per_batch=round(tms, 4))
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
 
     # ---- batched path: warmup, then K timed steps (one step = all events), nothing instrumented

@@ -367,8 +367,23 @@ int pxg_get_tuning(pxg_stream* s, uint32_t* doublet_lanes, uint32_t* grid_target
  * setting drop the captured graphs; a profiled stream runs eagerly.  No counterpart in the
  * reference, which launches the kernels of a stage one by one. */
 int pxg_set_graphs(pxg_stream* s, int on);
-/* Times the chain on resident events [first, first+count) for each launch shape of a fixed
- * grid, keeps the fastest and reports it (ms of one pass).  No counterpart in the reference. */
+/* The launch shape in full (products do not depend on it; no counterpart in the reference). */
+typedef struct {
+  uint32_t doublet_lanes;        /* lanes per doublet-search slot: power of two, 1..64 */
+  uint32_t grid_target;          /* workgroups of a grid-stride launch, shared by its events (>= 4) */
+  uint32_t max_blocks_per_event; /* at most this many of them per event (4..4096) */
+  uint32_t connect_stage;        /* candidate cells connect stages per wavefront in LDS: 128, 256, 512 */
+  uint32_t fishbone_window;      /* list entries the fishbone stages per wavefront: 128, 256, 384 */
+} pxg_launch_shape;
+int pxg_set_launch_shape(pxg_stream* s, const pxg_launch_shape* shape);
+int pxg_get_launch_shape(pxg_stream* s, pxg_launch_shape* shape);
+/* Times the chain on resident events [first, first+count) over launch shapes (median of 3
+ * timed passes each, after a warm-up pass): doublet lanes x grid target, then, around the
+ * best, connect's staging, the fishbone window and the workgroups per event; sets the fastest
+ * on the stream and reports it with its time (ms of one pass).  No counterpart in the reference. */
+int pxg_autotune_shape(pxg_stream* s, uint32_t first, uint32_t count, int upto, pxg_launch_shape* best,
+                       float* best_ms);
+/* The same, reporting only the lanes and the grid target. */
 int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* best_lanes,
                  uint32_t* best_grid, float* best_ms);
 

@@ -411,10 +411,16 @@ namespace pxg {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
-    constexpr uint32_t kStage = 256;  // staged candidate cells per wave
-    constexpr uint32_t kSegs = 64;    // staged candidate ranges per wave
+    // kStage: staged candidate cells per wave (a launch-shape knob: 128, 256 or 512); the
+    // LDS it takes (about 22 bytes per entry and wave) sets the waves per SIMD
+    constexpr uint32_t kSegs = 64;  // staged candidate ranges per wave
+    template <uint32_t kStage>
+    constexpr int connectWaves() {
+      return kStage <= 128 ? 8 : kStage <= 256 ? 7 : 3;
+    }
 
-    __global__ void __launch_bounds__(kCaBlock, 7) connectKernel(Pool p, CAPool ca) {
+    template <uint32_t kStage>
+    __global__ void __launch_bounds__(kCaBlock, connectWaves<kStage>()) connectKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
       __shared__ uint32_t sMe[kCaBlock / 64][kStage];
@@ -546,32 +552,52 @@ namespace pxg {
             }
             ++n;
           };
-          auto cuts = [&](const float4& p3) {
-            return alignedRZ(p1.w, p1.z, p2.w, p2.z, p3.w, p3.z, kCaPtMin, thetaCut) &&
-                   dcaCut(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, dcut, kCaHardCurvCut);
+          // The two cuts in two passes: the r-z alignment over every candidate, then the
+          // circle cut over the aligned ones only.  About 2 % of the candidates are
+          // aligned, so the circle cut (three square roots and a division) runs a couple
+          // of times per chunk instead of on every candidate some lane of the wave is
+          // still testing.  A candidate passes when both cuts pass, as in the single test.
+          auto aligned = [&](const float4& p3) {
+            return alignedRZ(p1.w, p1.z, p2.w, p2.z, p3.w, p3.z, kCaPtMin, thetaCut);
+          };
+          auto curved = [&](const float4& p3) {
+            return dcaCut(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, dcut, kCaHardCurvCut);
           };
           if (staged) {
             for (uint32_t b0 = 0; b0 < total; b0 += 64) {
               const uint32_t m = min(64u, total - b0);
-              uint64_t acc = 0;
+              const float4* cand = sPos[w] + gBase + b0;
+              uint64_t al = 0;
               for (uint32_t k = 0; k < m; ++k)
-                if (cuts(sPos[w][gBase + b0 + k]))
+                if (aligned(cand[k]))
+                  al |= 1ull << k;
+              uint64_t acc = 0;
+              for (uint64_t rest = al; rest; rest &= rest - 1) {
+                const uint32_t k = __ffsll((unsigned long long)rest) - 1;
+                if (curved(cand[k]))
                   acc |= 1ull << k;
+              }
               for (; acc; acc &= acc - 1)
                 append(sMe[w][gBase + b0 + __ffsll((unsigned long long)acc) - 1]);
             }
           } else {
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-              // chunks of up to 64 candidates: the cuts first, in a loop without
-              // stores (so the candidates' loads are issued together), then the
-              // accepted ones in ascending order
+              // chunks of up to 64 candidates: the alignment first, in a loop without
+              // stores (so the candidates' loads are issued together), then the circle
+              // cut on the aligned ones, then the accepted ones in ascending order
               for (uint32_t b0 = 0; b0 < len[q]; b0 += 64) {
                 const uint32_t m = min(64u, len[q] - b0), base = beg[q] + b0;
-                uint64_t acc = 0;
+                uint64_t al = 0;
                 for (uint32_t k = 0; k < m; ++k)
-                  if (cuts(H.pos[ca.cellOuter[cb + base + k]]))
+                  if (aligned(H.pos[ca.cellOuter[cb + base + k]]))
+                    al |= 1ull << k;
+                uint64_t acc = 0;
+                for (uint64_t rest = al; rest; rest &= rest - 1) {
+                  const uint32_t k = __ffsll((unsigned long long)rest) - 1;
+                  if (curved(H.pos[ca.cellOuter[cb + base + k]]))
                     acc |= 1ull << k;
+                }
                 for (; acc; acc &= acc - 1)
                   append(base + __ffsll((unsigned long long)acc) - 1);
               }
@@ -604,11 +630,10 @@ namespace pxg {
       float x, y, z, n;  // inner hit minus the list's hit, squared norm
     };
 
-    // lists that the wave's 64 positions touch, staged whole (up to 256 entries: six
-    // workgroups per CU; a wave whose lists span more reads them from memory)
-    constexpr uint32_t kFbWindow = 256;
-
-    template <bool kCheckTrack>
+    // lists that the wave's 64 positions touch, staged whole (up to kFbWindow entries, a
+    // launch-shape knob: 128, 256 or 384; at 256 six workgroups fit a CU); a wave whose
+    // lists span more reads them from memory
+    template <bool kCheckTrack, uint32_t kFbWindow>
     __global__ void __launch_bounds__(kCaBlock) fishboneKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       const HitView H = hitsOf(p, e);
@@ -1203,14 +1228,33 @@ namespace pxg {
   }
 
   void launchConnect(const CALaunch& l) {
-    PXG_K("connect", connectKernel, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+    const dim3 g(l.blocks, l.nEvents);
+    if (l.connectStage <= 128)
+      PXG_K("connect", connectKernel<128>, g, kCaBlock, *l.p, *l.ca);
+    else if (l.connectStage <= 256)
+      PXG_K("connect", connectKernel<256>, g, kCaBlock, *l.p, *l.ca);
+    else
+      PXG_K("connect", connectKernel<512>, g, kCaBlock, *l.p, *l.ca);
+  }
+
+  template <bool kCheckTrack>
+  void fishboneLaunch(const CALaunch& l, const char* name) {
+    const dim3 g(l.blocks, l.nEvents);
+    l.t->b(name);
+    if (l.fishboneWindow <= 128)
+      fishboneKernel<kCheckTrack, 128><<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca);
+    else if (l.fishboneWindow <= 256)
+      fishboneKernel<kCheckTrack, 256><<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca);
+    else
+      fishboneKernel<kCheckTrack, 384><<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca);
+    l.t->e();
   }
 
   void launchFishbone(const CALaunch& l, bool checkTrack) {
     if (checkTrack)
-      PXG_K("lateFishbone", fishboneKernel<true>, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+      fishboneLaunch<true>(l, "lateFishbone");
     else
-      PXG_K("fishbone", fishboneKernel<false>, dim3(l.blocks, l.nEvents), kCaBlock, *l.p, *l.ca);
+      fishboneLaunch<false>(l, "fishbone");
   }
 
   void launchNtuplets(const CALaunch& l) {

@@ -343,9 +343,10 @@ namespace pxg::detail {
     Timer timer;
     if (s->profiling)
       timer = {timerBegin, timerEnd, s};
-    const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(512u, s->gridTarget / count)),
+    const CALaunch l{&c,        &v,     &ca, count, std::max(4u, std::min(s->maxBlocksPerEvent, s->gridTarget / count)),
                      s->stream, &timer, s->doubletLanes, s->vertexAlgo, s->earlyFishbone, s->lateFishbone,
-                     s->riemannFit, s->doStats ? s->counters : nullptr, double(s->bField), s->vertexPtMin};
+                     s->riemannFit, s->doStats ? s->counters : nullptr, double(s->bField), s->vertexPtMin,
+                     s->connectStage, s->fishboneWindow};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
       launchOuterHitLists(l);
@@ -1009,54 +1010,135 @@ int pxg_get_tuning(pxg_stream* s, uint32_t* doubletLanes, uint32_t* gridTarget) 
   return 0;
 }
 
-int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* bestLanes,
-                 uint32_t* bestGrid, float* bestMs) {
+int pxg_set_launch_shape(pxg_stream* s, const pxg_launch_shape* shape) {
+  if (!s || !shape)
+    return fail(PXG_ERR_ARG, "pxg_set_launch_shape: null argument");
+  const pxg_launch_shape& h = *shape;
+  if (h.doublet_lanes == 0 || h.doublet_lanes > 64 || (h.doublet_lanes & (h.doublet_lanes - 1)) ||
+      h.grid_target < 4 || h.max_blocks_per_event < 4 || h.max_blocks_per_event > 4096 ||
+      (h.connect_stage != 128 && h.connect_stage != 256 && h.connect_stage != 512) ||
+      (h.fishbone_window != 128 && h.fishbone_window != 256 && h.fishbone_window != 384))
+    return fail(PXG_ERR_ARG, "pxg_set_launch_shape: lanes a power of two in [1, 64], grid target >= 4, "
+                             "4..4096 workgroups per event, connect stage 128/256/512, fishbone window 128/256/384");
+  s->doubletLanes = h.doublet_lanes;
+  s->gridTarget = h.grid_target;
+  s->maxBlocksPerEvent = h.max_blocks_per_event;
+  s->connectStage = h.connect_stage;
+  s->fishboneWindow = h.fishbone_window;
+  dropGraphs(s);
+  return 0;
+}
+
+int pxg_get_launch_shape(pxg_stream* s, pxg_launch_shape* shape) {
+  if (!s || !shape)
+    return fail(PXG_ERR_ARG, "pxg_get_launch_shape: null argument");
+  *shape = {s->doubletLanes, s->gridTarget, s->maxBlocksPerEvent, s->connectStage, s->fishboneWindow};
+  return 0;
+}
+
+namespace {
+  // median of `passes` timed runs of the chain (one warm-up run first)
+  int timeShape(pxg_stream* s, uint32_t first, uint32_t count, int upto, int passes, hipEvent_t a, hipEvent_t b,
+                float& ms) {
+    int rc = runStages(s, first, count, 1, upto);
+    float t[8];
+    for (int k = 0; !rc && k < passes; ++k) {
+      (void)hipEventRecord(a, s->stream);
+      rc = runStages(s, first, count, 1, upto);
+      (void)hipEventRecord(b, s->stream);
+      if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&t[k], a, b) != hipSuccess))
+        rc = fail(PXG_ERR_HIP, "pxg_autotune: timing failed");
+    }
+    if (rc)
+      return rc;
+    std::sort(t, t + passes);
+    ms = t[passes / 2];
+    return 0;
+  }
+}  // namespace
+
+int pxg_autotune_shape(pxg_stream* s, uint32_t first, uint32_t count, int upto, pxg_launch_shape* best,
+                       float* bestMs) {
   if (!s || upto < 1 || upto > 5 || count == 0 || first + count > s->nLoaded)
     return fail(PXG_ERR_ARG, "pxg_autotune: bad arguments");
+  constexpr int kPasses = 3;
   static const uint32_t lanes[] = {2, 4, 8, 16, 32, 64};
-  static const uint32_t grids[] = {1024, 2048, 4096, 8192};
+  static const uint32_t grids[] = {2048, 4096, 8192, 16384};
+  static const uint32_t stages[] = {128, 256, 512};
+  static const uint32_t windows[] = {128, 256, 384};
+  static const uint32_t perEvent[] = {256, 512, 1024};
   dropGraphs(s);  // captured with the shape it replaces
   const bool wasProfiling = s->profiling;
   s->profiling = false;
   hipEvent_t a, b;
   PXG_HIP(hipEventCreate(&a));
   PXG_HIP(hipEventCreate(&b));
-  float best = 1e30f;
-  uint32_t bl = s->doubletLanes, bg = s->gridTarget;
+  float bestT = 1e30f;
+  pxg_launch_shape cur{}, keep{};
+  (void)pxg_get_launch_shape(s, &cur);
+  keep = cur;
+  int rc = 0;
+  auto trial = [&](const pxg_launch_shape& h) {
+    if (rc)
+      return;
+    rc = pxg_set_launch_shape(s, &h);
+    float ms = 0;
+    if (!rc)
+      rc = timeShape(s, first, count, upto, kPasses, a, b, ms);
+    if (!rc && ms < bestT) {
+      bestT = ms;
+      keep = h;
+    }
+  };
+  // coordinate descent: doublet lanes x grid target, then connect's staging, the
+  // fishbone's window and the workgroups per event, each around the best so far
   for (uint32_t g : grids)
     for (uint32_t ln : lanes) {
-      s->doubletLanes = ln;
-      s->gridTarget = g;
-      int rc = runStages(s, first, count, 1, upto);  // warm
-      float ms = 0;
-      if (!rc) {
-        (void)hipEventRecord(a, s->stream);
-        rc = runStages(s, first, count, 1, upto);
-        (void)hipEventRecord(b, s->stream);
-      }
-      if (rc || hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        s->profiling = wasProfiling;
-        return rc ? rc : fail(PXG_ERR_HIP, "pxg_autotune: timing failed");
-      }
-      if (ms < best) {
-        best = ms;
-        bl = ln;
-        bg = g;
-      }
+      pxg_launch_shape h = cur;
+      h.doublet_lanes = ln;
+      h.grid_target = g;
+      trial(h);
     }
+  for (uint32_t v : stages) {
+    pxg_launch_shape h = keep;
+    h.connect_stage = v;
+    trial(h);
+  }
+  for (uint32_t v : windows) {
+    pxg_launch_shape h = keep;
+    h.fishbone_window = v;
+    trial(h);
+  }
+  for (uint32_t v : perEvent) {
+    pxg_launch_shape h = keep;
+    h.max_blocks_per_event = v;
+    trial(h);
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  s->doubletLanes = bl;
-  s->gridTarget = bg;
   s->profiling = wasProfiling;
-  if (bestLanes)
-    *bestLanes = bl;
-  if (bestGrid)
-    *bestGrid = bg;
+  if (rc)
+    return rc;
+  rc = pxg_set_launch_shape(s, &keep);
+  if (rc)
+    return rc;
+  if (best)
+    *best = keep;
   if (bestMs)
-    *bestMs = best;
+    *bestMs = bestT;
+  return 0;
+}
+
+int pxg_autotune(pxg_stream* s, uint32_t first, uint32_t count, int upto, uint32_t* bestLanes,
+                 uint32_t* bestGrid, float* bestMs) {
+  pxg_launch_shape h{};
+  const int rc = pxg_autotune_shape(s, first, count, upto, &h, bestMs);
+  if (rc)
+    return rc;
+  if (bestLanes)
+    *bestLanes = h.doublet_lanes;
+  if (bestGrid)
+    *bestGrid = h.grid_target;
   return 0;
 }
 

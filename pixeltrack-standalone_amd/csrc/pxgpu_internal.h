@@ -254,6 +254,8 @@ namespace pxg {
     unsigned long long* counters;  // doStats counters (11), null when off
     double bField;          // the fits' field in 1/GeV (a float widened, as makeTuples passes it)
     float vertexPtMin;      // tracks below this pt are left out of the vertex finder
+    uint32_t connectStage;  // candidate cells connect stages per wave in LDS: 128, 256 or 512
+    uint32_t fishboneWindow;  // list entries the fishbone stages per wave in LDS: 128, 256 or 384
   };
   void launchDoublets(const CALaunch& l);
   void launchOuterHitLists(const CALaunch& l);

@@ -40,6 +40,9 @@ struct pxg_stream {
   bool profiling = false;
   uint32_t doubletLanes = 8;   // tuning: lanes per doublet slot
   uint32_t gridTarget = 2048;  // tuning: workgroups per grid-stride launch (divided among the events)
+  uint32_t maxBlocksPerEvent = 512;  // tuning: cap of the workgroups per event of such a launch
+  uint32_t connectStage = 256;       // tuning: connect's staged candidates per wave (128, 256, 512)
+  uint32_t fishboneWindow = 256;     // tuning: the fishbone's staged list entries per wave (128, 256, 384)
   int vertexAlgo = 0;          // vertex clustering algorithm (PXG_VERTEX_*)
   bool earlyFishbone = true;   // the reference's defaults: early fishbone on, late off
   bool lateFishbone = false;

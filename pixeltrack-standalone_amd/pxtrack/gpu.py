@@ -63,6 +63,11 @@ class CAInternals(C.Structure):
                                   "hitToTupleBins")] + [("counts", u32 * 16)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("doublet_lanes", u32), ("grid_target", u32), ("max_blocks_per_event", u32), ("connect_stage", u32),
+                ("fishbone_window", u32)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", u32), ("n_streams", u32), ("hbm_budget", C.c_uint64), ("profile", C.c_int),
                 ("upto", C.c_int), ("doublet_lanes", u32), ("grid_target", u32), ("transfer", C.c_int)]
@@ -119,6 +124,9 @@ def _gpu():
         lib.pxg_server_release.argtypes = [vp, C.c_uint64]
         lib.pxg_server_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         lib.pxg_set_tuning.argtypes = [vp, u32, u32]
+        lib.pxg_set_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
+        lib.pxg_get_launch_shape.argtypes = [vp, C.POINTER(LaunchShape)]
+        lib.pxg_autotune_shape.argtypes = [vp, u32, u32, C.c_int, C.POINTER(LaunchShape), C.POINTER(C.c_float)]
         lib.pxg_set_vertex_algorithm.argtypes = [vp, C.c_int]
         lib.pxg_set_fishbone.argtypes = [vp, C.c_int, C.c_int]
         lib.pxg_set_riemann_fit.argtypes = [vp, C.c_int]
@@ -451,6 +459,24 @@ class Stream:
 
     def set_tuning(self, doublet_lanes, grid_target):
         _check(self.lib.pxg_set_tuning(self._h, doublet_lanes, grid_target), "pxg_set_tuning")
+
+    def set_launch_shape(self, shape):
+        """shape: dict with the fields of LaunchShape (missing ones keep their value)."""
+        cur = self.launch_shape()
+        cur.update(shape)
+        _check(self.lib.pxg_set_launch_shape(self._h, C.byref(LaunchShape(**cur))), "pxg_set_launch_shape")
+
+    def launch_shape(self):
+        h = LaunchShape()
+        _check(self.lib.pxg_get_launch_shape(self._h, C.byref(h)), "pxg_get_launch_shape")
+        return {f: getattr(h, f) for f, _ in LaunchShape._fields_}
+
+    def autotune_shape(self, first, count, upto=5):
+        """Chooses the fastest launch shape on resident events; returns (shape dict, ms per pass)."""
+        h, ms = LaunchShape(), C.c_float()
+        _check(self.lib.pxg_autotune_shape(self._h, first, count, upto, C.byref(h), C.byref(ms)),
+               "pxg_autotune_shape")
+        return {f: getattr(h, f) for f, _ in LaunchShape._fields_}, ms.value
 
     def set_vertex_algorithm(self, algo):
         """VERTEX_DENSITY (default), VERTEX_DBSCAN or VERTEX_ITERATIVE."""

@@ -357,8 +357,14 @@ def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
     s = gpu_ctx.stream(max_events=3, max_words=sum(w.size for w, _ in unpacked) + 1024)
     s.load(unpacked)
     ref = None
-    for lanes, grid in ((1, 64), (8, 2048), (64, 8192), (4, 333)):
-        s.set_tuning(lanes, grid)
+    shapes = [dict(doublet_lanes=1, grid_target=64), dict(doublet_lanes=8, grid_target=2048),
+              dict(doublet_lanes=64, grid_target=8192), dict(doublet_lanes=4, grid_target=333),
+              dict(connect_stage=128, fishbone_window=128, max_blocks_per_event=4),
+              dict(connect_stage=512, fishbone_window=384, max_blocks_per_event=4096, grid_target=65536),
+              dict(connect_stage=256, fishbone_window=256, max_blocks_per_event=512, grid_target=16384)]
+    for shape in shapes:
+        s.set_launch_shape(shape)
+        lanes, grid = s.launch_shape()["doublet_lanes"], s.launch_shape()["grid_target"]
         s.run(0, 3, upto=5)
         s.fetch_counts(0, 3)
         got = [s.event_products(e, upto=5) for e in range(3)]
@@ -380,3 +386,8 @@ def test_products_do_not_depend_on_launch_shape(detector, oracle, gpu_ctx):
                     assert np.array_equal(ref[e][k][:n], got[e][k][:n]), f"shape {lanes}/{grid} event {e}: {k}"
     lanes, grid, ms = s.autotune(0, 3, upto=5)
     assert lanes in (2, 4, 8, 16, 32, 64) and grid >= 1024 and ms > 0
+    shape, ms = s.autotune_shape(0, 3, upto=5)
+    assert shape == s.launch_shape() and ms > 0
+    assert shape["connect_stage"] in (128, 256, 512) and shape["fishbone_window"] in (128, 256, 384)
+    with pytest.raises(RuntimeError):
+        s.set_launch_shape(dict(connect_stage=200))

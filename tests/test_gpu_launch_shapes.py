@@ -50,10 +50,10 @@ def test_bench_launch_shapes_match_plain_and_oracle(detector, oracle, gpu_ctx, g
     plan = [(_load(gpu_ctx, in50), [(0, 167), (167, 167)]),
             (_load(gpu_ctx, inmix), [(0, 128)]),
             (_load(gpu_ctx, inmix), [(32 * k, 32) for k in range(4)])]
-    lanes, grid, ms = plan[0][0].autotune(0, 167, upto=5)
-    print(f"autotuned shape: {lanes} doublet lanes, {grid} workgroups per launch ({ms:.2f} ms)", flush=True)
+    shape, ms = plan[0][0].autotune_shape(0, 167, upto=5)
+    print(f"autotuned shape: {shape} ({ms:.2f} ms)", flush=True)
     for s, _ in plan[1:]:
-        s.set_tuning(lanes, grid)
+        s.set_launch_shape(shape)
     # every stream's launches enqueued before any result is read, as the benchmark does
     for s, launches in plan:
         for first, count in launches:
