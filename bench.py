@@ -656,7 +656,12 @@ def summarize_trace(d):
     tot_timed = sum(s.get("timed", (0, 0.0))[1] for s in stats.values())
     print("| kernel | label | timed launches | timed avg us | timed share | profiled avg us | isolated avg us |")
     print("|---|---|---|---|---|---|---|")
+    iso_sum = 0.0
     for sym, s in sorted(stats.items(), key=lambda kv: -kv[1].get("timed", (0, 0.0))[1]):
+        if not any(ph in s for ph in ("timed", "profiled", "isolated")):
+            continue  # ran only while the launch shape was tuned
+        ci, ti = s.get("isolated", (0, 0.0))
+        iso_sum += ti / ci if ci else 0.0
         def avg(ph):
             c, t = s.get(ph, (0, 0.0))
             return f"{t / c:.2f}" if c else ""
@@ -664,6 +669,8 @@ def summarize_trace(d):
         share = f"{100 * t / tot_timed:.2f} %" if tot_timed and c else ""
         print(f"| {sym} | {kernel_label(sym) or ''} | {c} | {avg('timed')} | {share} | {avg('profiled')} | "
               f"{avg('isolated')} |")
+    print(f"\nSum of the isolated averages (one launch of each kernel, nothing overlapping): "
+          f"{iso_sum:.1f} us")
     if phases:
         print()
         for a, b, p in phases:
