@@ -26,8 +26,7 @@ namespace pxg {
       const uint16_t* phiBins;
       const float4* pos;
       const HitAttr* attr;
-      const float4* posPhi;     // pos[phiBins[k]] at k
-      const HitAttr* attrPhi;   // attr[phiBins[k]] at k
+      const PhiCand* candPhi;   // hit phiBins[k] at k
     };
 
     __device__ __forceinline__ HitView hitsOf(const Pool& p, uint32_t e) {
@@ -44,8 +43,7 @@ namespace pxg {
               p.phiBins + h,
               p.pos + h,
               p.attr + h,
-              p.posPhi + h,
-              p.attrPhi + h};
+              p.candPhi + h};
     }
 
     __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
@@ -136,11 +134,10 @@ namespace pxg {
       constexpr float minRadius = 0.5f * 87.78f;
       constexpr float minRadius2T4 = 4.f * minRadius * minRadius;
       constexpr float i2p = 3.14159265358979323846 / 32768;
-      const HitAttr a = H.attrPhi[k];
+      const PhiCand a = H.candPhi[k];
       if (a.detInd > 2000)
         return false;
-      const float4 po = H.posPhi[k];
-      const float zo = po.z, ro = po.w;
+      const float zo = a.z, ro = a.r;
       const float dr = ro - q.mer;
       if (dr > q.maxr || dr < 0 || fabsf(q.mez * ro - q.mer * zo) > z0cut * dr)
         return false;
@@ -411,12 +408,14 @@ namespace pxg {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
-    // kStage: staged candidate cells per wave (a launch-shape knob: 128, 256 or 512); the
-    // LDS it takes (about 22 bytes per entry and wave) sets the waves per SIMD
+    // kStage: staged candidate cells per wave (a launch-shape knob: 128, 256 or 512).  At
+    // 256 the LDS it takes (about 22 bytes per entry and wave) allows seven waves per SIMD,
+    // whose 72 registers per lane hold the kernel without spills; 128 keeps the same
+    // bound (eight waves would leave 64 registers and spill), 512 allows three
     constexpr uint32_t kSegs = 64;  // staged candidate ranges per wave
     template <uint32_t kStage>
     constexpr int connectWaves() {
-      return kStage <= 128 ? 8 : kStage <= 256 ? 7 : 3;
+      return kStage <= 256 ? 7 : 3;
     }
 
     template <uint32_t kStage>
@@ -581,13 +580,20 @@ namespace pxg {
                 append(sMe[w][gBase + b0 + __ffsll((unsigned long long)acc) - 1]);
             }
           } else {
-#pragma unroll
+            // the ranges again (read anew rather than held in registers through the
+            // staging: a group that does not fit the staging area is rare)
             for (int q = 0; q < 6; ++q) {
+              const int pr = kPairsFromLayer[lo][q];
+              if (pr < 0)
+                break;
+              const uint32_t j = (pr ? T.cum[pr - 1] : 0) + (hm - T.layer[lo]);
+              const uint32_t rb = dOff[j], re = min(dOff[j + 1], nCells);
+              const uint32_t rl = re > rb ? re - rb : 0;
               // chunks of up to 64 candidates: the alignment first, in a loop without
               // stores (so the candidates' loads are issued together), then the circle
               // cut on the aligned ones, then the accepted ones in ascending order
-              for (uint32_t b0 = 0; b0 < len[q]; b0 += 64) {
-                const uint32_t m = min(64u, len[q] - b0), base = beg[q] + b0;
+              for (uint32_t b0 = 0; b0 < rl; b0 += 64) {
+                const uint32_t m = min(64u, rl - b0), base = rb + b0;
                 uint64_t al = 0;
                 for (uint32_t k = 0; k < m; ++k)
                   if (aligned(H.pos[ca.cellOuter[cb + base + k]]))

@@ -15,6 +15,9 @@
 //                       with pointer jumping through a per-wave LDS row until
 //                       a ballot sees no change, roots ranked with popcounts.
 //                       No workgroup barrier, no atomics but the charge sums.
+//                       The surviving clusters' extents and charges, which the
+//                       rec-hit stage needs, are accumulated in LDS on the way
+//                       and written once per cluster (ClusterExt).
 //                       Larger runs are appended to a work list.
 //   clusterBigKernel    persistent workgroups draining that list: pixels
 //                       staged in LDS, column histogram, lock-free union-find
@@ -269,9 +272,13 @@ namespace pxg {
   // two halves on two runs at once).  Every lane of the wavefront calls this together:
   // the propagation loop ends on a wavefront-wide ballot.  `live`: this group has a run.
   namespace {
+    // per-wavefront LDS of the cluster extents: 64 clusters of each of the 9 fields
+    enum : int { kMinRow, kMaxRow, kMinCol, kMaxCol, kQsum, kQfx, kQlx, kQfy, kQly, kExtFields };
+
     template <int G>
     __device__ void clusterRunLanes(const Pool& p, uint32_t e, uint32_t k, uint32_t first, uint32_t n,
-                                    const EventDesc& ev, int32_t* rowWave, uint32_t base, bool live) {
+                                    const EventDesc& ev, int32_t* rowWave, uint32_t (*extWave)[64], uint32_t base,
+                                    bool live) {
       const uint32_t lane = (threadIdx.x & 63) - base;
       constexpr uint64_t kGroupMask = G == 64 ? ~0ull : (1ull << G) - 1;
       auto ballot = [&](bool v) { return (__ballot(v) >> base) & kGroupMask; };
@@ -352,7 +359,64 @@ namespace pxg {
         p.clusInModule[size_t(e) * kMaxModuleRanges + me] = renumber ? kept : nclus;
         p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
       }
-      waveSync();  // the row is reused by the next run of the wavefront
+      // Each surviving cluster's extent and charges for the rec-hit stage (the pool of
+      // the no-allocation entry point has no room for them, and no rec-hit stage):
+      // LDS atomics over the pixels, exact in any order, then one lane per cluster
+      // writes the record at the run's first digi + cluster id.
+      if (p.clExt) {
+        const uint32_t nf = renumber ? kept : nclus;
+        int32_t fid = -1;  // the pixel's final cluster id
+        if (act && valid) {
+          if (!renumber)
+            fid = int32_t(cid);
+          else if ((okRoots >> label) & 1)
+            fid = int32_t(__popcll(okRoots & lanesBelow(label)));
+        }
+        const uint32_t slot = base + lane;
+        if (live && lane < nf) {
+          extWave[kMinRow][slot] = extWave[kMinCol][slot] = 0xffffffffu;
+          extWave[kMaxRow][slot] = extWave[kMaxCol][slot] = 0;
+          extWave[kQsum][slot] = extWave[kQfx][slot] = extWave[kQlx][slot] = 0;
+          extWave[kQfy][slot] = extWave[kQly][slot] = 0;
+        }
+        waveSync();
+        const uint32_t f = base + uint32_t(fid);
+        if (fid >= 0) {
+          atomicMin(&extWave[kMinRow][f], x);
+          atomicMax(&extWave[kMaxRow][f], x);
+          atomicMin(&extWave[kMinCol][f], y);
+          atomicMax(&extWave[kMaxCol][f], y);
+        }
+        waveSync();
+        if (fid >= 0) {
+          const uint32_t q = uint32_t(adc);
+          atomicAdd(&extWave[kQsum][f], q);
+          if (extWave[kMinRow][f] == x)
+            atomicAdd(&extWave[kQfx][f], q);
+          if (extWave[kMaxRow][f] == x)
+            atomicAdd(&extWave[kQlx][f], q);
+          if (extWave[kMinCol][f] == y)
+            atomicAdd(&extWave[kQfy][f], q);
+          if (extWave[kMaxCol][f] == y)
+            atomicAdd(&extWave[kQly][f], q);
+        }
+        waveSync();
+        if (live && lane < nf) {
+          ClusterExt c;
+          c.minRow = uint16_t(extWave[kMinRow][slot]);
+          c.maxRow = uint16_t(extWave[kMaxRow][slot]);
+          c.minCol = uint16_t(extWave[kMinCol][slot]);
+          c.maxCol = uint16_t(extWave[kMaxCol][slot]);
+          c.qsum = int32_t(extWave[kQsum][slot]);
+          c.qfx = int32_t(extWave[kQfx][slot]);
+          c.qlx = int32_t(extWave[kQlx][slot]);
+          c.qfy = int32_t(extWave[kQfy][slot]);
+          c.qly = int32_t(extWave[kQly][slot]);
+          c.pad = 0;
+          p.clExt[wo + first + lane] = c;
+        }
+      }
+      waveSync();  // the row and the extents are reused by the next run of the wavefront
     }
   }  // namespace
 
@@ -363,6 +427,7 @@ namespace pxg {
   template <uint32_t kRuns>
   __global__ void __launch_bounds__(kBlock) clusterSmallKernel(Pool p) {
     __shared__ int32_t rowLds[kWaves][64];
+    __shared__ uint32_t extLds[kWaves][kExtFields][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const LogicalBlock lb = logicalBlock();
     const uint32_t k0 = kRuns * (lb.x * kWaves + wave);
@@ -382,6 +447,7 @@ namespace pxg {
     uint32_t f0 = 0, f1 = 0;
     const uint32_t n0 = runOf(k0, f0), n1 = kRuns > 1 ? runOf(k0 + 1, f1) : 0;
     int32_t* row = rowLds[wave];
+    uint32_t(*ext)[64] = extLds[wave];
     for (uint32_t u = 0; u < 2; ++u) {
       const uint32_t n = u ? n1 : n0;
       if (n > 64 && lane == 0) {
@@ -391,11 +457,11 @@ namespace pxg {
     }
     if (n0 <= 32 && n1 <= 32) {
       const uint32_t half = lane >> 5;
-      clusterRunLanes<32>(p, e, k0 + half, half ? f1 : f0, half ? n1 : n0, ev, row, 32 * half,
+      clusterRunLanes<32>(p, e, k0 + half, half ? f1 : f0, half ? n1 : n0, ev, row, ext, 32 * half,
                           (half ? n1 : n0) > 0);
     } else {
-      clusterRunLanes<64>(p, e, k0, f0, n0, ev, row, 0, n0 > 0 && n0 <= 64);
-      clusterRunLanes<64>(p, e, k0 + 1, f1, n1, ev, row, 0, n1 > 0 && n1 <= 64);
+      clusterRunLanes<64>(p, e, k0, f0, n0, ev, row, ext, 0, n0 > 0 && n0 <= 64);
+      clusterRunLanes<64>(p, e, k0 + 1, f1, n1, ev, row, ext, 0, n1 > 0 && n1 <= 64);
     }
   }
 

@@ -4,11 +4,13 @@
 //                   holds ~30 digis and ~7 clusters at PU50, so a wavefront per
 //                   run would leave most lanes idle and make four times as many
 //                   latency chains), 64 for one event (more workgroups to fill
-//                   the device): cluster extents and integer charges
-//                   accumulated with LDS atomics (exact in any order), G
-//                   clusters per pass, then one cluster per lane for the
-//                   cluster position estimate and the local -> global
-//                   transform.  The float
+//                   the device), one cluster per lane for the cluster position
+//                   estimate and the local -> global transform.  A run of at
+//                   most 64 digis (nearly all at PU50) takes its clusters'
+//                   extents and integer charges from the records the clustering
+//                   wrote (ClusterExt); a larger run accumulates them from its
+//                   digis with LDS atomics (exact in any order), G clusters per
+//                   pass.  The float
 //                   arithmetic keeps the serial program's operation order and
 //                   the library is compiled without FMA contraction, so the
 //                   positions are bit-identical to the serial ones.
@@ -18,9 +20,10 @@
 //                   counters, which lists every bin in decreasing hit index;
 //                   each wavefront reproduces that order deterministically by
 //                   ranking equal bins inside a 64-hit chunk with 7 ballots.
-//                   Beside each hit index it writes the hit's packed position
-//                   and attributes (posPhi, attrPhi), so the doublet search
-//                   reads a phi window's candidates from contiguous memory.
+//                   Beside each hit index it writes what the doublet search
+//                   tests of the hit (candPhi: z, r, iphi, ysize, detInd in 16
+//                   bytes), so that search reads a phi window's candidates
+//                   from contiguous memory, one load each.
 //
 // Reference behaviour: plugin-SiPixelRecHits/gpuPixelRecHits.h:16-207,
 // CondFormats/pixelCPEforGPU.h:106-340, DataFormats/SOARotation.h:55-94,
@@ -136,12 +139,103 @@ namespace pxg {
       return;
     const uint32_t first = p.moduleStart[mo + 1 + k];
     const uint32_t end = (k + 1 < nModules) ? p.moduleStart[mo + 2 + k] : ev.nWords;
-    const uint32_t wo = ev.wordOffset;
     const uint32_t hitBase = p.clusModuleStart[mo + me];
     const size_t ho = size_t(e) * kMaxNumClusters;
     const CPEDetParams& det = c.cpeDet[me];
     const CPECommonParams com = *c.cpeCommon;
 
+    // the hit of the run's cluster ic from its extent and charges
+    auto emit = [&](uint32_t ic, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1, int32_t fx, int32_t lx,
+                    int32_t fy, int32_t ly, int32_t qs) {
+      const uint32_t h = hitBase + ic;
+      if (h >= kMaxNumClusters)
+        return;
+      // position (pixelCPEforGPU.h:176-259)
+      const uint32_t llxl = unitX(r0 + 1), llyl = unitY(c0 + 1), urxl = unitX(r1), uryl = unitY(c1);
+      const int mx = int(llxl + urxl), my = int(llyl + uryl);
+      int xsize = int(urxl) + 2 - int(llxl);
+      int ysize = int(uryl) + 2 - int(llyl);
+      if (bigX(r0))
+        ++xsize;
+      if (bigX(r1))
+        ++xsize;
+      if (bigY(c0))
+        ++ysize;
+      if (bigY(c1))
+        ++ysize;
+      const int unbX = int(8. * double(fabsf(float(fx - lx))) / double(float(fx + lx)));
+      const int unbY = int(8. * double(fabsf(float(fy - ly))) / double(float(fy + ly)));
+      xsize = 8 * xsize - unbX;
+      ysize = 8 * ysize - unbY;
+      int16_t sx = int16_t(min(xsize, 1023));
+      int16_t sy = int16_t(min(ysize, 1023));
+      if (r0 == 0 || r1 == kLastRow)
+        sx = int16_t(-sx);
+      if (c0 == 0 || c1 == kLastCol)
+        sy = int16_t(-sy);
+      const float xPos = det.shiftX + com.thePitchX * (0.5f * float(mx) + float(kXOffset));
+      const float yPos = det.shiftY + com.thePitchY * (0.5f * float(my) + float(kYOffset));
+      const float gvz = -1.f / det.z0;
+      const float cotalpha = (xPos - det.x0) * gvz;
+      const float cotbeta = (yPos - det.y0) * gvz;
+      const float thickness = det.isBarrel ? com.theThicknessB : com.theThicknessE;
+      const float xcorr = edgeCorrection(int(r1) - int(r0), fx, lx, llxl, urxl, det.chargeWidthX, thickness, cotalpha,
+                                         com.thePitchX, bigX(r0), bigX(r1));
+      const float ycorr = edgeCorrection(int(c1) - int(c0), fy, ly, llyl, uryl, det.chargeWidthY, thickness, cotbeta,
+                                         com.thePitchY, bigY(c0), bigY(c1));
+      const float xl = xPos + xcorr;
+      const float yl = yPos + ycorr;
+      // errors (pixelCPEforGPU.h:316-340)
+      const uint32_t dsx = r1 - r0, dsy = c1 - c0;
+      float xerr = 0.0050f, yerr = 0.0085f;
+      const uint32_t ix = (dsx == 0) + ((dsx == 0) && bigX(r0));
+      const uint32_t iy = (dsy == 0) + ((dsy == 0) && bigY(c0));
+      if (!(r0 == 0 || r1 == kLastRow))
+        xerr = det.sx[ix];
+      if (!(c0 == 0 || c1 == kLastCol))
+        yerr = det.sy[iy];
+      // local -> global, beam spot subtracted
+      const SOAFrameF& f = det.frame;
+      float xg = f.R11 * xl + f.R21 * yl;
+      float yg = f.R12 * xl + f.R22 * yl;
+      float zg = f.R13 * xl + f.R23 * yl;
+      xg += f.px;
+      yg += f.py;
+      zg += f.pz;
+      xg -= bs.x;
+      yg -= bs.y;
+      zg -= bs.z;
+      const size_t o = ho + h;
+      p.charge[o] = qs;
+      p.detInd[o] = uint16_t(me);
+      p.xl[o] = xl;
+      p.yl[o] = yl;
+      p.xsize[o] = sx;
+      p.ysize[o] = sy;
+      p.xerr[o] = xerr * xerr;
+      p.yerr[o] = yerr * yerr;
+      p.xg[o] = xg;
+      p.yg[o] = yg;
+      p.zg[o] = zg;
+      const float rg = sqrtf(xg * xg + yg * yg);
+      p.rg[o] = rg;
+      p.pos[o] = make_float4(xg, yg, zg, rg);
+      const int16_t iphi = atan2s7(yg, xg);
+      p.iphi[o] = iphi;
+      p.attr[o] = HitAttr{iphi, int16_t(sy), uint16_t(me), 0};
+    };
+
+    const uint32_t wo = ev.wordOffset;
+    if (end - first <= 64 && p.clExt) {
+      // a run of at most 64 digis: the clustering recorded its clusters' extents
+      const ClusterExt* ext = p.clExt + wo + first;
+      for (uint32_t ic = lane; ic < nclus; ic += kGroup) {
+        const ClusterExt x = ext[ic];
+        emit(ic, x.minRow, x.maxRow, x.minCol, x.maxCol, x.qfx, x.qlx, x.qfy, x.qly, x.qsum);
+      }
+      return;
+    }
+    // a larger run: cluster extents and charges from the digis, G clusters per pass
     __shared__ uint32_t sMinRow[kRuns][kChunk], sMaxRow[kRuns][kChunk], sMinCol[kRuns][kChunk],
         sMaxCol[kRuns][kChunk];
     __shared__ int32_t sQsum[kRuns][kChunk], sQfx[kRuns][kChunk], sQlx[kRuns][kChunk], sQfy[kRuns][kChunk],
@@ -194,84 +288,10 @@ namespace pxg {
       }
       waveSync();
       for (uint32_t ic = lane; ic < nIn; ic += kGroup) {
-        const uint32_t h = hitBase + start + ic;
-        if (h >= kMaxNumClusters)
+        if (hitBase + start + ic >= kMaxNumClusters)
           break;
-        const uint32_t r0 = minRow[ic], r1 = maxRow[ic], c0 = minCol[ic], c1 = maxCol[ic];
-        const int32_t fx = qfx[ic], lx = qlx[ic], fy = qfy[ic], ly = qly[ic];
-        // position (pixelCPEforGPU.h:176-259)
-        const uint32_t llxl = unitX(r0 + 1), llyl = unitY(c0 + 1), urxl = unitX(r1), uryl = unitY(c1);
-        const int mx = int(llxl + urxl), my = int(llyl + uryl);
-        int xsize = int(urxl) + 2 - int(llxl);
-        int ysize = int(uryl) + 2 - int(llyl);
-        if (bigX(r0))
-          ++xsize;
-        if (bigX(r1))
-          ++xsize;
-        if (bigY(c0))
-          ++ysize;
-        if (bigY(c1))
-          ++ysize;
-        const int unbX = int(8. * double(fabsf(float(fx - lx))) / double(float(fx + lx)));
-        const int unbY = int(8. * double(fabsf(float(fy - ly))) / double(float(fy + ly)));
-        xsize = 8 * xsize - unbX;
-        ysize = 8 * ysize - unbY;
-        int16_t sx = int16_t(min(xsize, 1023));
-        int16_t sy = int16_t(min(ysize, 1023));
-        if (r0 == 0 || r1 == kLastRow)
-          sx = int16_t(-sx);
-        if (c0 == 0 || c1 == kLastCol)
-          sy = int16_t(-sy);
-        const float xPos = det.shiftX + com.thePitchX * (0.5f * float(mx) + float(kXOffset));
-        const float yPos = det.shiftY + com.thePitchY * (0.5f * float(my) + float(kYOffset));
-        const float gvz = -1.f / det.z0;
-        const float cotalpha = (xPos - det.x0) * gvz;
-        const float cotbeta = (yPos - det.y0) * gvz;
-        const float thickness = det.isBarrel ? com.theThicknessB : com.theThicknessE;
-        const float xcorr = edgeCorrection(int(r1) - int(r0), fx, lx, llxl, urxl, det.chargeWidthX, thickness, cotalpha,
-                                           com.thePitchX, bigX(r0), bigX(r1));
-        const float ycorr = edgeCorrection(int(c1) - int(c0), fy, ly, llyl, uryl, det.chargeWidthY, thickness, cotbeta,
-                                           com.thePitchY, bigY(c0), bigY(c1));
-        const float xl = xPos + xcorr;
-        const float yl = yPos + ycorr;
-        // errors (pixelCPEforGPU.h:316-340)
-        const uint32_t dsx = r1 - r0, dsy = c1 - c0;
-        float xerr = 0.0050f, yerr = 0.0085f;
-        const uint32_t ix = (dsx == 0) + ((dsx == 0) && bigX(r0));
-        const uint32_t iy = (dsy == 0) + ((dsy == 0) && bigY(c0));
-        if (!(r0 == 0 || r1 == kLastRow))
-          xerr = det.sx[ix];
-        if (!(c0 == 0 || c1 == kLastCol))
-          yerr = det.sy[iy];
-        // local -> global, beam spot subtracted
-        const SOAFrameF& f = det.frame;
-        float xg = f.R11 * xl + f.R21 * yl;
-        float yg = f.R12 * xl + f.R22 * yl;
-        float zg = f.R13 * xl + f.R23 * yl;
-        xg += f.px;
-        yg += f.py;
-        zg += f.pz;
-        xg -= bs.x;
-        yg -= bs.y;
-        zg -= bs.z;
-        const size_t o = ho + h;
-        p.charge[o] = qsum[ic];
-        p.detInd[o] = uint16_t(me);
-        p.xl[o] = xl;
-        p.yl[o] = yl;
-        p.xsize[o] = sx;
-        p.ysize[o] = sy;
-        p.xerr[o] = xerr * xerr;
-        p.yerr[o] = yerr * yerr;
-        p.xg[o] = xg;
-        p.yg[o] = yg;
-        p.zg[o] = zg;
-        const float rg = sqrtf(xg * xg + yg * yg);
-        p.rg[o] = rg;
-        p.pos[o] = make_float4(xg, yg, zg, rg);
-        const int16_t iphi = atan2s7(yg, xg);
-        p.iphi[o] = iphi;
-        p.attr[o] = HitAttr{iphi, int16_t(sy), uint16_t(me), 0};
+        emit(start + ic, minRow[ic], maxRow[ic], minCol[ic], maxCol[ic], qfx[ic], qlx[ic], qfy[ic], qly[ic],
+             qsum[ic]);
       }
       waveSync();
     }
@@ -292,8 +312,7 @@ namespace pxg {
     const int16_t* iphi = p.iphi + size_t(e) * kMaxNumClusters;
     const float4* pos = p.pos + size_t(e) * kMaxNumClusters;
     const HitAttr* attr = p.attr + size_t(e) * kMaxNumClusters;
-    float4* posPhi = p.posPhi + size_t(e) * kMaxNumClusters;
-    HitAttr* attrPhi = p.attrPhi + size_t(e) * kMaxNumClusters;
+    PhiCand* candPhi = p.candPhi + size_t(e) * kMaxNumClusters;
 
     __shared__ uint32_t ls[kNumLayers + 1];
     __shared__ uint32_t cnt[kPhiHistTotBins + 1];
@@ -374,8 +393,7 @@ namespace pxg {
       if (act) {
         const uint32_t k = cnt[bin] - 1 - (before + rank);
         bins[k] = uint16_t(i);
-        posPhi[k] = pv;
-        attrPhi[k] = av;
+        candPhi[k] = PhiCand{pv.z, pv.w, av.iphi, av.ysize, av.detInd, 0};
         if (rank == group - 1)
           taken[bin] = before + group;
       }

@@ -72,6 +72,7 @@ namespace pxg::detail {
     f(p.boundaryMask, 2 * (W / 64));
     f(p.errorsSparse, W);
     f(p.errors, W);
+    f(p.clExt, W);
     f(p.work, 64);
     f(p.bigList, M);
     f(p.moduleStart, M);
@@ -88,8 +89,7 @@ namespace pxg::detail {
     f(p.rg, H);
     f(p.pos, H);
     f(p.attr, H);
-    f(p.posPhi, H);
-    f(p.attrPhi, H);
+    f(p.candPhi, H);
     f(p.iphi, H);
     f(p.charge, H);
     f(p.xsize, H);
@@ -199,8 +199,7 @@ namespace pxg::detail {
     v.phiBins = p.phiBins + h;
     v.pos = p.pos + h;
     v.attr = p.attr + h;
-    v.posPhi = p.posPhi + h;
-    v.attrPhi = p.attrPhi + h;
+    v.candPhi = p.candPhi + h;
     v.hitsLayerStart = p.hitsLayerStart + size_t(first) * kLayerStartStride;
     v.phiOff = p.phiOff + size_t(first) * kPhiOffStride;
     v.avg = p.avg + first;

@@ -71,6 +71,22 @@ namespace pxg {
     uint16_t detInd, pad;
   };
 
+  // What the doublet search reads of a candidate outer hit, in one 16-byte load
+  // (the phi histogram's copy of the hit: z, r, iphi, ysize, detInd)
+  struct alignas(16) PhiCand {
+    float z, r;
+    int16_t iphi, ysize;
+    uint16_t detInd, pad;
+  };
+
+  // A cluster's pixel extent and integer charges, as the rec-hit position estimate
+  // uses them (gpuPixelRecHits.h:73-140): the charge sum and the charges on its first
+  // and last row and column.
+  struct alignas(16) ClusterExt {
+    uint16_t minRow, maxRow, minCol, maxCol;
+    int32_t qsum, qfx, qlx, qfy, qly, pad;
+  };
+
   // Device pointers of one stream's pool (batch capacity fixed at creation).
   struct Pool {
     uint32_t maxEvents, maxWords;
@@ -86,6 +102,9 @@ namespace pxg {
     uint64_t* boundaryMask;  // [maxWords / 64]
     PixelErrorCompact* errorsSparse;  // [maxWords], error records at their word index
     PixelErrorCompact* errors;        // event e: errors[wordOffset[e] ...], word order
+    // [maxWords]: the clusters of a module run of at most 64 digis, found by the
+    // clustering, at the run's first digi + cluster id
+    ClusterExt* clExt;
     uint32_t* work;     // [0] large module runs listed, [1] next one to take
     uint32_t* bigList;  // large module runs of the launch: event << 16 | run
     // clusters, per event
@@ -101,8 +120,7 @@ namespace pxg {
     uint32_t* hitsLayerStart;  // stride 16
     uint32_t* phiOff;          // stride kPhiHistTotBins (padded to 1288)
     uint16_t* phiBins;         // stride kMaxNumClusters
-    float4* posPhi;            // pos of phiBins[k] at k: a phi window's candidates in one contiguous range
-    HitAttr* attrPhi;          // attr of phiBins[k] at k
+    PhiCand* candPhi;          // hit phiBins[k] at k: a phi window's candidates in one contiguous range
     AverageGeometry* avg;      // per event
   };
 
