@@ -74,6 +74,7 @@ KERNEL_LABEL = {
     "clusterBigKernel": "clusterize", "clusStartKernel": "clusterize", "hitsKernel": "hits",
     "phiBinnerKernel": "phiBinner", "doubletCountKernel": "doubletCount", "doubletScanKernel": "doubletScan",
     "doubletFillKernel": "doubletFill", "ohScanKernel": "outerHitScan", "ohFillKernel": "outerHitFill",
+    "ohBuildKernel": "outerHitFill",
     "connectKernel": "connect", "fishboneKernel": "fishbone", "startMaskKernel": "startMask",
     "startCompactKernel": "startCompact", "ntupletCountKernel": "ntupletCount", "ntupletScanKernel": "ntupletScan",
     "ntupletFillKernel": "ntupletFill", "trackListAllocKernel": "trackListAlloc",

@@ -266,7 +266,10 @@ namespace pxg {
       }
     }
 
-    template <int G>
+    // kRank: each cell takes its place in its outer hit's list with an atomic on the
+    // hit's counter (the lists are then built by ohScanKernel and ohFillKernel); without
+    // it, ohBuildKernel builds the lists per event in LDS
+    template <int G, bool kRank>
     __global__ void __launch_bounds__(kCaBlock) doubletFillKernel(Pool p, CAPool ca) {
       const uint32_t e = logicalBlock().y;
       __shared__ PairTable T;
@@ -291,7 +294,8 @@ namespace pxg {
           ca.cellInner[cb + c] = uint16_t(q.i);
           ca.cellOuter[cb + c] = uint16_t(oi);
           ca.cellPair[cb + c] = uint8_t(q.pairId);
-          ca.ohRank[cb + c] = atomicAdd(&ohc[oi], 1u);
+          if (kRank)
+            ca.ohRank[cb + c] = atomicAdd(&ohc[oi], 1u);
         };
         // the first 128 candidates: the count pass recorded their verdicts; the
         // r-th accepted one is the slot's r-th cell, written by lane r % G
@@ -368,6 +372,86 @@ namespace pxg {
           const uint32_t c = c0 + u * stride;
           if (c < nCells) {
             ca.ohCells[cb + off[h[u]] + pos[u]] = make_uint2(c, hi[u] | h[u] << 16);
+            ca.cellFlags[cb + c] = 0;
+            ca.nbSlot[cb + c] = -1;
+            ca.nbCount[cb + c] = 0;
+            ca.trSlot[cb + c] = -1;
+            ca.trCount[cb + c] = 0;
+          }
+        }
+      }
+    }
+
+    // The outer-hit lists of one event by one workgroup, with the counts in LDS: a global
+    // atomic per cell costs ~48 ns of device throughput at scattered addresses on this
+    // chip, an LDS atomic a few cycles of one CU.  Counts per hit are 16 bits, two to a
+    // word (a hit's cells come from the inner hits of at most the 48k of the event, so
+    // no count reaches 65536): the counts, a workgroup scan into the offsets, then each
+    // cell at its hit's offset plus its rank, taken with a second round of LDS atomics
+    // (the consumers do not depend on the order within a list).  The pass over the
+    // cells also clears their state for the later stages.
+    __global__ void __launch_bounds__(kEventBlock) ohBuildKernel(Pool p, CAPool ca) {
+      const uint32_t e = blockIdx.x;
+      __shared__ uint32_t hist[kMaxNumClusters / 2];
+      __shared__ uint32_t lds[kEventBlock / 64];
+      const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
+      const uint32_t nCells = ca.cnt[e].nCells;
+      const size_t cb = ca.cellBase(e);
+      uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
+      const uint32_t nWords = (nHits + 1) / 2;
+      for (uint32_t w = threadIdx.x; w < nWords; w += kEventBlock)
+        hist[w] = 0;
+      __syncthreads();
+      constexpr uint32_t kU = 4;  // cells per thread and iteration: their loads are issued together
+      for (uint32_t c0 = threadIdx.x; c0 < nCells; c0 += kU * kEventBlock) {
+        uint32_t h[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+          const uint32_t c = c0 + u * kEventBlock;
+          h[u] = c < nCells ? ca.cellOuter[cb + c] : 0xffffffffu;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u)
+          if (h[u] != 0xffffffffu)
+            atomicAdd(&hist[h[u] >> 1], 1u << ((h[u] & 1) * 16));
+      }
+      __syncthreads();
+      // offsets: each thread scans an even-sized run of hits (whole words), the runs in order
+      const uint32_t per = 2 * ((nWords + kEventBlock - 1) / kEventBlock);
+      const uint32_t h0 = min(threadIdx.x * per, nHits), h1 = min(h0 + per, nHits);
+      uint32_t sum = 0;
+      for (uint32_t h = h0; h < h1; ++h)
+        sum += (hist[h >> 1] >> ((h & 1) * 16)) & 0xffffu;
+      uint32_t total;
+      uint32_t run = wgExclusiveScan<kEventBlock>(sum, total, lds);
+      for (uint32_t h = h0; h < h1; ++h) {
+        off[h] = run;
+        run += (hist[h >> 1] >> ((h & 1) * 16)) & 0xffffu;
+      }
+      if (threadIdx.x == 0)
+        off[nHits] = total;
+      __syncthreads();
+      for (uint32_t w = h0 / 2; w < (h1 + 1) / 2; ++w)
+        hist[w] = 0;  // the counts become ranks
+      __syncthreads();
+      for (uint32_t c0 = threadIdx.x; c0 < nCells; c0 += kU * kEventBlock) {
+        uint32_t h[kU], hi[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+          const uint32_t c = c0 + u * kEventBlock;
+          h[u] = hi[u] = 0;
+          if (c < nCells) {
+            h[u] = ca.cellOuter[cb + c];
+            hi[u] = ca.cellInner[cb + c];
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+          const uint32_t c = c0 + u * kEventBlock;
+          if (c < nCells) {
+            const uint32_t sh = (h[u] & 1) * 16;
+            const uint32_t rank = (atomicAdd(&hist[h[u] >> 1], 1u << sh) >> sh) & 0xffffu;
+            ca.ohCells[cb + off[h[u]] + rank] = make_uint2(c, hi[u] | h[u] << 16);
             ca.cellFlags[cb + c] = 0;
             ca.nbSlot[cb + c] = -1;
             ca.nbCount[cb + c] = 0;
@@ -1207,12 +1291,22 @@ namespace pxg {
     l.t->e();                                    \
   } while (0)
 
+  // the doublets and the outer-hit lists: per event in LDS for batches of 8 or more
+  // events (one workgroup per event fills the device), with per-cell atomics across
+  // the grid for smaller ones
   template <int G>
   void doubletKernels(const CALaunch& l) {
     const dim3 g(l.blocks, l.nEvents);
     PXG_K("doubletCount", doubletCountKernel<G>, g, kCaBlock, *l.p, *l.ca);
     PXG_K("doubletScan", doubletScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
-    PXG_K("doubletFill", doubletFillKernel<G>, g, kCaBlock, *l.p, *l.ca);
+    if (l.nEvents >= kListsInLdsMinEvents) {
+      PXG_K("doubletFill", (doubletFillKernel<G, false>), g, kCaBlock, *l.p, *l.ca);
+      PXG_K("outerHitFill", ohBuildKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
+    } else {
+      PXG_K("doubletFill", (doubletFillKernel<G, true>), g, kCaBlock, *l.p, *l.ca);
+      PXG_K("outerHitScan", ohScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
+      PXG_K("outerHitFill", ohFillKernel, g, kCaBlock, *l.ca);
+    }
   }
 
   void launchDoublets(const CALaunch& l) {
@@ -1225,12 +1319,6 @@ namespace pxg {
       case 32: doubletKernels<32>(l); break;
       default: doubletKernels<64>(l); break;
     }
-  }
-
-  void launchOuterHitLists(const CALaunch& l) {
-    const dim3 g(l.blocks, l.nEvents);
-    PXG_K("outerHitScan", ohScanKernel, l.nEvents, kEventBlock, *l.p, *l.ca);
-    PXG_K("outerHitFill", ohFillKernel, g, kCaBlock, *l.ca);
   }
 
   void launchConnect(const CALaunch& l) {

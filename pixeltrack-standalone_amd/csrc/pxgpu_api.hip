@@ -348,7 +348,6 @@ namespace pxg::detail {
                      s->connectStage, s->fishboneWindow};
     if (from <= 4 && upto >= 4) {
       launchDoublets(l);
-      launchOuterHitLists(l);
       launchConnect(l);
       if (l.earlyFishbone)
         launchFishbone(l, false);

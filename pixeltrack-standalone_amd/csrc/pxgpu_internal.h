@@ -133,6 +133,7 @@ namespace pxg {
   constexpr uint32_t kHitStride = kMaxNumClusters + 64;      // per-event stride of per-hit offset arrays
   constexpr uint32_t kCaBlock = 256;                         // workgroup of the grid-stride CA kernels
   constexpr uint32_t kEventBlock = 1024;                     // workgroup of the per-event kernels
+  constexpr uint32_t kListsInLdsMinEvents = 8;               // batches this large build lists per event in LDS
   // vertex clustering algorithms (gpuVertexFinder.h:46-60; density is the reference's default)
   enum : int { kVertexDensity = 0, kVertexDBSCAN = 1, kVertexIterative = 2 };
   constexpr uint32_t kDupPredCap = 6;                        // predecessors kept per cell for the duplicate remover
@@ -276,7 +277,6 @@ namespace pxg {
     uint32_t fishboneWindow;  // list entries the fishbone stages per wave in LDS: 128, 256 or 384
   };
   void launchDoublets(const CALaunch& l);
-  void launchOuterHitLists(const CALaunch& l);
   void launchConnect(const CALaunch& l);
   void launchFishbone(const CALaunch& l, bool checkTrack);
   void launchNtuplets(const CALaunch& l);
