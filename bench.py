@@ -78,10 +78,12 @@ KERNEL_LABEL = {
     "connectKernel": "connect", "fishboneKernel": "fishbone", "startMaskKernel": "startMask",
     "startCompactKernel": "startCompact", "ntupletCountKernel": "ntupletCount", "ntupletScanKernel": "ntupletScan",
     "ntupletFillKernel": "ntupletFill", "trackListAllocKernel": "trackListAlloc",
-    "trackListFillKernel": "trackListFill", "trackListSortKernel": "trackListSort",
+    "trackListFillKernel": "trackListFill", "trackListBuildKernel": "trackListFill",
+    "trackListSortKernel": "trackListSort",
     "multiplicityKernel": "multiplicity", "fitKernel": "fit", "classifyKernel": "classify",
     "dupPredsKernel": "dupPreds", "fastDuplicateKernel": "fastDuplicate", "hitToTupleCountKernel": "hitToTupleCount",
     "hitToTupleScanKernel": "hitToTupleScan", "hitToTupleFillKernel": "hitToTupleFill",
+    "hitToTupleBuildKernel": "hitToTupleFill",
     "tripletCleanerKernel": "tripletCleaner", "vertexKernel": "vertices",
 }
 

@@ -125,7 +125,7 @@ typedef struct {
   uint16_t *cellInner, *cellOuter;
   uint8_t* cellPair;
   uint8_t* cellFlags;    /* bit0 listed in its outer hit's list, bit1 killed by the fishbone */
-  uint32_t* cellUsed;    /* one bit per cell */
+  uint8_t* cellUsed;     /* one byte per cell: 1 if the cell has a neighbour or is one */
   uint32_t* outerHitOff;
   uint32_t* outerHitCells;  /* cells by outer hit in fill order, as u32 pairs (cell, inner hit | outer hit << 16) */
   int32_t* neighborSlot; uint8_t* neighborCount; uint32_t* neighborPool;  /* 36 per slot */
@@ -192,7 +192,7 @@ int pxg_raw_to_cluster(pxg_stream* s, const uint32_t* words, const uint8_t* fedI
  * with PXG_ERR_CAPACITY.  Calls that overlap on one GPU share its scratch array and race:
  * serialise them.  Returns the status of pxg_raw_to_cluster. */
 #define PXG_NOALLOC_MAX_WORDS 300000u /* the words of one event at most (150 FEDs x 2000) */
-#define PXG_NOALLOC_SCRATCH_BYTES 3684352u
+#define PXG_NOALLOC_SCRATCH_BYTES 3684096u
 int pxg_raw_to_cluster_noalloc(pxg_context* ctx, void* hip_stream, const uint32_t* words, const uint8_t* fed_ids,
                                uint32_t n_words, pxg_digis* digis, pxg_clusters* clusters, pxg_errors* errors);
 

@@ -49,14 +49,6 @@ namespace pxg {
     __device__ __forceinline__ uint32_t gridStart() { return logicalBlock().x * blockDim.x + threadIdx.x; }
     __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
-    // increments the 16-bit counter i of a 4-byte aligned array through the 32-bit word
-    // holding it and returns its old value (no carry: the counts stay below 65536)
-    __device__ __forceinline__ uint32_t atomicIncHalf(uint16_t* base, uint32_t i) {
-      uint32_t* w = reinterpret_cast<uint32_t*>(base) + (i >> 1);
-      const uint32_t sh = (i & 1) * 16;
-      return (atomicAdd(w, 1u << sh) >> sh) & 0xffffu;
-    }
-
     // wave index within the event's workgroups, uniform over the wave
     __device__ __forceinline__ uint32_t waveStart() {
       return __builtin_amdgcn_readfirstlane(logicalBlock().x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -247,15 +239,9 @@ namespace pxg {
       const uint32_t nCells = min(total, ca.caps.maxDoublets);
       const uint32_t nHits = T.layer[10];
       // cleared for the following kernels
-      uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
-      for (uint32_t w = threadIdx.x; w < (nCells + 31) / 32; w += kEventBlock)
-        used[w] = 0;
       uint32_t* ohc = ca.ohCount + size_t(e) * kHitStride;
-      uint32_t* h2c = ca.h2tCount + size_t(e) * kHitStride;
-      for (uint32_t h = threadIdx.x; h < nHits + 1; h += kEventBlock) {
+      for (uint32_t h = threadIdx.x; h < nHits + 1; h += kEventBlock)
         ohc[h] = 0;
-        h2c[h] = 0;
-      }
       if (threadIdx.x == 0) {
         CACounts c{};
         c.nCells = nCells;
@@ -373,6 +359,7 @@ namespace pxg {
           if (c < nCells) {
             ca.ohCells[cb + off[h[u]] + pos[u]] = make_uint2(c, hi[u] | h[u] << 16);
             ca.cellFlags[cb + c] = 0;
+            ca.cellUsed[cb + c] = 0;
             ca.nbSlot[cb + c] = -1;
             ca.nbCount[cb + c] = 0;
             ca.trSlot[cb + c] = -1;
@@ -453,6 +440,7 @@ namespace pxg {
             const uint32_t rank = (atomicAdd(&hist[h[u] >> 1], 1u << sh) >> sh) & 0xffffu;
             ca.ohCells[cb + off[h[u]] + rank] = make_uint2(c, hi[u] | h[u] << 16);
             ca.cellFlags[cb + c] = 0;
+            ca.cellUsed[cb + c] = 0;
             ca.nbSlot[cb + c] = -1;
             ca.nbCount[cb + c] = 0;
             ca.trSlot[cb + c] = -1;
@@ -518,7 +506,7 @@ namespace pxg {
       const size_t cb = ca.cellBase(e);
       const uint32_t* dOff = ca.dOff + size_t(e) * (kDoubletSlots + 64);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
-      uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      uint8_t* used = ca.cellUsed + cb;
       uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
       const uint32_t lane = laneId(), w = threadIdx.x >> 6;
       uint32_t status = 0, stored = 0;
@@ -615,9 +603,10 @@ namespace pxg {
           uint32_t n = 0;
           // the accepted candidates, in ascending order within each range
           auto append = [&](uint32_t me) {
-            atomicOr(&used[me >> 5], 1u << (me & 31));
+            // a byte per cell: every writer stores the same value, no atomic needed
+            used[me] = 1;
             if (n == 0)
-              atomicOr(&used[oc >> 5], 1u << (oc & 31));
+              used[oc] = 1;
             if (n < kCaMaxNeighbors) {
               if (slot == -1) {
                 const uint32_t sl = waveAlloc(&ca.cnt[e].nbLists, true);
@@ -737,10 +726,10 @@ namespace pxg {
       const size_t cb = ca.cellBase(e);
       const uint32_t* off = ca.ohOff + size_t(e) * kHitStride;
       const uint2* cells = ca.ohCells + cb;
-      const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      const uint8_t* used = ca.cellUsed + cb;
       const uint16_t* trCount = ca.trCount + cb;
       auto isUsed = [&](uint32_t c) {
-        return ((used[c >> 5] >> (c & 31)) & 1) && (!kCheckTrack || trCount[c] > 0);
+        return used[c] && (!kCheckTrack || trCount[c] > 0);
       };
       // a list entry seen from its own list's hit
       auto entry = [&](uint32_t k) {
@@ -916,7 +905,6 @@ namespace pxg {
                 const uint32_t ck = cellAt(k);
                 bins[k] = ca.cellInner[cb + ck];
                 ca.tupleCells[4 * tb + 4 * size_t(m) + k] = ck;
-                atomicIncHalf(ca.trCount + cb, ck);
               }
               bins[depth] = ca.cellOuter[cb + cell];
               for (int k = depth; k < 4; ++k)
@@ -1041,95 +1029,110 @@ namespace pxg {
       }
     }
 
-    // HistoContainer.h:193-204 bulkFinalizeFill, and one track list slot per cell in a tuple
-    __global__ void __launch_bounds__(kCaBlock) trackListAllocKernel(CAPool ca) {
-      const uint32_t e = logicalBlock().y;
-      const CACounts cnt = ca.cnt[e];
+    // Per-cell track lists (the tuples each cell is in; CAHitNtupletGeneratorKernelsImpl.h:
+    // 118-149 fills them during the walk), built per event in LDS: workgroup (s, e) owns the
+    // cells [s * kTrSlice, (s + 1) * kTrSlice) of event e and passes twice over the event's
+    // tuples (about 16 bytes each, 10k at PU50): counting each of its cells' tuples with LDS
+    // atomics, then, once every cell with tuples has a slot of the list store (one counter
+    // update per workgroup, slots in cell order within it), writing each tuple into its
+    // cells' lists at the rank an LDS atomic gives (the sort below orders each list).
+    // Slot s records its cell and the cell's tuple count (the walk's scratch is free now),
+    // so the later passes visit only the cells that are in tuples.  Workgroup 0 of each
+    // event also finalises the tuple offsets (HistoContainer.h:193-204 bulkFinalizeFill).
+    constexpr uint32_t kTrSlice = 16384;  // cells per workgroup: 64 KB of counts
+
+    __global__ void __launch_bounds__(kEventBlock) trackListBuildKernel(CAPool ca) {
+      const uint32_t s = blockIdx.x, e = blockIdx.y;
+      __shared__ uint32_t cnt[kTrSlice];
+      __shared__ uint32_t lds[kEventBlock / 64];
+      __shared__ uint32_t sBase;
+      const CACounts cn = ca.cnt[e];
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
       const uint32_t M = ca.caps.maxTuples;
-      if (logicalBlock().x == 0) {
+      if (s == 0) {
         uint32_t* off = ca.tupleOff + ca.offBase(e);
-        if (cnt.apcM >= M) {
+        if (cn.apcM >= M) {
           if (threadIdx.x == 0)
             off[M] = off[M - 1];
         } else {
-          for (uint32_t i = cnt.apcM + threadIdx.x; i <= M; i += blockDim.x)
-            off[i] = cnt.apcN;
+          for (uint32_t i = cn.apcM + threadIdx.x; i <= M; i += kEventBlock)
+            off[i] = cn.apcN;
         }
       }
-      // slots are handed out compactly: slot s records its cell and the cell's
-      // tuple count (the walk's scratch is free now), so the later passes visit
-      // only the cells that are in tuples
+      const uint32_t c0 = s * kTrSlice;
+      if (c0 >= cn.nCells)
+        return;  // uniform per workgroup
+      const uint32_t n = min(kTrSlice, cn.nCells - c0);
+      const uint32_t nT = cn.nTuples;
+      const uint4* cells4 = reinterpret_cast<const uint4*>(ca.tupleCells + 4 * tb);
+      for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
+        cnt[i] = 0;
+      __syncthreads();
+      auto inSlice = [&](uint32_t c) { return c != 0xffffffffu && c - c0 < n; };
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        const uint4 c = cells4[t];
+        if (inSlice(c.x))
+          atomicAdd(&cnt[c.x - c0], 1u);
+        if (inSlice(c.y))
+          atomicAdd(&cnt[c.y - c0], 1u);
+        if (inSlice(c.z))
+          atomicAdd(&cnt[c.z - c0], 1u);
+        if (inSlice(c.w))
+          atomicAdd(&cnt[c.w - c0], 1u);
+      }
+      __syncthreads();
+      // slots: each thread takes a run of consecutive cells, the runs in order
+      const uint32_t per = (n + kEventBlock - 1) / kEventBlock;
+      const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
+      uint32_t mine = 0;
+      for (uint32_t i = i0; i < i1; ++i)
+        mine += cnt[i] > 0;
+      uint32_t total;
+      uint32_t rank = wgExclusiveScan<kEventBlock>(mine, total, lds);
+      if (threadIdx.x == 0)
+        sBase = total ? atomicAdd(&ca.cnt[e].trLists, total) : 0;
+      __syncthreads();
       uint32_t* trCells = ca.dfsT + cb;
       uint32_t* trTotal = ca.dfsH + cb;
       uint32_t status = 0;
-      // four cells per lane and iteration, one counter update per wave for all four
-      const uint32_t stride = gridStride();
-      for (uint32_t c0 = gridStart(); c0 < cnt.nCells; c0 += 4 * stride) {
-        uint32_t n[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t c = c0 + u * stride;
-          n[u] = c < cnt.nCells ? ca.trCount[cb + c] : 0;
-        }
-        uint64_t m[4];
-        uint32_t total = 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t c = c0 + u * stride;
-          if (c < cnt.nCells)
-            ca.trCount[cb + c] = 0;
-          m[u] = __ballot(n[u] > 0);
-          total += __popcll(m[u]);
-        }
-        if (total == 0)
+      for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t k = cnt[i];
+        if (k == 0)
           continue;
-        const uint32_t leader = __ffsll((unsigned long long)__ballot(true)) - 1;
-        uint32_t base = 0;
-        if (laneId() == leader)
-          base = atomicAdd(&ca.cnt[e].trLists, total);
-        base = __shfl(base, leader);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t s = base + __popcll(m[u] & lanesBelow());
-          base += __popcll(m[u]);
-          if (n[u] == 0)
-            continue;
-          const uint32_t c = c0 + u * stride;
-          if (s + 1 < ca.caps.maxCellLists) {
-            ca.trSlot[cb + c] = int32_t(s);
-            trCells[s] = c;
-            trTotal[s] = n[u];
-          } else {
-            ca.trSlot[cb + c] = -2;
-            status |= kCaTrackStoreOverflow;
-          }
+        const uint32_t slot = sBase + rank++;
+        const uint32_t c = c0 + i;
+        if (slot + 1 < ca.caps.maxCellLists) {
+          ca.trSlot[cb + c] = int32_t(slot);
+          trCells[slot] = c;
+          trTotal[slot] = k;
+        } else {
+          ca.trSlot[cb + c] = -2;
+          status |= kCaTrackStoreOverflow;
         }
+        cnt[i] = 0;  // the counts become ranks
+      }
+      __syncthreads();
+      TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
+      auto place = [&](uint32_t c, uint32_t t) {
+        if (!inSlice(c))
+          return;
+        const int32_t slot = ca.trSlot[cb + c];
+        if (slot < 0)
+          return;  // no list: the store is full
+        const uint32_t pos = atomicAdd(&cnt[c - c0], 1u);
+        if (pos < kCaMaxCellTracks)
+          pool[size_t(slot) * kCaMaxCellTracks + pos] = t;
+      };
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        const uint4 c = cells4[t];
+        place(c.x, t);
+        place(c.y, t);
+        place(c.z, t);
+        place(c.w, t);
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
-    }
-
-    __global__ void __launch_bounds__(kCaBlock) trackListFillKernel(CAPool ca) {
-      const uint32_t e = logicalBlock().y;
-      const uint32_t nT = ca.cnt[e].nTuples;
-      const size_t cb = ca.cellBase(e);
-      const size_t tb = ca.tupleBase(e);
-      TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
-      for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t c = ca.tupleCells[4 * tb + 4 * size_t(t) + k];
-          if (c == 0xffffffffu)
-            break;
-          const int32_t slot = ca.trSlot[cb + c];
-          if (slot < 0)
-            continue;  // no list: the store is full
-          const uint32_t pos = atomicIncHalf(ca.trCount + cb, c);
-          if (pos < kCaMaxCellTracks)
-            pool[size_t(slot) * kCaMaxCellTracks + pos] = t;
-        }
-      }
     }
 
     // sorts each cell's track list (ascending tuple index, first 48 kept) and
@@ -1358,8 +1361,8 @@ namespace pxg {
     PXG_K("ntupletCount", ntupletCountKernel, g, kCaBlock, *l.ca);
     PXG_K("ntupletScan", ntupletScanKernel, l.nEvents, kEventBlock, *l.ca);
     PXG_K("ntupletFill", ntupletFillKernel, g, kCaBlock, *l.ca);
-    PXG_K("trackListAlloc", trackListAllocKernel, g, kCaBlock, *l.ca);
-    PXG_K("trackListFill", trackListFillKernel, g, kCaBlock, *l.ca);
+    PXG_K("trackListFill", trackListBuildKernel, dim3((l.ca->caps.maxDoublets + kTrSlice - 1) / kTrSlice, l.nEvents),
+          kEventBlock, *l.ca);
     PXG_K("trackListSort", trackListSortKernel, g, kCaBlock, *l.ca);
   }
 

@@ -287,13 +287,13 @@ namespace pxg {
       const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
       const size_t cb = ca.cellBase(e);
       const size_t tb = ca.tupleBase(e);
-      const uint32_t* used = ca.cellUsed + size_t(e) * (ca.caps.maxDoublets / 32 + 64);
+      const uint8_t* used = ca.cellUsed + cb;
       const uint32_t* toff = ca.tupleOff + ca.offBase(e);
       const uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
       uint32_t good = 0, usedHits = 0, dupHits = 0, killed = 0, empty = 0;
       for (uint32_t c = gridStart(); c < cnt.nCells; c += gridStride()) {
         killed += (ca.cellFlags[cb + c] & 2) != 0;
-        empty += ((used[c >> 5] >> (c & 31)) & 1) == 0;
+        empty += used[c] == 0;
       }
       // the serial loop stops at the first empty tuple: only the last stored one can be
       for (uint32_t t = gridStart(); t < cnt.nTuples; t += gridStride())
@@ -322,49 +322,82 @@ namespace pxg {
       }
     }
 
-    // a22 hit -> tuple association of the loose tuples (HitToTuple, CAConstants.h:65): a
-    // one-to-many container over the event's hits, counted, scanned and filled
-    __device__ __forceinline__ OneToMany hitToTuple(const CAPool& ca, uint32_t e, uint32_t nHits) {
-      return {ca.h2tCount + size_t(e) * kHitStride, ca.h2tOff + size_t(e) * kHitStride, nHits,
-              ca.caps.maxHitToTuple};
-    }
+    // a22 hit -> tuple association of the loose tuples (HitToTuple, CAConstants.h:65), a
+    // one-to-many container over the event's hits with the reference's capacity, built in
+    // LDS: workgroup (s, e) owns the hits [s * kH2tSlice, (s + 1) * kH2tSlice) of event e
+    // and passes twice over the event's loose tuples.  The first pass counts its hits'
+    // tuples with LDS atomics and the entries of lower hits in a register, so that the
+    // offsets (clamped to the capacity, as the container's) need no other workgroup; the
+    // second writes each entry at its hit's offset plus the rank an LDS atomic gives (the
+    // triplet cleaner sorts each hit's list).  An entry past the capacity is dropped.
+    constexpr uint32_t kH2tSlice = 16384;  // hits per workgroup: 64 KB of counts
 
-    __global__ void __launch_bounds__(kCaBlock) hitToTupleCountKernel(CAPool ca) {
-      const uint32_t e = logicalBlock().y;
-      const uint32_t nT = ca.cnt[e].nTuples;
-      const size_t tb = ca.tupleBase(e);
-      const uint32_t* off = ca.tupleOff + ca.offBase(e);
-      const OneToMany h2t = hitToTuple(ca, e, 0);
-      for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
-        if (ca.quality[tb + t] != kQualityLoose)
-          continue;
-        for (uint32_t k = off[t]; k < off[t + 1]; ++k)
-          h2t.countEntry(ca.tupleBins[5 * tb + k]);
-      }
-    }
-
-    __global__ void __launch_bounds__(kEventBlock) hitToTupleScanKernel(Pool p, CAPool ca) {
-      const uint32_t e = blockIdx.x;
+    __global__ void __launch_bounds__(kEventBlock) hitToTupleBuildKernel(Pool p, CAPool ca) {
+      const uint32_t s = blockIdx.x, e = blockIdx.y;
+      __shared__ uint32_t cnt[kH2tSlice];
       __shared__ uint32_t lds[kEventBlock / 64];
       const uint32_t nHits = p.hitsLayerStart[size_t(e) * kLayerStartStride + 10];
-      const OneToMany h2t = hitToTuple(ca, e, nHits);
-      const uint32_t total = h2t.scan<kEventBlock>(lds);
-      if (threadIdx.x == 0 && total > h2t.capacity)
-        atomicOr(&ca.cnt[e].status, kCaHitToTupleOverflow);
-    }
-
-    __global__ void __launch_bounds__(kCaBlock) hitToTupleFillKernel(CAPool ca) {
-      const uint32_t e = logicalBlock().y;
+      const uint32_t h0 = s * kH2tSlice;
+      if (h0 >= nHits && s > 0)
+        return;  // uniform per workgroup; workgroup 0 always writes the end offset
+      const uint32_t n = h0 < nHits ? min(kH2tSlice, nHits - h0) : 0;
       const uint32_t nT = ca.cnt[e].nTuples;
       const size_t tb = ca.tupleBase(e);
-      const uint32_t* off = ca.tupleOff + ca.offBase(e);
-      const OneToMany h2t = hitToTuple(ca, e, 0);
-      TupleIndex* bins = ca.h2tBins + size_t(e) * ca.caps.maxHitToTuple;
-      for (uint32_t t = gridStart(); t < nT; t += gridStride()) {
-        if (ca.quality[tb + t] != kQualityLoose)
+      const uint32_t* toff = ca.tupleOff + ca.offBase(e);
+      const uint16_t* tbins = ca.tupleBins + 5 * tb;
+      const uint8_t* q = ca.quality + tb;
+      const uint32_t cap = ca.caps.maxHitToTuple;
+      uint32_t* hoff = ca.h2tOff + size_t(e) * kHitStride;
+      TupleIndex* bins = ca.h2tBins + size_t(e) * cap;
+      for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
+        cnt[i] = 0;
+      __syncthreads();
+      uint32_t below = 0;
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        if (q[t] != kQualityLoose)
           continue;
-        for (uint32_t k = off[t]; k < off[t + 1]; ++k)
-          h2t.fill(ca.tupleBins[5 * tb + k], TupleIndex(t), bins);
+        for (uint32_t k = toff[t]; k < toff[t + 1]; ++k) {
+          const uint32_t h = tbins[k];
+          if (h < h0)
+            ++below;
+          else if (h - h0 < n)
+            atomicAdd(&cnt[h - h0], 1u);
+        }
+      }
+      uint32_t base;
+      wgExclusiveScan<kEventBlock>(below, base, lds);  // base: the entries of the lower hits
+      // offsets: each thread takes a run of consecutive hits, the runs in order
+      const uint32_t per = (n + kEventBlock - 1) / kEventBlock;
+      const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
+      uint32_t mine = 0;
+      for (uint32_t i = i0; i < i1; ++i)
+        mine += cnt[i];
+      uint32_t total;
+      uint32_t run = base + wgExclusiveScan<kEventBlock>(mine, total, lds);
+      for (uint32_t i = i0; i < i1; ++i) {
+        hoff[h0 + i] = min(run, cap);
+        run += cnt[i];
+        cnt[i] = 0;  // the counts become ranks
+      }
+      if (h0 + n == nHits && threadIdx.x == 0) {
+        hoff[nHits] = min(base + total, cap);
+        if (base + total > cap)
+          atomicOr(&ca.cnt[e].status, kCaHitToTupleOverflow);
+      }
+      __syncthreads();
+      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+        if (q[t] != kQualityLoose)
+          continue;
+        for (uint32_t k = toff[t]; k < toff[t + 1]; ++k) {
+          const uint32_t h = tbins[k];
+          if (h - h0 >= n)
+            continue;
+          // hoff[h] is the unclamped offset unless it reached the capacity, where every
+          // entry of h is dropped: the entry is kept iff its place is below the capacity
+          const uint32_t pos = hoff[h] + atomicAdd(&cnt[h - h0], 1u);
+          if (pos < cap)
+            bins[pos] = TupleIndex(t);
+        }
       }
     }
 
@@ -442,14 +475,9 @@ namespace pxg {
 
   void launchHitToTuple(const CALaunch& l) {
     const dim3 g(l.blocks, l.nEvents);
-    l.t->b("hitToTupleCount");
-    hitToTupleCountKernel<<<g, kCaBlock, 0, l.s>>>(*l.ca);
-    l.t->e();
-    l.t->b("hitToTupleScan");
-    hitToTupleScanKernel<<<l.nEvents, kEventBlock, 0, l.s>>>(*l.p, *l.ca);
-    l.t->e();
     l.t->b("hitToTupleFill");
-    hitToTupleFillKernel<<<g, kCaBlock, 0, l.s>>>(*l.ca);
+    hitToTupleBuildKernel<<<dim3((kMaxNumClusters + kH2tSlice - 1) / kH2tSlice, l.nEvents), kEventBlock, 0, l.s>>>(
+        *l.p, *l.ca);
     l.t->e();
     l.t->b("tripletCleaner");
     tripletCleanerKernel<<<g, kCaBlock, 0, l.s>>>(*l.p, *l.ca);

@@ -18,8 +18,9 @@
 //                       The surviving clusters' extents and charges, which the
 //                       rec-hit stage needs, are accumulated in LDS on the way
 //                       and written once per cluster (ClusterExt).
-//                       Larger runs are appended to a work list.
-//   clusterBigKernel    persistent workgroups draining that list: pixels
+//                       Larger runs are listed per event in order by the
+//                       module-start scan.
+//   clusterBigKernel    a few workgroups per event over that list: pixels
 //                       staged in LDS, column histogram, lock-free union-find
 //                       that hooks the larger root under the smaller (so the
 //                       root of a component is its smallest index), roots
@@ -37,7 +38,8 @@ namespace pxg {
     constexpr int kWaves = kBlock / 64;
     constexpr uint32_t kNoPixel = 0xffffffffu;
     constexpr int kCols = kColsPerModule;  // 416
-    constexpr int kBigWorkgroups = 768;  // persistent workgroups for the large module runs (3 per CU by LDS)
+    // workgroups for the large module runs: ~768 per launch (3 per CU by LDS), 1 to 64 per event
+    constexpr uint32_t kBigWorkgroups = 768, kBigPerEventMax = 64;
 
     __device__ __forceinline__ uint32_t findRoot(volatile uint32_t* parent, uint32_t x) {
       uint32_t p = parent[x];
@@ -448,13 +450,6 @@ namespace pxg {
     const uint32_t n0 = runOf(k0, f0), n1 = kRuns > 1 ? runOf(k0 + 1, f1) : 0;
     int32_t* row = rowLds[wave];
     uint32_t(*ext)[64] = extLds[wave];
-    for (uint32_t u = 0; u < 2; ++u) {
-      const uint32_t n = u ? n1 : n0;
-      if (n > 64 && lane == 0) {
-        const uint32_t slot = atomicAdd(&p.work[0], 1u);
-        p.bigList[slot] = (e << 16) | (k0 + u);
-      }
-    }
     if (n0 <= 32 && n1 <= 32) {
       const uint32_t half = lane >> 5;
       clusterRunLanes<32>(p, e, k0 + half, half ? f1 : f0, half ? n1 : n0, ev, row, ext, 32 * half,
@@ -465,21 +460,14 @@ namespace pxg {
     }
   }
 
-  // ---- larger module runs: persistent workgroups over the work list ---------------
+  // ---- larger module runs: each event's list, its runs strided over its workgroups -------
   __global__ void __launch_bounds__(kBlock) clusterBigKernel(Pool p) {
     __shared__ BigLds L;
-    __shared__ uint32_t item;
-    const uint32_t total = p.work[0];
-    while (true) {
-      if (threadIdx.x == 0)
-        item = atomicAdd(&p.work[1], 1u);
-      __syncthreads();
-      const uint32_t it = item;
-      __syncthreads();
-      if (it >= total)
-        break;  // every workgroup leaves once the list is drained
-      const uint32_t code = p.bigList[it];
-      clusterRunWG(p, code >> 16, code & 0xffff, L);
+    const uint32_t e = blockIdx.y;
+    const uint32_t n = p.counts[e].nBigRuns;
+    const uint32_t* list = p.bigList + size_t(e) * kMaxModuleRanges;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // uniform per workgroup
+      clusterRunWG(p, e, list[i], L);
       __syncthreads();
     }
   }
@@ -517,7 +505,8 @@ namespace pxg {
       clusterSmallKernel<1><<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(p);
     else
       clusterSmallKernel<2><<<dim3((kMaxNumModules + 2 * kWaves - 1) / (2 * kWaves), nEvents), kBlock, 0, s>>>(p);
-    clusterBigKernel<<<kBigWorkgroups, kBlock, 0, s>>>(p);
+    const uint32_t big = nEvents ? std::max(1u, std::min(kBigPerEventMax, kBigWorkgroups / nEvents)) : 1u;
+    clusterBigKernel<<<dim3(big, nEvents), kBlock, 0, s>>>(p);
     clusStartKernel<<<nEvents, 1024, 0, s>>>(p);
   }
 

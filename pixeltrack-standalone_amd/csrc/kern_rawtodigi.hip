@@ -286,6 +286,7 @@ namespace pxg {
 
   __global__ void __launch_bounds__(kScanBlock) moduleStartKernel(Pool p) {
     __shared__ uint64_t lds[kScanBlock / 64];
+    __shared__ uint32_t lds32[kScanBlock / 64];
     const uint32_t e = blockIdx.x;
     const uint32_t wordOffset = p.events[e].wordOffset;
     const uint32_t nWords = p.events[e].nWords;
@@ -330,16 +331,31 @@ namespace pxg {
       baseMod += uint32_t(total & 0xffffffffu);
       baseErr += uint32_t(total >> 32);
     }
-    if (e == 0 && threadIdx.x < 2)
-      p.work[threadIdx.x] = 0;  // the clusteriser's list of large module runs
+    // the runs of more than 64 digis, in order, for the clusteriser's workgroup kernel
+    __syncthreads();
+    const uint32_t nMod = baseMod < kMaxNumModules ? baseMod : kMaxNumModules;
+    uint32_t* bigList = p.bigList + size_t(e) * kMaxModuleRanges;
+    uint32_t nBig = 0;
+    for (uint32_t c0 = 0; c0 < nMod; c0 += kScanBlock) {
+      const uint32_t k = c0 + threadIdx.x;
+      bool big = false;
+      if (k < nMod) {
+        const uint32_t end = k + 1 < nMod ? moduleStart[2 + k] : nWords;
+        big = end - moduleStart[1 + k] > 64;
+      }
+      uint32_t total;
+      const uint32_t at = nBig + wgExclusiveScan<kScanBlock>(big ? 1u : 0u, total, lds32);
+      if (big)
+        bigList[at] = k;
+      nBig += total;
+    }
     if (threadIdx.x == 0) {
-      const uint32_t nMod = baseMod < kMaxNumModules ? baseMod : kMaxNumModules;
       moduleStart[0] = nMod;
       EventCounts& cnt = p.counts[e];
       cnt.nModules = nMod;
       cnt.nErrors = baseErr;
       cnt.nClusters = 0;
-      cnt.modulesDone = 0;
+      cnt.nBigRuns = nBig;
       cnt.status = baseMod > kMaxNumModules ? kStatusModuleOverflow : 0;
     }
   }

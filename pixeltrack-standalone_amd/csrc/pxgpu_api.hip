@@ -73,7 +73,6 @@ namespace pxg::detail {
     f(p.errorsSparse, W);
     f(p.errors, W);
     f(p.clExt, W);
-    f(p.work, 64);
     f(p.bigList, M);
     f(p.moduleStart, M);
     f(p.clusInModule, M);
@@ -108,7 +107,7 @@ namespace pxg::detail {
     f(ca.cellOuter, D);
     f(ca.cellPair, D);
     f(ca.cellFlags, D);
-    f(ca.cellUsed, E * (ca.caps.maxDoublets / 32 + 64));
+    f(ca.cellUsed, D);
     f(ca.ohCount, HS);
     f(ca.ohOff, HS);
     f(ca.ohCells, D);
@@ -138,7 +137,6 @@ namespace pxg::detail {
     f(ca.pt, T);
     f(ca.eta, T);
     f(ca.chi2, T);
-    f(ca.h2tCount, HS);
     f(ca.h2tOff, HS);
     f(ca.h2tBins, E * ca.caps.maxHitToTuple);
     f(ca.idv, T);
@@ -182,6 +180,7 @@ namespace pxg::detail {
     v.clusInModule = p.clusInModule + m;
     v.moduleId = p.moduleId + m;
     v.clusModuleStart = p.clusModuleStart + m;
+    v.bigList = p.bigList + m;
     const size_t h = size_t(first) * kMaxNumClusters;
     v.xl = p.xl + h;
     v.yl = p.yl + h;
@@ -218,7 +217,7 @@ namespace pxg::detail {
     v.cellOuter += D;
     v.cellPair += D;
     v.cellFlags += D;
-    v.cellUsed += size_t(first) * (c.caps.maxDoublets / 32 + 64);
+    v.cellUsed += D;
     v.ohCount += H;
     v.ohOff += H;
     v.ohCells += D;
@@ -248,7 +247,6 @@ namespace pxg::detail {
     v.pt += T;
     v.eta += T;
     v.chi2 += T;
-    v.h2tCount += H;
     v.h2tOff += H;
     v.h2tBins += size_t(first) * c.caps.maxHitToTuple;
     v.idv += T;

@@ -62,7 +62,7 @@ namespace pxg {
     uint32_t nClusters;
     uint32_t nErrors;
     uint32_t status;
-    uint32_t modulesDone;  // completion counter of the clustering grid
+    uint32_t nBigRuns;     // module runs of more than 64 digis, listed in bigList
     uint32_t pad[3];
   };
 
@@ -105,8 +105,7 @@ namespace pxg {
     // [maxWords]: the clusters of a module run of at most 64 digis, found by the
     // clustering, at the run's first digi + cluster id
     ClusterExt* clExt;
-    uint32_t* work;     // [0] large module runs listed, [1] next one to take
-    uint32_t* bigList;  // large module runs of the launch: event << 16 | run
+    uint32_t* bigList;  // per event (stride kMaxModuleRanges): its module runs of more than 64 digis
     // clusters, per event
     uint32_t *moduleStart, *clusInModule, *moduleId, *clusModuleStart;  // stride kMaxModuleRanges
     // hits, per event (stride kMaxNumClusters)
@@ -191,7 +190,7 @@ namespace pxg {
     uint16_t *cellInner, *cellOuter;  // [maxDoublets]
     uint8_t* cellPair;
     uint8_t* cellFlags;      // bit0 listed in its outer hit's list, bit1 killed, bit2 duplicate pass done
-    uint32_t* cellUsed;      // bit per cell [maxDoublets / 32 + 64]
+    uint8_t* cellUsed;       // [maxDoublets] 1: the cell has a neighbour or is one
     uint32_t* ohCount;       // [kHitStride] cells per outer hit, then fill cursors
     uint32_t* ohOff;         // [kHitStride]
     uint2* ohCells;          // [maxDoublets] cells by outer hit (fill order): x cell, y inner hit | outer hit << 16
@@ -218,7 +217,6 @@ namespace pxg {
     float* state;            // [5][maxTuples]: phi, tip, q/pt, cot(theta), zip
     float* cov;              // [15][maxTuples]: packed upper triangle, row by row
     float *pt, *eta, *chi2;  // [maxTuples]
-    uint32_t* h2tCount;      // [kHitStride]
     uint32_t* h2tOff;        // [kHitStride]
     TupleIndex* h2tBins;     // [maxHitToTuple]
     // vertices

@@ -25,8 +25,7 @@ namespace pxg {
     constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
     constexpr size_t kOffEvents = 0;
     constexpr size_t kOffCounts = kOffEvents + align256(sizeof(EventDesc));
-    constexpr size_t kOffWork = kOffCounts + align256(sizeof(EventCounts));
-    constexpr size_t kOffBigList = kOffWork + align256(64 * sizeof(uint32_t));
+    constexpr size_t kOffBigList = kOffCounts + align256(sizeof(EventCounts));
     constexpr size_t kOffMasks = kOffBigList + align256(kMaxModuleRanges * sizeof(uint32_t));
     constexpr size_t kOffSparse = kOffMasks + align256(2 * (kNoAllocWordCap / 64) * sizeof(uint64_t));
     constexpr size_t kScratchBytes = kOffSparse + align256(size_t(kNoAllocWordCap) * sizeof(PixelErrorCompact));
@@ -99,7 +98,6 @@ extern "C" int pxg_raw_to_cluster_noalloc(pxg_context* ctx, void* hipStream, con
   p.fedIds = const_cast<uint8_t*>(fedIds);
   p.events = reinterpret_cast<EventDesc*>(base + kOffEvents);
   p.counts = reinterpret_cast<EventCounts*>(base + kOffCounts);
-  p.work = reinterpret_cast<uint32_t*>(base + kOffWork);
   p.bigList = reinterpret_cast<uint32_t*>(base + kOffBigList);
   p.boundaryMask = reinterpret_cast<uint64_t*>(base + kOffMasks);
   p.errorsSparse = reinterpret_cast<PixelErrorCompact*>(base + kOffSparse);

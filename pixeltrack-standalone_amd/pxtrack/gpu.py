@@ -321,9 +321,7 @@ class Stream:
         out["cellLayerPair"] = self._copy(ci.cellPair, np.uint8, nc).astype(np.int16)
         flags = self._copy(ci.cellFlags, np.uint8, nc)
         out["cellDoubletId"] = np.where(flags & 2, -1, np.arange(nc)).astype(np.int32)
-        used = self._copy(ci.cellUsed, np.uint32, (nc + 31) // 32)
-        bits = np.unpackbits(used.view(np.uint8), bitorder="little")[:nc] if nc else np.zeros(0, np.uint8)
-        out["cellUsed"] = bits.astype(np.uint16)
+        out["cellUsed"] = self._copy(ci.cellUsed, np.uint8, nc).astype(np.uint16)
         ohoff = self._copy(ci.outerHitOff, np.uint32, n_hits + 1)
         ohc = self._copy(ci.outerHitCells, np.uint32, 2 * int(ohoff[-1]) if n_hits else 0)[0::2]
         # the device keeps each hit's cells in fill order; the serial container holds the
