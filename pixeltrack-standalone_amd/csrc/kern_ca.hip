@@ -389,7 +389,7 @@ namespace pxg {
       for (uint32_t w = threadIdx.x; w < nWords; w += kEventBlock)
         hist[w] = 0;
       __syncthreads();
-      constexpr uint32_t kU = 4;  // cells per thread and iteration: their loads are issued together
+      constexpr uint32_t kU = 16;  // cells per thread and iteration: their loads are issued together
       for (uint32_t c0 = threadIdx.x; c0 < nCells; c0 += kU * kEventBlock) {
         uint32_t h[kU];
 #pragma unroll

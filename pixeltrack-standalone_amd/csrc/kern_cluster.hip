@@ -20,11 +20,13 @@
 //                       and written once per cluster (ClusterExt).
 //                       Larger runs are listed per event in order by the
 //                       module-start scan.
-//   clusterBigKernel    a few workgroups per event over that list: pixels
-//                       staged in LDS, column histogram, lock-free union-find
-//                       that hooks the larger root under the smaller (so the
-//                       root of a component is its smallest index), roots
-//                       numbered by a workgroup scan.
+//   clusterMediumKernel runs of 65 to 256 digis, one wavefront each (a
+//   clusterBigKernel    workgroup of one wave), and the larger ones, a few
+//                       256-thread workgroups per event: pixels staged in LDS,
+//                       column histogram, lock-free union-find that hooks the
+//                       larger root under the smaller (so the root of a
+//                       component is its smallest index), roots numbered by a
+//                       workgroup scan.
 //   clusStartKernel     one workgroup per event: the clamped prefix over the
 //                       2000 cluster counts (SiPixelRawToClusterGPUKernel.cc:
 //                       464-506).
@@ -38,8 +40,9 @@ namespace pxg {
     constexpr int kWaves = kBlock / 64;
     constexpr uint32_t kNoPixel = 0xffffffffu;
     constexpr int kCols = kColsPerModule;  // 416
-    // workgroups for the large module runs: ~768 per launch (3 per CU by LDS), 1 to 64 per event
-    constexpr uint32_t kBigWorkgroups = 768, kBigPerEventMax = 64;
+    // workgroups per event for the runs of 65 to 256 digis (one wave each, ~4096 per
+    // launch) and for the larger ones (a few per event at most)
+    constexpr uint32_t kMediumWorkgroups = 4096, kMediumPerEventMax = 64, kBigPerEvent = 2;
 
     __device__ __forceinline__ uint32_t findRoot(volatile uint32_t* parent, uint32_t x) {
       uint32_t p = parent[x];
@@ -70,17 +73,23 @@ namespace pxg {
     }
   }  // namespace
 
-  struct BigLds {
-    uint32_t sxy[kMaxPixInModule];     // x | y << 16, kNoPixel if invalid
-    uint32_t parent[kMaxPixInModule];  // union-find; later the cluster id
-    uint32_t work[kMaxPixInModule];    // column-sorted pixels; later charges/renumbering
+  // LDS of one module run of at most MAXPIX digis on a workgroup of NT threads
+  template <int NT, uint32_t MAXPIX>
+  struct RunLds {
+    static constexpr uint32_t kClusCap = MAXPIX < kMaxClustersPerModule ? MAXPIX : kMaxClustersPerModule;
+    uint32_t sxy[MAXPIX];     // x | y << 16, kNoPixel if invalid
+    uint32_t parent[MAXPIX];  // union-find; later the cluster id
+    uint32_t work[MAXPIX > 2 * kClusCap ? MAXPIX : 2 * kClusCap];  // column-sorted pixels; later charges/renumbering
     uint32_t colStart[kCols + 2];
     uint32_t colFill[kCols];
-    uint32_t wsum[kWaves];
+    uint32_t wsum[NT / 64];
   };
 
-  // one module run with a whole workgroup (any size up to 4000 digis)
-  __device__ void clusterRunWG(const Pool& p, uint32_t e, uint32_t k, BigLds& L) {
+  // one module run with a whole workgroup of NT threads (a run of up to MAXPIX digis;
+  // the largest runs, up to 4000 digis, on 256 threads)
+  template <int NT, uint32_t MAXPIX>
+  __device__ void clusterRunWG(const Pool& p, uint32_t e, uint32_t k, RunLds<NT, MAXPIX>& L) {
+    constexpr uint32_t kClusCap = RunLds<NT, MAXPIX>::kClusCap;
     uint32_t* sxy = L.sxy;
     uint32_t* parent = L.parent;
     uint32_t* work = L.work;
@@ -100,15 +109,16 @@ namespace pxg {
         atomicOr(&cnt.status, kStatusPixelOverflow);
       n = kMaxPixInModule;  // truncated as in the reference
     }
+    n = n < MAXPIX ? n : MAXPIX;  // runs are given to a workgroup whose LDS holds them
     const uint16_t me = p.moduleInd[wo + first];
     if (me >= kNumModules)
       n = 0;  // a run always starts on a valid digi; never taken for valid conditions
 
     // ---- stage pixels, column counts
-    for (int i = threadIdx.x; i < kCols + 2; i += kBlock)
+    for (int i = threadIdx.x; i < kCols + 2; i += NT)
       colStart[i] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const uint32_t g = wo + first + i;
       const bool valid = p.moduleInd[g] != kInvId;
       uint32_t xy = kNoPixel;
@@ -143,10 +153,10 @@ namespace pxg {
     __syncthreads();
     // fill the column lists (order inside a column is irrelevant)
     uint32_t* colFill = L.colFill;
-    for (int i = threadIdx.x; i < kCols; i += kBlock)
+    for (int i = threadIdx.x; i < kCols; i += NT)
       colFill[i] = colStart[i];
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const uint32_t xy = sxy[i];
       if (xy != kNoPixel)
         work[atomicAdd(&colFill[xy >> 16], 1u)] = i;
@@ -154,7 +164,7 @@ namespace pxg {
     __syncthreads();
 
     // ---- union over 8-neighbours: same column (pairs once) and next column
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const uint32_t xy = sxy[i];
       if (xy == kNoPixel)
         continue;
@@ -177,18 +187,18 @@ namespace pxg {
       }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < n; i += NT)
       if (sxy[i] != kNoPixel)
         parent[i] = findRoot(parent, i);
     __syncthreads();
 
     // ---- number the roots in increasing pixel index
     uint32_t nclus = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += kBlock) {
+    for (uint32_t c0 = 0; c0 < n; c0 += NT) {
       const uint32_t i = c0 + threadIdx.x;
       const bool root = i < n && sxy[i] != kNoPixel && parent[i] == i;
       uint32_t tot;
-      const uint32_t excl = wgExclusiveScan<kBlock>(root ? 1u : 0u, tot, wsum);
+      const uint32_t excl = wgExclusiveScan<NT>(root ? 1u : 0u, tot, wsum);
       if (root)
         work[i] = nclus + excl;  // rank of the root
       nclus += tot;
@@ -196,7 +206,7 @@ namespace pxg {
     __syncthreads();
     const uint32_t found = nclus;
     // cluster id per pixel (roots' ranks); ids >= 1024 are dropped as in the reference
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < n; i += NT)
       if (sxy[i] != kNoPixel)
         parent[i] = work[parent[i]];
     __syncthreads();
@@ -208,23 +218,23 @@ namespace pxg {
     }
 
     // ---- charge cut
-    int32_t* charge = reinterpret_cast<int32_t*>(work);                        // [1024]
-    uint32_t* newId = work + kMaxClustersPerModule;                            // [1024]
-    for (uint32_t c = threadIdx.x; c < ncut; c += kBlock)
+    int32_t* charge = reinterpret_cast<int32_t*>(work);                        // [kClusCap]
+    uint32_t* newId = work + kClusCap;                                         // [kClusCap]
+    for (uint32_t c = threadIdx.x; c < ncut; c += NT)
       charge[c] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       if (sxy[i] != kNoPixel && parent[i] < ncut)
         atomicAdd(&charge[parent[i]], int32_t(p.adc[wo + first + i]));
     }
     __syncthreads();
     const int cut = me < 96 ? 2000 : 4000;
     uint32_t kept = 0;
-    for (uint32_t c0 = 0; c0 < ncut; c0 += kBlock) {
+    for (uint32_t c0 = 0; c0 < ncut; c0 += NT) {
       const uint32_t c = c0 + threadIdx.x;
       const uint32_t ok = (c < ncut && charge[c] > cut) ? 1u : 0u;
       uint32_t tot;
-      const uint32_t excl = wgExclusiveScan<kBlock>(ok, tot, wsum);
+      const uint32_t excl = wgExclusiveScan<NT>(ok, tot, wsum);
       if (c < ncut)
         newId[c] = ok ? (kept + excl) : 9999u;
       kept += tot;
@@ -233,7 +243,7 @@ namespace pxg {
     const bool renumber = ncut > 0 && kept != ncut;
 
     // ---- write back
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const uint32_t g = wo + first + i;
       if (sxy[i] == kNoPixel) {
         p.clus[g] = -9999;
@@ -460,14 +470,28 @@ namespace pxg {
     }
   }
 
-  // ---- larger module runs: each event's list, its runs strided over its workgroups -------
+  // ---- larger module runs: each event's lists, their runs strided over its workgroups ----
+  // runs of 65 to 256 digis: one wavefront each (a workgroup of one wave, 7.4 KB of LDS)
+  constexpr uint32_t kMediumMaxPix = 256;
+  __global__ void __launch_bounds__(64) clusterMediumKernel(Pool p) {
+    __shared__ RunLds<64, kMediumMaxPix> L;
+    const uint32_t e = blockIdx.y;
+    const uint32_t n = p.counts[e].nMediumRuns;
+    const uint32_t* list = p.bigList + size_t(e) * kMaxModuleRanges;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // uniform per workgroup
+      clusterRunWG<64, kMediumMaxPix>(p, e, list[i], L);
+      __syncthreads();
+    }
+  }
+
+  // runs of more than 256 digis (up to 4000), listed from the back of the event's list
   __global__ void __launch_bounds__(kBlock) clusterBigKernel(Pool p) {
-    __shared__ BigLds L;
+    __shared__ RunLds<kBlock, kMaxPixInModule> L;
     const uint32_t e = blockIdx.y;
     const uint32_t n = p.counts[e].nBigRuns;
     const uint32_t* list = p.bigList + size_t(e) * kMaxModuleRanges;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // uniform per workgroup
-      clusterRunWG(p, e, list[i], L);
+      clusterRunWG<kBlock, kMaxPixInModule>(p, e, list[kMaxModuleRanges - 1 - i], L);
       __syncthreads();
     }
   }
@@ -505,8 +529,9 @@ namespace pxg {
       clusterSmallKernel<1><<<dim3((kMaxNumModules + kWaves - 1) / kWaves, nEvents), kBlock, 0, s>>>(p);
     else
       clusterSmallKernel<2><<<dim3((kMaxNumModules + 2 * kWaves - 1) / (2 * kWaves), nEvents), kBlock, 0, s>>>(p);
-    const uint32_t big = nEvents ? std::max(1u, std::min(kBigPerEventMax, kBigWorkgroups / nEvents)) : 1u;
-    clusterBigKernel<<<dim3(big, nEvents), kBlock, 0, s>>>(p);
+    const uint32_t med = nEvents ? std::max(1u, std::min(kMediumPerEventMax, kMediumWorkgroups / nEvents)) : 1u;
+    clusterMediumKernel<<<dim3(med, nEvents), 64, 0, s>>>(p);
+    clusterBigKernel<<<dim3(kBigPerEvent, nEvents), kBlock, 0, s>>>(p);
     clusStartKernel<<<nEvents, 1024, 0, s>>>(p);
   }
 

@@ -286,7 +286,6 @@ namespace pxg {
 
   __global__ void __launch_bounds__(kScanBlock) moduleStartKernel(Pool p) {
     __shared__ uint64_t lds[kScanBlock / 64];
-    __shared__ uint32_t lds32[kScanBlock / 64];
     const uint32_t e = blockIdx.x;
     const uint32_t wordOffset = p.events[e].wordOffset;
     const uint32_t nWords = p.events[e].nWords;
@@ -331,23 +330,26 @@ namespace pxg {
       baseMod += uint32_t(total & 0xffffffffu);
       baseErr += uint32_t(total >> 32);
     }
-    // the runs of more than 64 digis, in order, for the clusteriser's workgroup kernel
+    // the runs the clusteriser's wavefront kernel cannot take, in order: those of 65 to
+    // 256 digis from the front of the event's list, the larger ones from its back
     __syncthreads();
     const uint32_t nMod = baseMod < kMaxNumModules ? baseMod : kMaxNumModules;
     uint32_t* bigList = p.bigList + size_t(e) * kMaxModuleRanges;
-    uint32_t nBig = 0;
+    uint32_t nMed = 0, nBig = 0;
     for (uint32_t c0 = 0; c0 < nMod; c0 += kScanBlock) {
       const uint32_t k = c0 + threadIdx.x;
-      bool big = false;
-      if (k < nMod) {
-        const uint32_t end = k + 1 < nMod ? moduleStart[2 + k] : nWords;
-        big = end - moduleStart[1 + k] > 64;
-      }
-      uint32_t total;
-      const uint32_t at = nBig + wgExclusiveScan<kScanBlock>(big ? 1u : 0u, total, lds32);
+      uint32_t n = 0;
+      if (k < nMod)
+        n = (k + 1 < nMod ? moduleStart[2 + k] : nWords) - moduleStart[1 + k];
+      const bool med = n > 64 && n <= 256, big = n > 256;
+      uint64_t total;
+      const uint64_t at = wgExclusiveScan<kScanBlock>(uint64_t(med) | uint64_t(big) << 32, total, lds);
+      if (med)
+        bigList[nMed + uint32_t(at)] = k;
       if (big)
-        bigList[at] = k;
-      nBig += total;
+        bigList[kMaxModuleRanges - 1 - (nBig + uint32_t(at >> 32))] = k;
+      nMed += uint32_t(total);
+      nBig += uint32_t(total >> 32);
     }
     if (threadIdx.x == 0) {
       moduleStart[0] = nMod;
@@ -355,6 +357,7 @@ namespace pxg {
       cnt.nModules = nMod;
       cnt.nErrors = baseErr;
       cnt.nClusters = 0;
+      cnt.nMediumRuns = nMed;
       cnt.nBigRuns = nBig;
       cnt.status = baseMod > kMaxNumModules ? kStatusModuleOverflow : 0;
     }

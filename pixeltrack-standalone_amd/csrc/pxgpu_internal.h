@@ -62,8 +62,9 @@ namespace pxg {
     uint32_t nClusters;
     uint32_t nErrors;
     uint32_t status;
-    uint32_t nBigRuns;     // module runs of more than 64 digis, listed in bigList
-    uint32_t pad[3];
+    uint32_t nMediumRuns;  // module runs of 65 to 256 digis, listed at the front of bigList
+    uint32_t nBigRuns;     // module runs of more than 256 digis, listed from the back of bigList
+    uint32_t pad[2];
   };
 
   struct alignas(8) HitAttr {
