@@ -80,8 +80,9 @@ namespace pxg {
     uint32_t sxy[MAXPIX];     // x | y << 16, kNoPixel if invalid
     uint32_t parent[MAXPIX];  // union-find; later the cluster id
     uint32_t work[MAXPIX > 2 * kClusCap ? MAXPIX : 2 * kClusCap];  // column-sorted pixels; later charges/renumbering
-    uint32_t colStart[kCols + 2];
-    uint32_t colFill[kCols];
+    // column starts [0, kCols + 2) and fill cursors [kCols + 2, 2 kCols + 2); once the
+    // components are found, the cluster extents of a run of at most 256 digis
+    uint32_t cols[2 * kCols + 2];
     uint32_t wsum[NT / 64];
   };
 
@@ -93,7 +94,7 @@ namespace pxg {
     uint32_t* sxy = L.sxy;
     uint32_t* parent = L.parent;
     uint32_t* work = L.work;
-    uint32_t* colStart = L.colStart;
+    uint32_t* colStart = L.cols;
     uint32_t* wsum = L.wsum;
     const EventDesc& ev = p.events[e];
     EventCounts& cnt = p.counts[e];
@@ -152,7 +153,7 @@ namespace pxg {
     }
     __syncthreads();
     // fill the column lists (order inside a column is irrelevant)
-    uint32_t* colFill = L.colFill;
+    uint32_t* colFill = L.cols + kCols + 2;
     for (int i = threadIdx.x; i < kCols; i += NT)
       colFill[i] = colStart[i];
     __syncthreads();
@@ -268,6 +269,75 @@ namespace pxg {
       p.moduleId[size_t(e) * kMaxModuleRanges + k] = me;
     }
 
+    // Runs of at most 256 digis also record each cluster's extent and charges for the
+    // rec-hit stage (ClusterExt, as the wavefront kernel does), accumulated with LDS
+    // atomics in the column area, which holds the nine fields of 92 clusters at a time
+    // (nearly always all of the run's).  Every cluster of such a run has pixels.
+    if constexpr (MAXPIX <= 256) {
+      if (p.clExt && n > 0) {
+        const uint32_t nf = renumber ? kept : found;
+        constexpr uint32_t kChunk = (2 * kCols + 2) / 9;
+        uint32_t* ext = L.cols;  // field j of chunk cluster c at ext[j * kChunk + c]
+        auto finalId = [&](uint32_t i) -> uint32_t {
+          if (sxy[i] == kNoPixel || parent[i] >= ncut)
+            return 0xffffffffu;
+          const uint32_t cid = renumber ? newId[parent[i]] : parent[i];
+          return cid == 9999u ? 0xffffffffu : cid;
+        };
+        for (uint32_t f0 = 0; f0 < nf; f0 += kChunk) {
+          const uint32_t m = min(kChunk, nf - f0);
+          __syncthreads();  // the area is free (column lists, or the previous chunk)
+          for (uint32_t c = threadIdx.x; c < m; c += NT) {
+            ext[0 * kChunk + c] = ext[2 * kChunk + c] = 0xffffffffu;
+            for (uint32_t j : {1u, 3u, 4u, 5u, 6u, 7u, 8u})
+              ext[j * kChunk + c] = 0;
+          }
+          __syncthreads();
+          for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const uint32_t f = finalId(i) - f0;
+            if (f < m) {
+              const uint32_t x = sxy[i] & 0xffff, y = sxy[i] >> 16;
+              atomicMin(&ext[0 * kChunk + f], x);
+              atomicMax(&ext[1 * kChunk + f], x);
+              atomicMin(&ext[2 * kChunk + f], y);
+              atomicMax(&ext[3 * kChunk + f], y);
+            }
+          }
+          __syncthreads();
+          for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const uint32_t f = finalId(i) - f0;
+            if (f < m) {
+              const uint32_t x = sxy[i] & 0xffff, y = sxy[i] >> 16;
+              const uint32_t q = p.adc[wo + first + i];
+              atomicAdd(&ext[4 * kChunk + f], q);
+              if (ext[0 * kChunk + f] == x)
+                atomicAdd(&ext[5 * kChunk + f], q);
+              if (ext[1 * kChunk + f] == x)
+                atomicAdd(&ext[6 * kChunk + f], q);
+              if (ext[2 * kChunk + f] == y)
+                atomicAdd(&ext[7 * kChunk + f], q);
+              if (ext[3 * kChunk + f] == y)
+                atomicAdd(&ext[8 * kChunk + f], q);
+            }
+          }
+          __syncthreads();
+          for (uint32_t c = threadIdx.x; c < m; c += NT) {
+            ClusterExt r;
+            r.minRow = uint16_t(ext[0 * kChunk + c]);
+            r.maxRow = uint16_t(ext[1 * kChunk + c]);
+            r.minCol = uint16_t(ext[2 * kChunk + c]);
+            r.maxCol = uint16_t(ext[3 * kChunk + c]);
+            r.qsum = int32_t(ext[4 * kChunk + c]);
+            r.qfx = int32_t(ext[5 * kChunk + c]);
+            r.qlx = int32_t(ext[6 * kChunk + c]);
+            r.qfy = int32_t(ext[7 * kChunk + c]);
+            r.qly = int32_t(ext[8 * kChunk + c]);
+            r.pad = 0;
+            p.clExt[wo + first + f0 + c] = r;
+          }
+        }
+      }
+    }
   }
 
   namespace {

@@ -6,7 +6,7 @@
 //                   latency chains), 64 for one event (more workgroups to fill
 //                   the device), one cluster per lane for the cluster position
 //                   estimate and the local -> global transform.  A run of at
-//                   most 64 digis (nearly all at PU50) takes its clusters'
+//                   most 256 digis (all but a few at PU50) takes its clusters'
 //                   extents and integer charges from the records the clustering
 //                   wrote (ClusterExt); a larger run accumulates them from its
 //                   digis with LDS atomics (exact in any order), G clusters per
@@ -226,8 +226,8 @@ namespace pxg {
     };
 
     const uint32_t wo = ev.wordOffset;
-    if (end - first <= 64 && p.clExt) {
-      // a run of at most 64 digis: the clustering recorded its clusters' extents
+    if (end - first <= 256 && p.clExt) {
+      // a run of at most 256 digis: the clustering recorded its clusters' extents
       const ClusterExt* ext = p.clExt + wo + first;
       for (uint32_t ic = lane; ic < nclus; ic += kGroup) {
         const ClusterExt x = ext[ic];
