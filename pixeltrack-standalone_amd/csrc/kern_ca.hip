@@ -271,6 +271,18 @@ namespace pxg {
       const uint64_t* dMask = ca.dMask + size_t(e) * 2 * kDoubletSlots;
       const uint32_t sub = laneId() % G;
       const uint64_t below = (1ull << sub) - 1;
+      if (!kRank) {
+        // the cells' state for the later stages, cleared in cell order on the whole grid
+        // (coalesced; the per-event list build would do it on one CU)
+        for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
+          ca.cellFlags[cb + c] = 0;
+          ca.cellUsed[cb + c] = 0;
+          ca.nbSlot[cb + c] = -1;
+          ca.nbCount[cb + c] = 0;
+          ca.trSlot[cb + c] = -1;
+          ca.trCount[cb + c] = 0;
+        }
+      }
       for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
         const uint32_t base = dOff[j];
         if (base >= nCells || dOff[j + 1] == base)
@@ -375,8 +387,8 @@ namespace pxg {
     // word (a hit's cells come from the inner hits of at most the 48k of the event, so
     // no count reaches 65536): the counts, a workgroup scan into the offsets, then each
     // cell at its hit's offset plus its rank, taken with a second round of LDS atomics
-    // (the consumers do not depend on the order within a list).  The pass over the
-    // cells also clears their state for the later stages.
+    // (the consumers do not depend on the order within a list).  The doublet fill has
+    // cleared the cells' state for the later stages.
     __global__ void __launch_bounds__(kEventBlock) ohBuildKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.x;
       __shared__ uint32_t hist[kMaxNumClusters / 2];
@@ -439,12 +451,6 @@ namespace pxg {
             const uint32_t sh = (h[u] & 1) * 16;
             const uint32_t rank = (atomicAdd(&hist[h[u] >> 1], 1u << sh) >> sh) & 0xffffu;
             ca.ohCells[cb + off[h[u]] + rank] = make_uint2(c, hi[u] | h[u] << 16);
-            ca.cellFlags[cb + c] = 0;
-            ca.cellUsed[cb + c] = 0;
-            ca.nbSlot[cb + c] = -1;
-            ca.nbCount[cb + c] = 0;
-            ca.trSlot[cb + c] = -1;
-            ca.trCount[cb + c] = 0;
           }
         }
       }
