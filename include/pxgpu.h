@@ -132,8 +132,9 @@ typedef struct {
   int32_t* trackSlot; uint16_t* trackCount; uint32_t* trackPool;          /* 48 tuple indices per slot */
   uint32_t* multOff; uint32_t* multBins;   /* tuple multiplicity (tuple indices) */
   uint32_t *hitToTupleOff; uint32_t* hitToTupleBins;
-  uint32_t counts[16];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,
-                            nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors, nStart */
+  uint32_t counts[20];   /* nCells, nDoublets, apcM, apcN, nTuples, status, nbLists, trLists,
+                            nMulti, nMult, nVertices, vtxTracks, vstatus, nSlots, nNeighbors, nStart,
+                            nPaths, 0, 0, 0 */
 } pxg_ca_internals;
 
 /* Context: device + conditions uploaded once, immutable afterwards (pxg_init is the

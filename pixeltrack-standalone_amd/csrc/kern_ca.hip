@@ -853,10 +853,14 @@ namespace pxg {
     // ---- a15 kernel_find_ntuplets: GPUCACell.h:278-331 ----------------------------------
     // Iterative depth-first walk from a start cell, visiting neighbours in list
     // order: tuples come out in the serial program's order.  Count mode counts
-    // tuples and hits; fill mode writes them from the cell's scanned offsets.
+    // tuples and hits and records the paths of the start cell's first kPaths tuples
+    // (the neighbour-list position taken at each level, the tuple's size and its place
+    // among the cell's tuples and hits: 27 bits); fill mode writes the tuples from the
+    // cell's scanned offsets, from the skip-th on.
+    constexpr uint32_t kPaths = 4;
     template <bool kFill>
     __device__ void walk(const CAPool& ca, uint32_t e, uint32_t c, uint32_t tBase, uint32_t hBase, uint32_t& nT,
-                         uint32_t& nH, uint32_t& status) {
+                         uint32_t& nH, uint32_t& status, uint32_t skip, uint32_t (&paths)[kPaths]) {
       const size_t cb = ca.cellBase(e);
       const uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
       const size_t tb = ca.tupleBase(e);
@@ -901,7 +905,15 @@ namespace pxg {
           continue;
         if (!((anyBits >> top) & 1) && depth >= 2) {
           const uint32_t nh = uint32_t(depth) + 1;
-          if (kFill) {
+          if (!kFill && nT < kPaths) {
+            const uint32_t code = (((posBytes >> 0) & 0xff) - 1) | ((((posBytes >> 8) & 0xff) - 1) & 63) << 6 |
+                                  ((((posBytes >> 16) & 0xff) - 1) & 63) << 12 | uint32_t(depth - 2) << 18 |
+                                  nT << 20 | nH << 22;
+#pragma unroll
+            for (uint32_t i = 0; i < kPaths; ++i)
+              paths[i] = i == nT ? code : paths[i];
+          }
+          if (kFill && nT >= skip) {
             const uint32_t m = tBase + nT;
             if (m < ca.caps.maxTuples) {
               const uint32_t first = hBase + nH;
@@ -976,13 +988,30 @@ namespace pxg {
       const uint32_t e = logicalBlock().y;
       const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
       const uint32_t* list = ca.startCells + cb;
       uint32_t status = 0;
-      for (uint32_t k = gridStart(); k < nStart; k += gridStride()) {
+      const uint32_t lane = laneId();
+      for (uint32_t k0 = gridStart() - lane; k0 < nStart; k0 += gridStride()) {  // uniform per wave
+        const uint32_t k = k0 + lane;
         uint32_t nT = 0, nH = 0;
-        walk<false>(ca, e, list[k], 0, 0, nT, nH, status);
-        ca.dfsT[cb + k] = nT;
-        ca.dfsH[cb + k] = nH;
+        uint32_t paths[kPaths] = {0, 0, 0, 0};
+        if (k < nStart) {
+          walk<false>(ca, e, list[k], 0, 0, nT, nH, status, 0, paths);
+          ca.dfsT[cb + k] = nT;
+          ca.dfsH[cb + k] = nH;
+        }
+        // the recorded paths, stored compactly: one counter update per wave
+        const uint32_t n = min(nT, kPaths);
+        const uint32_t at = waveExclusiveScan(n);
+        const uint32_t total = __shfl(at + n, 63);
+        uint32_t base = 0;
+        if (lane == 0 && total)
+          base = atomicAdd(&ca.cnt[e].nPaths, total);
+        base = __shfl(base, 0);
+        for (uint32_t i = 0; i < n; ++i)
+          if (base + at + i < ca.caps.maxTuples)
+            ca.tuplePaths[tb + base + at + i] = make_uint2(k, paths[i]);
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
@@ -1019,19 +1048,54 @@ namespace pxg {
       }
     }
 
+    // The tuples whose paths the counting walk recorded are written one per thread,
+    // following the recorded neighbour-list positions; only start cells with more than
+    // kPaths tuples are walked again, for the rest of their tuples (all of them if the
+    // path store overflowed: the tuples then written twice are written identically).
     __global__ void __launch_bounds__(kCaBlock) ntupletFillKernel(CAPool ca) {
       const uint32_t e = logicalBlock().y;
-      const uint32_t nStart = ca.cnt[e].nStart;
+      const CACounts cnt = ca.cnt[e];
       const size_t cb = ca.cellBase(e);
+      const size_t tb = ca.tupleBase(e);
       const uint32_t* list = ca.startCells + cb;
+      const uint32_t M = ca.caps.maxTuples;
+      const uint32_t* pool = ca.nbPool + size_t(e) * ca.caps.maxCellLists * kCaMaxNeighbors;
+      const uint32_t nPaths = min(cnt.nPaths, M);
+      for (uint32_t r = gridStart(); r < nPaths; r += gridStride()) {
+        const uint2 rec = ca.tuplePaths[tb + r];
+        const uint32_t k = rec.x, code = rec.y;
+        const uint32_t m = ca.dfsT[cb + k] + ((code >> 20) & 3);
+        if (m >= M)
+          continue;
+        const uint32_t depth = ((code >> 18) & 3) + 2;
+        const uint32_t first = ca.dfsH[cb + k] + (code >> 22);
+        uint32_t cell = list[k];
+        ca.tupleOff[ca.offBase(e) + m] = first;
+        uint16_t* bins = ca.tupleBins + 5 * tb + first;
+        uint32_t* cells = ca.tupleCells + 4 * tb + 4 * size_t(m);
+        for (uint32_t d = 0; d < depth; ++d) {
+          if (d > 0) {
+            const int32_t slot = ca.nbSlot[cb + cell];
+            cell = pool[size_t(slot) * kCaMaxNeighbors + ((code >> (6 * (d - 1))) & 63)];
+          }
+          bins[d] = ca.cellInner[cb + cell];
+          cells[d] = cell;
+        }
+        bins[depth] = ca.cellOuter[cb + cell];
+        for (uint32_t d = depth; d < 4; ++d)
+          cells[d] = 0xffffffffu;
+        ca.quality[tb + m] = kQualityBad;
+      }
+      const bool rewalkAll = cnt.nPaths > M;
       uint32_t status = 0;
-      for (uint32_t k = gridStart(); k < nStart; k += gridStride()) {
+      uint32_t paths[kPaths];
+      for (uint32_t k = gridStart(); k < cnt.nStart; k += gridStride()) {
         const uint32_t tBase = ca.dfsT[cb + k];
-        const uint32_t next = k + 1 < nStart ? ca.dfsT[cb + k + 1] : ca.cnt[e].apcM;
-        if (next == tBase || tBase >= ca.caps.maxTuples)
-          continue;  // no tuple stored from this cell
+        const uint32_t next = k + 1 < cnt.nStart ? ca.dfsT[cb + k + 1] : cnt.apcM;
+        if (next - tBase <= (rewalkAll ? 0 : kPaths) || tBase >= M)
+          continue;  // every tuple of this cell was recorded (or none is stored)
         uint32_t nT, nH;
-        walk<true>(ca, e, list[k], tBase, ca.dfsH[cb + k], nT, nH, status);
+        walk<true>(ca, e, list[k], tBase, ca.dfsH[cb + k], nT, nH, status, rewalkAll ? 0 : kPaths, paths);
       }
     }
 

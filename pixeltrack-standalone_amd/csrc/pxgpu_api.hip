@@ -128,6 +128,7 @@ namespace pxg::detail {
     f(ca.tupleOff, E * (ca.caps.maxTuples + 64));
     f(ca.tupleBins, 5 * T);
     f(ca.tupleCells, 4 * T);
+    f(ca.tuplePaths, T);
     f(ca.quality, T);
     f(ca.multOff, 16 * E);
     f(ca.multBins, E * ca.caps.maxTuplesMult);
@@ -238,6 +239,7 @@ namespace pxg::detail {
     v.tupleOff += c.offBase(first);
     v.tupleBins += 5 * T;
     v.tupleCells += 4 * T;
+    v.tuplePaths += T;
     v.quality += T;
     v.multOff += 16 * E;
     v.multBins += size_t(first) * c.caps.maxTuplesMult;
@@ -781,6 +783,7 @@ int pxg_event_ca_internals(pxg_stream* s, uint32_t e, pxg_ca_internals* o) {
   o->hitToTupleOff = ca.h2tOff;
   o->hitToTupleBins = ca.h2tBins;
   static_assert(sizeof(CACounts) <= sizeof(o->counts), "counts");
+  std::memset(o->counts, 0, sizeof(o->counts));
   std::memcpy(o->counts, &s->hostCA[e], sizeof(CACounts));
   return 0;
 }

@@ -179,6 +179,7 @@ namespace pxg {
     uint32_t nSlots;       // (layer pair, inner hit) slots of the doublet search
     uint32_t nNeighbors;   // neighbour entries stored
     uint32_t nStart;       // cells the ntuplet walk starts from
+    uint32_t nPaths;       // tuples whose path the counting walk recorded
   };
 
   // Device arrays of the ntuplet and vertex stages, all per event at e * stride
@@ -211,6 +212,7 @@ namespace pxg {
     uint32_t* tupleOff;      // [maxTuples + 64]
     uint16_t* tupleBins;     // [5 * maxTuples]
     uint32_t* tupleCells;    // [4 * maxTuples] cells of a tuple (0xffffffff past the end)
+    uint2* tuplePaths;       // [maxTuples] (start index, path) of the tuples the counting walk recorded
     uint8_t* quality;        // [maxTuples]
     uint32_t* multOff;       // [16]
     TupleIndex* multBins;    // [maxTuplesMult]

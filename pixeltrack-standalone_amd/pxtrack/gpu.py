@@ -60,7 +60,7 @@ class CAInternals(C.Structure):
     _fields_ = [(n, vp) for n in ("cellInner", "cellOuter", "cellPair", "cellFlags", "cellUsed", "outerHitOff",
                                   "outerHitCells", "neighborSlot", "neighborCount", "neighborPool", "trackSlot",
                                   "trackCount", "trackPool", "multOff", "multBins", "hitToTupleOff",
-                                  "hitToTupleBins")] + [("counts", u32 * 16)]
+                                  "hitToTupleBins")] + [("counts", u32 * 20)]
 
 
 class LaunchShape(C.Structure):
@@ -80,7 +80,7 @@ class ServerResult(C.Structure):
 
 
 CA_COUNTS = ["nCells", "nDoublets", "apcM", "apcN", "nTuples", "caStatus", "nbLists", "trLists", "nMulti", "nMult",
-             "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors", "nStart"]
+             "nVertices", "vtxTracks", "vstatus", "nSlots", "nNeighbors", "nStart", "nPaths"]
 PROFILE_REFERENCE, PROFILE_HIGH_OCCUPANCY = 0, 1
 BFIELD = 0.0114256972711507  # 1/GeV, CAHitNtupletCUDA.cc:31
 VERTEX_PTMIN = 0.5           # GeV, PixelVertexProducerCUDA.cc:37
