@@ -262,7 +262,7 @@ def main():
                          "staged candidates, fishbone window) instead of the autotuner's")
     ap.add_argument("--streams", type=int, default=3, help="HIP streams the batches are spread over")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
-    ap.add_argument("--serve-events", type=int, default=600)
+    ap.add_argument("--serve-events", type=int, default=2000)
     ap.add_argument("--mix-events", type=int, default=200)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--upto", type=int, default=5, help="last stage: 3 rec-hits, 4 ntuplets + fits, 5 vertices")
