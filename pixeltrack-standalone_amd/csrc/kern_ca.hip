@@ -1017,27 +1017,38 @@ namespace pxg {
         atomicOr(&ca.cnt[e].status, status);
     }
 
+    // tuples and hits per start cell scanned together (the two counts packed in 64 bits,
+    // four start cells per thread and pass)
     __global__ void __launch_bounds__(kEventBlock) ntupletScanKernel(CAPool ca) {
       const uint32_t e = blockIdx.x;
-      __shared__ uint32_t lds[kEventBlock / 64];
+      __shared__ uint64_t lds[kEventBlock / 64];
       const uint32_t nStart = ca.cnt[e].nStart;
       const size_t cb = ca.cellBase(e);
       uint32_t* t = ca.dfsT + cb;
       uint32_t* h = ca.dfsH + cb;
-      uint32_t carryT = 0, carryH = 0;
-      for (uint32_t base = 0; base < nStart; base += kEventBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t vt = i < nStart ? t[i] : 0, vh = i < nStart ? h[i] : 0;
-        uint32_t totT, totH;
-        const uint32_t exT = wgExclusiveScan<kEventBlock>(vt, totT, lds);
-        const uint32_t exH = wgExclusiveScan<kEventBlock>(vh, totH, lds);
-        if (i < nStart) {
-          t[i] = carryT + exT;
-          h[i] = carryH + exH;
+      constexpr uint32_t kV = 4;
+      uint64_t carry = 0;
+      for (uint32_t base = 0; base < nStart; base += kEventBlock * kV) {
+        const uint32_t i0 = base + threadIdx.x * kV;
+        uint64_t v[kV], sum = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kV; ++u) {
+          v[u] = i0 + u < nStart ? uint64_t(t[i0 + u]) | uint64_t(h[i0 + u]) << 32 : 0;
+          sum += v[u];
         }
-        carryT += totT;
-        carryH += totH;
+        uint64_t tot;
+        uint64_t ex = carry + wgExclusiveScan<kEventBlock>(sum, tot, lds);
+#pragma unroll
+        for (uint32_t u = 0; u < kV; ++u) {
+          if (i0 + u < nStart) {
+            t[i0 + u] = uint32_t(ex);
+            h[i0 + u] = uint32_t(ex >> 32);
+          }
+          ex += v[u];
+        }
+        carry += tot;
       }
+      const uint32_t carryT = uint32_t(carry), carryH = uint32_t(carry >> 32);
       if (threadIdx.x == 0) {
         CACounts& c = ca.cnt[e];
         c.apcM = carryT;

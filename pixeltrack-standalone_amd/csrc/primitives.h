@@ -89,17 +89,28 @@ namespace pxg {
   }
 
   // In-place exclusive scan of a[0, n) by one workgroup of NT threads; a[n] = total.
-  // lds: NT / 64 words.
+  // Four consecutive elements per thread and pass, so that a pass takes one workgroup
+  // scan (three barriers) per 4 NT elements.  lds: NT / 64 words.
   template <int NT>
   __device__ uint32_t wgScanArray(uint32_t* a, uint32_t n, uint32_t* lds) {
+    constexpr uint32_t kV = 4;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += NT) {
-      const uint32_t i = base + threadIdx.x;
-      const uint32_t v = i < n ? a[i] : 0;
+    for (uint32_t base = 0; base < n; base += NT * kV) {
+      const uint32_t i0 = base + threadIdx.x * kV;
+      uint32_t v[kV], sum = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < kV; ++u) {
+        v[u] = i0 + u < n ? a[i0 + u] : 0;
+        sum += v[u];
+      }
       uint32_t tot;
-      const uint32_t ex = wgExclusiveScan<NT>(v, tot, lds);
-      if (i < n)
-        a[i] = carry + ex;
+      uint32_t ex = carry + wgExclusiveScan<NT>(sum, tot, lds);
+#pragma unroll
+      for (uint32_t u = 0; u < kV; ++u) {
+        if (i0 + u < n)
+          a[i0 + u] = ex;
+        ex += v[u];
+      }
       carry += tot;
     }
     if (threadIdx.x == 0)
