@@ -272,15 +272,16 @@ namespace pxg {
       const uint32_t sub = laneId() % G;
       const uint64_t below = (1ull << sub) - 1;
       if (!kRank) {
-        // the cells' state for the later stages, cleared in cell order on the whole grid
-        // (coalesced; the per-event list build would do it on one CU)
-        for (uint32_t c = gridStart(); c < nCells; c += gridStride()) {
-          ca.cellFlags[cb + c] = 0;
-          ca.cellUsed[cb + c] = 0;
-          ca.nbSlot[cb + c] = -1;
-          ca.nbCount[cb + c] = 0;
-          ca.trSlot[cb + c] = -1;
-          ca.trCount[cb + c] = 0;
+        // the cells' state for connect and the fishbone, cleared in cell order on the
+        // whole grid, four cells per thread with one store per array (the per-event list
+        // build would do it on one CU); the cell stride of an event is a multiple of four,
+        // so the stores stay inside the event's arrays
+        static_assert(kCapsReference.maxDoublets % 4 == 0 && kCapsHighOccupancy.maxDoublets % 4 == 0, "cell stride");
+        for (uint32_t c = 4 * gridStart(); c < nCells; c += 4 * gridStride()) {
+          *reinterpret_cast<uint32_t*>(ca.cellFlags + cb + c) = 0;
+          *reinterpret_cast<uint32_t*>(ca.cellUsed + cb + c) = 0;
+          *reinterpret_cast<uint32_t*>(ca.nbCount + cb + c) = 0;
+          *reinterpret_cast<int4*>(ca.nbSlot + cb + c) = make_int4(-1, -1, -1, -1);
         }
       }
       for (uint32_t j = groupSlotStart<G>(); j < ntot; j += groupSlotStride<G>()) {
@@ -1163,6 +1164,12 @@ namespace pxg {
           atomicAdd(&cnt[c.w - c0], 1u);
       }
       __syncthreads();
+      // the cells of the slice that are in no tuple: no list (coalesced)
+      for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
+        if (cnt[i] == 0) {
+          ca.trSlot[cb + c0 + i] = -1;
+          ca.trCount[cb + c0 + i] = 0;
+        }
       // slots: each thread takes a run of consecutive cells, the runs in order
       const uint32_t per = (n + kEventBlock - 1) / kEventBlock;
       const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
