@@ -191,26 +191,18 @@ namespace pxg {
       return d.moduleId;
     }
 
-    __device__ __forceinline__ uint32_t eventOfBlock(const EventDesc* ev, uint32_t nEvents, uint32_t b) {
-      uint32_t lo = 0, hi = nEvents - 1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (ev[mid].blockOffset <= b)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      return lo;
-    }
-
   }  // namespace
 
-  __global__ void __launch_bounds__(kDecodeBlock) decodeKernel(DevConditions c, Pool p, uint32_t nEvents, uint32_t blockBase) {
-    const uint32_t b = blockIdx.x + blockBase;
-    const uint32_t e = eventOfBlock(p.events, nEvents, b);
+  // a (blocks per event, events) grid, an event's blocks on one XCD: the event of a
+  // workgroup is its grid row, with no search over the events' block offsets
+  __global__ void __launch_bounds__(kDecodeBlock) decodeKernel(DevConditions c, Pool p) {
+    const LogicalBlock lb = logicalBlock();
+    const uint32_t e = lb.y;
     const uint32_t wordOffset = p.events[e].wordOffset;
     const uint32_t nWords = p.events[e].nWords;
-    const uint32_t local = (b - p.events[e].blockOffset) * kDecodeBlock + threadIdx.x;
+    const uint32_t local = lb.x * kDecodeBlock + threadIdx.x;
+    if (lb.x * kDecodeBlock >= nWords)
+      return;  // uniform per workgroup: the event is shorter than the longest of the launch
     const uint32_t g = wordOffset + local;
     const uint32_t lane = threadIdx.x & 63;
     const bool active = local < nWords;
@@ -363,10 +355,10 @@ namespace pxg {
     }
   }
 
-  void launchDecode(const DevConditions& c, const Pool& p, uint32_t nEvents, uint32_t nBlocks, uint32_t blockBase,
-                    hipStream_t s) {
-    if (nBlocks)
-      decodeKernel<<<nBlocks, kDecodeBlock, 0, s>>>(c, p, nEvents, blockBase);
+  void launchDecode(const DevConditions& c, const Pool& p, uint32_t nEvents, uint32_t maxWords, hipStream_t s) {
+    const uint32_t blocks = (maxWords + kDecodeBlock - 1) / kDecodeBlock;
+    if (blocks && nEvents)
+      decodeKernel<<<dim3(blocks, nEvents), kDecodeBlock, 0, s>>>(c, p);
   }
 
   void launchModuleStarts(const Pool& p, uint32_t nEvents, hipStream_t s) {

@@ -318,12 +318,12 @@ namespace pxg::detail {
     markStages(s, first, count, from, upto);
     const Pool v = viewFrom(s->pool, first);
     const DevConditions& c = s->ctx->cond;
-    const uint32_t blockBase = s->hostEvents[first].blockOffset;
-    const EventDesc& last = s->hostEvents[first + count - 1];
-    const uint32_t nBlocks = last.blockOffset + (last.nWords + kDecodeBlock - 1) / kDecodeBlock - blockBase;
+    uint32_t maxWords = 0;  // the longest event of the launch sets the decode grid's width
+    for (uint32_t e = first; e < first + count; ++e)
+      maxWords = std::max(maxWords, s->hostEvents[e].nWords);
     int rc = 0;
     if (from <= 1 && upto >= 1) {
-      if ((rc = launchTimed(s, "decode", [&] { launchDecode(c, v, count, nBlocks, blockBase, s->stream); })))
+      if ((rc = launchTimed(s, "decode", [&] { launchDecode(c, v, count, maxWords, s->stream); })))
         return rc;
       if ((rc = launchTimed(s, "moduleStarts", [&] { launchModuleStarts(v, count, s->stream); })))
         return rc;

@@ -289,7 +289,7 @@ namespace pxg {
   void launchVertices(const CALaunch& l);
 
   // kernel launchers (stage A, B); nBlocks = total decode workgroups of the batch
-  void launchDecode(const DevConditions& c, const Pool& p, uint32_t nEvents, uint32_t nBlocks, uint32_t blockBase,
+  void launchDecode(const DevConditions& c, const Pool& p, uint32_t nEvents, uint32_t maxWords,
                     hipStream_t s);
   void launchModuleStarts(const Pool& p, uint32_t nEvents, hipStream_t s);
   void launchClusterize(const Pool& p, uint32_t nEvents, hipStream_t s);

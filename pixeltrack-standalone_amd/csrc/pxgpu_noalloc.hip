@@ -122,8 +122,7 @@ extern "C" int pxg_raw_to_cluster_noalloc(pxg_context* ctx, void* hipStream, con
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return fail(PXG_ERR_HIP, "launch of the event descriptor failed: %s", hipGetErrorString(e));
-  const uint32_t nBlocks = (nWords + kDecodeBlock - 1) / kDecodeBlock;
-  launchDecode(ctx->cond, p, 1, nBlocks, 0, s);
+  launchDecode(ctx->cond, p, 1, nWords, s);
   launchModuleStarts(p, 1, s);
   launchClusterize(p, 1, s);
   e = hipGetLastError();
