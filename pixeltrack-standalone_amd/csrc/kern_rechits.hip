@@ -97,8 +97,10 @@ namespace pxg {
     }
   }  // namespace
 
+  // eight waves per SIMD (64 registers, no spills): the kernel waits on loads far more
+  // than it computes (0.167 → 0.155 ms per 167-event launch against five)
   template <int G>
-  __global__ void __launch_bounds__(kBlock) hitsKernel(DevConditions c, Pool p) {
+  __global__ void __launch_bounds__(kBlock, 8) hitsKernel(DevConditions c, Pool p) {
     constexpr int kGroup = G;              // lanes per module run
     constexpr int kRuns = kBlock / kGroup;  // module runs per workgroup
     constexpr int kChunk = kGroup;          // clusters per pass of a run
