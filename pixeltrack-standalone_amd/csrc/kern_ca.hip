@@ -1123,7 +1123,7 @@ namespace pxg {
     // event also finalises the tuple offsets (HistoContainer.h:193-204 bulkFinalizeFill).
     constexpr uint32_t kTrSlice = 16384;  // cells per workgroup: 64 KB of counts
 
-    __global__ void __launch_bounds__(kEventBlock) trackListBuildKernel(CAPool ca) {
+    __global__ void __launch_bounds__(kEventBlock) trackListBuildKernel(Pool p, CAPool ca) {
       const uint32_t s = blockIdx.x, e = blockIdx.y;
       __shared__ uint32_t cnt[kTrSlice];
       __shared__ uint32_t lds[kEventBlock / 64];
@@ -1140,6 +1140,24 @@ namespace pxg {
         } else {
           for (uint32_t i = cn.apcM + threadIdx.x; i <= M; i += kEventBlock)
             off[i] = cn.apcN;
+        }
+      }
+      // a17 kernel_fillHitDetIndices, and the fit outputs of every stored tuple cleared
+      // (the fits write those of the tuples they fit), spread over the event's workgroups
+      {
+        const uint16_t* detInd = p.detInd + size_t(e) * kMaxNumClusters;
+        const uint32_t nEntries = cn.apcM >= M ? ca.tupleOff[ca.offBase(e) + M - 1] : cn.apcN;
+        const uint32_t stride = gridDim.x * kEventBlock;
+        for (uint32_t i = s * kEventBlock + threadIdx.x; i < nEntries; i += stride)
+          ca.detIndices[5 * tb + i] = detInd[ca.tupleBins[5 * tb + i]];
+        for (uint32_t t = s * kEventBlock + threadIdx.x; t < cn.nTuples; t += stride) {
+          for (int k = 0; k < 5; ++k)
+            ca.state[5 * tb + size_t(k) * M + t] = 0;
+          for (int k = 0; k < 15; ++k)
+            ca.cov[15 * tb + size_t(k) * M + t] = 0;
+          ca.pt[tb + t] = 0;
+          ca.eta[tb + t] = 0;
+          ca.chi2[tb + t] = 0;
         }
       }
       const uint32_t c0 = s * kTrSlice;
@@ -1285,7 +1303,6 @@ namespace pxg {
     // ---- a16 tuple multiplicity (kernel_countMultiplicity / fillMultiplicity) ----------
     // One workgroup per event.  Within a size the serial fill leaves tuples in
     // descending index order; each tuple's place is computed from its rank.
-    // Also writes the detector indices (a17) and clears the fit outputs.
     __global__ void __launch_bounds__(kEventBlock) multiplicityKernel(Pool p, CAPool ca) {
       const uint32_t e = blockIdx.x;
       __shared__ uint32_t binCount[8], binEnd[8], carry[8];
@@ -1355,21 +1372,6 @@ namespace pxg {
           carry[threadIdx.x] += s;
         }
         __syncthreads();
-      }
-      // a17 kernel_fillHitDetIndices
-      const uint16_t* detInd = p.detInd + size_t(e) * kMaxNumClusters;
-      const uint32_t nEntries = off[ca.caps.maxTuples];
-      for (uint32_t i = threadIdx.x; i < nEntries; i += kEventBlock)
-        ca.detIndices[5 * tb + i] = detInd[ca.tupleBins[5 * tb + i]];
-      const uint32_t M = ca.caps.maxTuples;
-      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
-        for (int k = 0; k < 5; ++k)
-          ca.state[5 * tb + size_t(k) * M + t] = 0;
-        for (int k = 0; k < 15; ++k)
-          ca.cov[15 * tb + size_t(k) * M + t] = 0;
-        ca.pt[tb + t] = 0;
-        ca.eta[tb + t] = 0;
-        ca.chi2[tb + t] = 0;
       }
     }
 
@@ -1450,7 +1452,7 @@ namespace pxg {
     PXG_K("ntupletScan", ntupletScanKernel, l.nEvents, kEventBlock, *l.ca);
     PXG_K("ntupletFill", ntupletFillKernel, g, kCaBlock, *l.ca);
     PXG_K("trackListFill", trackListBuildKernel, dim3((l.ca->caps.maxDoublets + kTrSlice - 1) / kTrSlice, l.nEvents),
-          kEventBlock, *l.ca);
+          kEventBlock, *l.p, *l.ca);
     PXG_K("trackListSort", trackListSortKernel, g, kCaBlock, *l.ca);
   }
 
