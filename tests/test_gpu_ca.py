@@ -158,6 +158,32 @@ def test_high_occupancy_profile(detector, conditions, gpu_ctx):
         compare_ca(o, _gpu_event(gpu_ctx, s, raw), f"pu200[{idx}]/high-occupancy")
 
 
+def test_high_occupancy_batch(detector, conditions, gpu_ctx):
+    """A batch of 8 events (two PU200, six PU50) under the high-occupancy capacities, as one
+    launch per stage: the per-event LDS builds of the outer-hit lists (hits past 32768, so
+    both 16-bit halves of the counters), the track lists (cells in many slices) and the
+    hit -> tuple map (hits in several slices), and module runs of more than 256 digis in the
+    clustering and rec-hit kernels, each event against the oracle."""
+    import pyoracle
+    from pxtrack import gpu
+    orc = pyoracle.Oracle(conditions, profile=1)
+    evs = [event(detector, "pu200", i)[0] for i in (1, 4)] + [event(detector, "pu50", i)[0] for i in range(6)]
+    unpacked = [gpu.unpack_raw(r, gpu_ctx.fed_ids) for r in evs]
+    s = gpu_ctx.stream(max_events=len(evs), max_words=sum(w.size for w, _ in unpacked) + 256 * len(evs),
+                       profile=gpu.PROFILE_HIGH_OCCUPANCY)
+    s.load(unpacked)
+    s.run(0, len(evs), upto=5)
+    s.fetch_counts(0, len(evs))
+    for e, raw in enumerate(evs):
+        print(f"high-occupancy batch event {e}", flush=True)
+        o = orc.process(raw, upto=5)
+        assert o["caStatus"] == 0
+        g = s.event_products(e, upto=5)
+        g.update(s.ca_internals(e, int(g["nHits"])))
+        compare_ca(o, g, f"high-occupancy batch[{e}]")
+    assert max(int(orc.process(r, upto=3)["nHits"]) for r in evs[:2]) > 32768
+
+
 def test_reference_capacities_overflow(detector, oracle, gpu_ctx):
     """PU200 under the reference's capacities (CAConstants.h): more doublets than
     maxNumberOfDoublets, more tuples than maxNumberOfTuples, a full multiplicity
