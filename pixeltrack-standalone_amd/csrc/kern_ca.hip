@@ -952,7 +952,9 @@ namespace pxg {
         bool start = false;
         if (c < nCells) {
           const int pid = ca.cellPair[cb + c];
-          start = !(ca.cellFlags[cb + c] & 2) && (pid < 8 || pid > 12) && ca.nbSlot[cb + c] >= 0;
+          // a cell has a neighbour list exactly when its count is positive (one byte
+          // read instead of the four of its slot)
+          start = !(ca.cellFlags[cb + c] & 2) && (pid < 8 || pid > 12) && ca.nbCount[cb + c] > 0;
         }
         const uint64_t m = __ballot(start);
         if ((threadIdx.x & 63) == 0)
