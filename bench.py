@@ -71,7 +71,8 @@ LABEL_ROW = {lab: row for row, (labs, _) in ROWS.items() for lab in labs}
 # device kernel symbol -> profiler label (several kernels make up "clusterize")
 KERNEL_LABEL = {
     "decodeKernel": "decode", "moduleStartKernel": "moduleStarts", "clusterSmallKernel": "clusterize",
-    "clusterBigKernel": "clusterize", "clusStartKernel": "clusterize", "hitsKernel": "hits",
+    "clusterMediumKernel": "clusterize", "clusterBigKernel": "clusterize", "clusStartKernel": "clusterize",
+    "hitsKernel": "hits",
     "phiBinnerKernel": "phiBinner", "doubletCountKernel": "doubletCount", "doubletScanKernel": "doubletScan",
     "doubletFillKernel": "doubletFill", "ohScanKernel": "outerHitScan", "ohFillKernel": "outerHitFill",
     "ohBuildKernel": "outerHitFill",
