@@ -480,7 +480,9 @@ def main():
             mean[key] = float(np.mean([c[name] for c in cac]))
         mean["Nbar"] = mean["TH"] / mean["T"] if mean["T"] else 0.0
 
-    # roofline: the kernel with the largest share of the profiled timed steps, and connect
+    # roofline: of the kernels that make up a scope row on their own (so that the row's
+    # algorithmic bytes are that kernel's), the one with the largest share of the profiled
+    # timed steps; the track-list build shares its row with the cleaners and is not a candidate
     steps_events = n * args.steps
     single = [lab for lab in prof if lab in LABEL_ROW and len(ROWS[LABEL_ROW[lab]][0]) == 1]
     total_prof = sum(v[1] for v in prof.values())
