@@ -1114,16 +1114,24 @@ namespace pxg {
     }
 
     // Per-cell track lists (the tuples each cell is in; CAHitNtupletGeneratorKernelsImpl.h:
-    // 118-149 fills them during the walk), built per event in LDS: workgroup (s, e) owns the
-    // cells [s * kTrSlice, (s + 1) * kTrSlice) of event e and passes twice over the event's
-    // tuples (about 16 bytes each, 10k at PU50): counting each of its cells' tuples with LDS
-    // atomics, then, once every cell with tuples has a slot of the list store (one counter
-    // update per workgroup, slots in cell order within it), writing each tuple into its
-    // cells' lists at the rank an LDS atomic gives (the sort below orders each list).
+    // 118-149 fills them during the walk), built per event in LDS: workgroup (s, e) takes the
+    // slices of kTrSlice cells s, s + gridDim.x, ... of event e and passes twice over the
+    // event's tuples per slice (about 16 bytes each, 10k at PU50): counting each of its
+    // cells' tuples with LDS atomics, then, once every cell with tuples has a slot of the
+    // list store (one counter update per workgroup, slots in cell order within it),
+    // writing each tuple into its cells' lists at the rank an LDS atomic gives (the sort
+    // below orders each list).  Between the passes a cell's LDS word becomes its list's
+    // rank in the workgroup (high 14 bits) plus its tuples placed so far (low 18 bits; a
+    // cell has fewer than 2^18 tuples), so the second pass reads no slot from memory.
     // Slot s records its cell and the cell's tuple count (the walk's scratch is free now),
     // so the later passes visit only the cells that are in tuples.  Workgroup 0 of each
     // event also finalises the tuple offsets (HistoContainer.h:193-204 bulkFinalizeFill).
-    constexpr uint32_t kTrSlice = 16384;  // cells per workgroup: 64 KB of counts
+    constexpr uint32_t kTrSlice = 16384;       // cells per slice: 64 KB of counts
+    constexpr uint32_t kTrGroupCells = 40960;  // capacity cells per workgroup: 12 per event at
+                                               // 458752 cells, one slice each up to 196608 cells
+    static_assert(kTrSlice <= (1u << 14) && kCapsReference.maxTuples < (1u << 18) &&
+                      kCapsHighOccupancy.maxTuples < (1u << 18),
+                  "rank and position share a 32-bit LDS word");
 
     __global__ void __launch_bounds__(kEventBlock) trackListBuildKernel(Pool p, CAPool ca) {
       const uint32_t s = blockIdx.x, e = blockIdx.y;
@@ -1162,82 +1170,78 @@ namespace pxg {
           ca.chi2[tb + t] = 0;
         }
       }
-      const uint32_t c0 = s * kTrSlice;
-      if (c0 >= cn.nCells)
-        return;  // uniform per workgroup
-      const uint32_t n = min(kTrSlice, cn.nCells - c0);
       const uint32_t nT = cn.nTuples;
       const uint4* cells4 = reinterpret_cast<const uint4*>(ca.tupleCells + 4 * tb);
-      for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
-        cnt[i] = 0;
-      __syncthreads();
-      auto inSlice = [&](uint32_t c) { return c != 0xffffffffu && c - c0 < n; };
-      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
-        const uint4 c = cells4[t];
-        if (inSlice(c.x))
-          atomicAdd(&cnt[c.x - c0], 1u);
-        if (inSlice(c.y))
-          atomicAdd(&cnt[c.y - c0], 1u);
-        if (inSlice(c.z))
-          atomicAdd(&cnt[c.z - c0], 1u);
-        if (inSlice(c.w))
-          atomicAdd(&cnt[c.w - c0], 1u);
-      }
-      __syncthreads();
-      // the cells of the slice that are in no tuple: no list (coalesced)
-      for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
-        if (cnt[i] == 0) {
-          ca.trSlot[cb + c0 + i] = -1;
-          ca.trCount[cb + c0 + i] = 0;
-        }
-      // slots: each thread takes a run of consecutive cells, the runs in order
-      const uint32_t per = (n + kEventBlock - 1) / kEventBlock;
-      const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
-      uint32_t mine = 0;
-      for (uint32_t i = i0; i < i1; ++i)
-        mine += cnt[i] > 0;
-      uint32_t total;
-      uint32_t rank = wgExclusiveScan<kEventBlock>(mine, total, lds);
-      if (threadIdx.x == 0)
-        sBase = total ? atomicAdd(&ca.cnt[e].trLists, total) : 0;
-      __syncthreads();
       uint32_t* trCells = ca.dfsT + cb;
       uint32_t* trTotal = ca.dfsH + cb;
-      uint32_t status = 0;
-      for (uint32_t i = i0; i < i1; ++i) {
-        const uint32_t k = cnt[i];
-        if (k == 0)
-          continue;
-        const uint32_t slot = sBase + rank++;
-        const uint32_t c = c0 + i;
-        if (slot + 1 < ca.caps.maxCellLists) {
-          ca.trSlot[cb + c] = int32_t(slot);
-          trCells[slot] = c;
-          trTotal[slot] = k;
-        } else {
-          ca.trSlot[cb + c] = -2;
-          status |= kCaTrackStoreOverflow;
-        }
-        cnt[i] = 0;  // the counts become ranks
-      }
-      __syncthreads();
       TupleIndex* pool = ca.trPool + size_t(e) * ca.caps.maxCellLists * kCaMaxCellTracks;
-      auto place = [&](uint32_t c, uint32_t t) {
-        if (!inSlice(c))
-          return;
-        const int32_t slot = ca.trSlot[cb + c];
-        if (slot < 0)
-          return;  // no list: the store is full
-        const uint32_t pos = atomicAdd(&cnt[c - c0], 1u);
-        if (pos < kCaMaxCellTracks)
-          pool[size_t(slot) * kCaMaxCellTracks + pos] = t;
-      };
-      for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
-        const uint4 c = cells4[t];
-        place(c.x, t);
-        place(c.y, t);
-        place(c.z, t);
-        place(c.w, t);
+      uint32_t status = 0;
+      for (uint32_t c0 = s * kTrSlice; c0 < cn.nCells; c0 += gridDim.x * kTrSlice) {
+        const uint32_t n = min(kTrSlice, cn.nCells - c0);
+        __syncthreads();  // the previous slice's ranks are no longer read
+        for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
+          cnt[i] = 0;
+        __syncthreads();
+        auto inSlice = [&](uint32_t c) { return c != 0xffffffffu && c - c0 < n; };
+        for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+          const uint4 c = cells4[t];
+          if (inSlice(c.x))
+            atomicAdd(&cnt[c.x - c0], 1u);
+          if (inSlice(c.y))
+            atomicAdd(&cnt[c.y - c0], 1u);
+          if (inSlice(c.z))
+            atomicAdd(&cnt[c.z - c0], 1u);
+          if (inSlice(c.w))
+            atomicAdd(&cnt[c.w - c0], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kEventBlock)
+          if (cnt[i] == 0) {
+            ca.trSlot[cb + c0 + i] = -1;
+            ca.trCount[cb + c0 + i] = 0;
+          }
+        const uint32_t per = (n + kEventBlock - 1) / kEventBlock;
+        const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
+        uint32_t mine = 0;
+        for (uint32_t i = i0; i < i1; ++i)
+          mine += cnt[i] > 0;
+        uint32_t total;
+        uint32_t rank = wgExclusiveScan<kEventBlock>(mine, total, lds);
+        if (threadIdx.x == 0)
+          sBase = total ? atomicAdd(&ca.cnt[e].trLists, total) : 0;
+        __syncthreads();
+        for (uint32_t i = i0; i < i1; ++i) {
+          const uint32_t k = cnt[i];
+          if (k == 0)
+            continue;
+          const uint32_t slot = sBase + rank++;
+          const uint32_t c = c0 + i;
+          if (slot + 1 < ca.caps.maxCellLists) {
+            ca.trSlot[cb + c] = int32_t(slot);
+            trCells[slot] = c;
+            trTotal[slot] = k;
+          } else {
+            ca.trSlot[cb + c] = -2;
+            status |= kCaTrackStoreOverflow;
+          }
+          cnt[i] = (rank - 1) << 18;  // the list's rank in the workgroup above, tuples placed below
+        }
+        __syncthreads();
+        auto place = [&](uint32_t c, uint32_t t) {
+          if (!inSlice(c))
+            return;
+          const uint32_t v = atomicAdd(&cnt[c - c0], 1u);
+          const uint32_t slot = sBase + (v >> 18), pos = v & 0x3ffffu;
+          if (slot + 1 < ca.caps.maxCellLists && pos < kCaMaxCellTracks)
+            pool[size_t(slot) * kCaMaxCellTracks + pos] = t;
+        };
+        for (uint32_t t = threadIdx.x; t < nT; t += kEventBlock) {
+          const uint4 c = cells4[t];
+          place(c.x, t);
+          place(c.y, t);
+          place(c.z, t);
+          place(c.w, t);
+        }
       }
       if (status)
         atomicOr(&ca.cnt[e].status, status);
@@ -1453,7 +1457,7 @@ namespace pxg {
     PXG_K("ntupletCount", ntupletCountKernel, g, kCaBlock, *l.ca);
     PXG_K("ntupletScan", ntupletScanKernel, l.nEvents, kEventBlock, *l.ca);
     PXG_K("ntupletFill", ntupletFillKernel, g, kCaBlock, *l.ca);
-    PXG_K("trackListFill", trackListBuildKernel, dim3((l.ca->caps.maxDoublets + kTrSlice - 1) / kTrSlice, l.nEvents),
+    PXG_K("trackListFill", trackListBuildKernel, dim3((l.ca->caps.maxDoublets + kTrGroupCells - 1) / kTrGroupCells, l.nEvents),
           kEventBlock, *l.p, *l.ca);
     PXG_K("trackListSort", trackListSortKernel, g, kCaBlock, *l.ca);
   }
